@@ -1,0 +1,182 @@
+#!/usr/bin/env python3
+"""Sampled tokens/s per Gibbs sweep at K=1024 on the ~100M-token synthetic
+corpus (V=102,660, D=300,000, log-normal doc lengths mean 330 / sigma 0.8,
+Zipf(1.07) vocabulary, dense WT head = top 1% of words), on one MI355X.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W]
+
+A step is one whole sweep: WT/n_k rebuild, top-3, DT rebuild, sampling.  The
+corpus and model are resident in HBM before the timed region; the K timed
+sweeps run back to back on the library's stream with HIP events around every
+stage (no host synchronisation inside the region).  Prints one JSON line.
+"""
+from __future__ import annotations
+
+import argparse
+import ctypes
+import json
+import math
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+
+WORKLOADS = {
+    # north-star corpus: ~100M tokens, K=1024
+    "cfg2": dict(gen=dict(V=102660, D=300000, K_true=1024, len_dist=1, len_mean=330.0, len_sigma=0.8,
+                          zipf_s=1.07, phi_conc=0.01, theta_conc=0.1, seed=1002),
+                 K=1024, seed=2002, head_frac=0.01,
+                 desc="V=102660, D=300000, lognormal(mean 330, sigma 0.8), Zipf(1.07) base, ~99M tokens, "
+                      "K=1024, alpha=50/K, beta=0.01, dense head = top 1% of words"),
+    # the reference's CPU-sized case
+    "cfg1": dict(gen=dict(V=10000, D=10000, K_true=64, len_dist=0, len_mean=100.0, len_sigma=0.0,
+                          zipf_s=0.0, phi_conc=0.01, theta_conc=0.1, seed=1001),
+                 K=64, seed=2001, head_frac=None,
+                 desc="V=10000, D=10000, Poisson(100), ~1M tokens, K=64, alpha=50/K, beta=0.01"),
+}
+
+
+def log(msg: str) -> None:
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def device_sync() -> None:
+    try:
+        ctypes.CDLL("libamdhip64.so").hipDeviceSynchronize()
+    except OSError:
+        pass
+
+
+def cpu_baseline(corpus, K: int, seed: int, max_docs: int, sweeps: int = 2):
+    """The OpenMP build of the CPU oracle (same sweep semantics, bit-exact)
+    on the first max_docs docs of the same corpus; tokens/s over `sweeps`
+    timed sweeps after one warm-up sweep."""
+    from oracle import binding as ob
+
+    D = min(max_docs, corpus.D)
+    off = corpus.doc_off[: D + 1].copy()
+    words = corpus.words[: int(off[-1])].copy()
+    orc = ob.Oracle(K, float(np.float32(50.0 / K)), 0.01, seed, corpus.V, off, words, omp=True)
+    orc.sweep(1)
+    t = time.perf_counter()
+    orc.sweep(sweeps)
+    dt = time.perf_counter() - t
+    cores = int(ob.lib(True).orc_num_threads())
+    n = int(off[-1])
+    orc.close()
+    return {"value": n * sweeps / dt, "unit": "tokens/s", "cores": cores, "kind": "port",
+            "sample": f"first {D} docs ({n} tokens) of the same corpus, full vocabulary, K={K}, "
+                      f"{sweeps} sweeps after 1 warm-up sweep, OpenMP oracle ({cores} threads)"}
+
+
+def main() -> int:
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=10)
+    ap.add_argument("--workload", default="cfg2", choices=sorted(WORKLOADS))
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--cpu-docs", type=int, default=30000)
+    ap.add_argument("--no-graph", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU: the Gibbs sweep here is single-GPU (--gpus {args.gpus} is not "
+              f"supported; see DESIGN.md for how the path would shard)", file=sys.stderr)
+        return 2
+
+    from lda_sc20_amd import ezlda
+
+    wl = WORKLOADS[args.workload]
+    t = time.perf_counter()
+    corpus = ezlda.gen_corpus(**wl["gen"])
+    log(f"generated {corpus.N} tokens in {time.perf_counter() - t:.1f}s")
+    K = wl["K"]
+    dense = -1 if wl["head_frac"] is None else int(math.ceil(wl["head_frac"] * corpus.V))
+    ctx = ezlda.Context(K=K, alpha=50.0 / K, beta=0.01, seed=wl["seed"], dense_words=dense, shortcut=1)
+    t = time.perf_counter()
+    ctx.load(corpus)
+    info = ctx.info()
+    log(f"loaded in {time.perf_counter() - t:.1f}s: N={info.num_tokens} V={info.num_words} D={info.num_docs} "
+        f"V_D={info.num_head_words} items={info.num_sample_items} device={info.device_bytes / 2**30:.2f} GiB")
+
+    t = time.perf_counter()
+    ctx.sweep(args.warmup)
+    log(f"{args.warmup} warm-up sweeps in {time.perf_counter() - t:.2f}s")
+
+    device_sync()
+    ctx.reset_stats()
+    t0 = time.perf_counter()
+    total_ms, stage_ms = ctx.time_sweeps(args.steps, use_graph=False)
+    device_sync()
+    wall = time.perf_counter() - t0
+    n_full, n_scanned, tail_nnz, dt_nnz = ctx.stats()
+    K_steps = args.steps
+    N = int(info.num_tokens)
+    ms_per_step = total_ms / K_steps
+    value = N * K_steps / (total_ms / 1e3)
+    log(f"timed {K_steps} sweeps: {total_ms:.2f} ms (wall {wall * 1e3:.1f} ms), stages {stage_ms}")
+
+    # algorithmic bytes of one sampler launch (per-token streams + gathers,
+    # phi staging once per word, one read of every scanned DT entry)
+    VD = int(info.num_head_words)
+    s_scanned = n_scanned / K_steps
+    b_sample = 18 * N + 4 * K * VD + 4 * tail_nnz + 4 * K + 4 * s_scanned
+    t_sample = stage_ms["sample"] / K_steps / 1e3
+    achieved = b_sample / t_sample / 1e9
+    # whole-sweep algorithmic bytes (same accounting for every stage)
+    b_sweep = (18 * N + 4 * K * VD + 4 * tail_nnz + 4 * K + 4 * s_scanned  # sample
+               + 2 * N + 2 * N + 4 * K * VD + 4 * tail_nnz                  # n_k + WT rebuild
+               + 4 * K * VD + 4 * tail_nnz + 8 * info.num_words             # top-3
+               + 8 * N + 2 * N + 4 * N + 4 * dt_nnz + 8 * info.num_docs)    # DT + aux
+    graph_ms = None
+    if not args.no_graph:
+        gms, _ = ctx.time_sweeps(K_steps, use_graph=True)
+        graph_ms = gms / K_steps
+        log(f"graph replay: {graph_ms:.3f} ms/sweep")
+    ll = ctx.loglik()
+    out = {
+        "metric": "sampled tokens/s per sweep at K=1024 (+% HBM peak)" if K == 1024 else
+                  f"sampled tokens/s per sweep at K={K}",
+        "value": value,
+        "unit": "tokens/s",
+        "n_gpus": 1,
+        "steps": K_steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_per_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic (LDA generative process, Philox-seeded)",
+        "config": {"workload": f"{args.workload}: {wl['desc']}", "num_tokens": N, "K": K,
+                   "num_head_words": VD, "shortcut": True, "sweeps_before_timing": args.warmup},
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": None, "kernel": "k_sample",
+                     "bytes_per_launch": b_sample, "avg_launch_ms": t_sample * 1e3},
+        "stage_ms_per_sweep": {k: v / K_steps for k, v in stage_ms.items()},
+        "sweep_algorithmic_GBs": b_sweep / (ms_per_step / 1e3) / 1e9,
+        "full_draw_fraction": n_full / (N * K_steps),
+        "dt_entries_per_full_draw": (n_scanned / n_full) if n_full else 0.0,
+        "graph_ms_per_step": graph_ms,
+        "perplexity_after": math.exp(-ll / N),
+        "wall_ms_timed_region": wall * 1e3,
+    }
+    ctx.close()
+    if not args.no_cpu_baseline:
+        t = time.perf_counter()
+        out["cpu_baseline"] = cpu_baseline(corpus, K, wl["seed"], args.cpu_docs)
+        log(f"cpu baseline in {time.perf_counter() - t:.1f}s: {out['cpu_baseline']['value']:.3e} tokens/s")
+    else:
+        out["cpu_baseline"] = None
+    print(json.dumps(out), flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

@@ -1,0 +1,219 @@
+/*
+ * ezlda.h -- C ABI of the MI355X-native EZLDA Gibbs-sampling sweep.
+ *
+ * Plain C: no HIP, CUDA or torch types.  Every function returns an int
+ * status (EZLDA_OK = 0) unless stated; the message of the last failure is
+ * ezlda_last_error(ctx) (ctx may be NULL for failures before a context
+ * exists).  A context owns all of its device memory; arrays passed in or out
+ * are host arrays owned by the caller.  One context per host thread; calls
+ * on one context must not overlap.
+ *
+ * The reference exposes its hot path as ten host launchers that take its
+ * container classes and a cudaStream_t& (EZLDA/SamplingKernel.cuh:10-15,
+ * DTUpdateKernel.cuh:13, WTUpdateKernel.cuh:23, WTDenUpdateKernel.cuh:25-26,
+ * WTAddKernel.cuh:24).  Each stage entry point below names the launchers it
+ * replaces.  The corpus stays resident on the device, so the reference's
+ * per-chunk staging methods (Doc.cu:266-319, WT.cu:212-239, DT.cu:135-142)
+ * have no counterpart here.
+ */
+#ifndef EZLDA_H
+#define EZLDA_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define EZLDA_OK 0
+#define EZLDA_EINVAL 1  /* bad argument or corpus */
+#define EZLDA_EHIP 2    /* HIP runtime error */
+#define EZLDA_EIO 3     /* file missing, truncated or inconsistent */
+#define EZLDA_ESTATE 4  /* call out of order (e.g. no corpus loaded) */
+#define EZLDA_ENOMEM 5  /* over the context's memory budget */
+
+typedef struct ezlda_ctx ezlda_ctx;
+
+/* Run-time form of the reference's compile-time Argument surface
+ * (EZLDA/Argument.cuh:3-9) plus the constants main() hard-codes
+ * (mainKernel.cu:50-51) and Doc.cuh:30 tokenSegment. */
+typedef struct ezlda_args {
+    int32_t K;               /* topics (Argument.cuh:6); 1 <= K <= 65535 */
+    float alpha;             /* Argument.cuh:9; <= 0 means 50/K */
+    float beta;              /* Argument.cuh:8; <= 0 means 0.01 */
+    int32_t block_dim;       /* Argument.cuh:4 BlockDim: threads per workgroup
+                                of the sweep kernels; multiple of 64, <= 1024 */
+    int32_t grid_dim;        /* Argument.cuh:5 GridDim: 0 = one workgroup per
+                                work item; > 0 = that many persistent
+                                workgroups pulling work items from a queue */
+    int32_t num_chunks;      /* mainKernel.cu:50; recorded, corpus is resident */
+    int32_t num_streams;     /* Argument.cuh:7; recorded */
+    int32_t token_segment;   /* Doc.cuh:30: max tokens of one word per sampler
+                                work item (0 = 4096) */
+    uint64_t seed;           /* Philox key */
+    int32_t dense_words;     /* head size V_D (words by rank); < 0 = use
+                                dense_threshold */
+    int32_t dense_threshold; /* count >= threshold makes a head word
+                                (preprocess/streamTestBalance.cpp:291 uses K);
+                                <= 0 = K */
+    int32_t shortcut;        /* 1 = max-topic shortcut, 0 = plain sampler;
+                                both give identical topics */
+    int32_t topic_bits;      /* width of a stored topic; only 16 */
+} ezlda_args;
+
+typedef struct ezlda_info {
+    int64_t num_tokens;      /* N */
+    int32_t num_words;       /* V */
+    int32_t num_docs;        /* D */
+    int32_t num_head_words;  /* V_D */
+    int32_t K;
+    int64_t dt_capacity;     /* sum_d min(n_d, K) */
+    int64_t wt_tail_capacity;/* sum_tail min(n_w, K) */
+    int32_t num_sample_items;/* sampler work items (word segments) */
+    uint32_t sweep;          /* sweeps done (the Philox sweep counter) */
+    int64_t device_bytes;    /* device memory held by the context */
+} ezlda_info;
+
+void ezlda_default_args(ezlda_args *args);
+const char *ezlda_last_error(const ezlda_ctx *ctx);
+
+/* Creates a context on the current HIP device.  NULL on failure. */
+ezlda_ctx *ezlda_create(const ezlda_args *args);
+void ezlda_destroy(ezlda_ctx *ctx);
+
+/* Loads a doc-major corpus: doc d holds words[doc_off[d] .. doc_off[d+1]),
+ * word ids in [0, V).  z0 (nullable) gives the initial topic of every token
+ * in the same order, 0-based; NULL draws it from Philox.  Replaces
+ * Document::loadDocument + GPUMemAllocate (Doc.cu:45-108, :179-246),
+ * DTChunk::loadDocDTLength/loadDTCountOffset (DT.cu:4-100), WTAll/WTD
+ * set-up (WT.cu:3-193, WTDense.cu:3-43). */
+int ezlda_load_docs(ezlda_ctx *ctx, int32_t V, int32_t D, const int64_t *doc_off,
+                    const int32_t *words, const uint16_t *z0);
+
+/* Loads the chunked text corpus written by preprocess/streamTestBalance.cpp
+ * (lengthVec.txt, TLLength.txt, docLength.txt, chunk<c>/TL.txt,
+ * word2DocMap.txt, wordCountOffset.txt, docCountOffset.txt ...); topics in
+ * TL.txt are 1-based.  Reference readers: DataChunk.cu:50-100, Doc.cu:45-108. */
+int ezlda_load_chunked(ezlda_ctx *ctx, const char *prefix, int32_t num_chunks);
+
+int ezlda_get_info(ezlda_ctx *ctx, ezlda_info *info);
+
+/* ---- stages (for parity tests and custom drivers; ezlda_sweep runs all) */
+
+/* WT (dense head rows + sorted sparse tail rows) and n_k from the current z.
+ * Replaces UpdateWTDenKernel (WTDenUpdateKernel.cuh:25), UpdateWTKernel
+ * (WTUpdateKernel.cuh:23), WTAdditionKernel (WTAddKernel.cuh:24) and
+ * UpdateWTDenRowSumKernel (WTDenUpdateKernel.cuh:26). */
+int ezlda_build_wt(ezlda_ctx *ctx);
+/* 1/(n_k + V beta) and every word's top-3 topics by phi.
+ * Replaces MaxTopicKernel (SamplingKernel.cuh:12). */
+int ezlda_topk(ezlda_ctx *ctx);
+/* Sorted DT rows and each token's (n_dk1, n_dk2).
+ * Replaces UpdateDTKernel (DTUpdateKernel.cuh:13). */
+int ezlda_build_dt(ezlda_ctx *ctx);
+/* Draws a new topic for every token and advances the sweep counter.
+ * Replaces UpdateProbKernelD (SamplingKernel.cuh:15), SampleKernelD (:10)
+ * and SampleKernel (:11). */
+int ezlda_sample(ezlda_ctx *ctx);
+/* FP64 log-likelihood sum_t log p(w_t | d_t) of the current z (rebuilds the
+ * counts it needs).  Replaces PerplexityKernel (SamplingKernel.cuh:13);
+ * perplexity = exp(-ll / N). */
+int ezlda_loglik(ezlda_ctx *ctx, double *ll);
+
+/* ---- whole sweeps (mainKernel.cu:158-380) */
+int ezlda_sweep(ezlda_ctx *ctx, int32_t num_sweeps);
+/* The same sweeps replayed from one captured hipGraph. */
+int ezlda_sweep_graph(ezlda_ctx *ctx, int32_t num_sweeps);
+/* Timed sweeps: total wall time (HIP events on the context's stream) and
+ * the summed time of each stage (EZLDA_STAGE_*), all in ms. */
+#define EZLDA_STAGE_WT 0
+#define EZLDA_STAGE_TOPK 1
+#define EZLDA_STAGE_DT 2
+#define EZLDA_STAGE_SAMPLE 3
+#define EZLDA_NUM_STAGES 4
+int ezlda_time_sweeps(ezlda_ctx *ctx, int32_t num_sweeps, int32_t use_graph,
+                      double *total_ms, double *stage_ms);
+/* Sampler counters summed since the last reset: stats[0] = tokens that took
+ * the full three-bucket draw (the rest took the max-topic shortcut),
+ * stats[1] = DT entries those draws read.  The reference counts its
+ * full-sampled tokens the same way (utility.cu:1570,2004).  Also, of the
+ * current counts: stats[2] = nonzeros of the sparse WT tail, stats[3] =
+ * nonzeros of DT.  stats must hold 4 values. */
+int ezlda_get_stats(ezlda_ctx *ctx, uint64_t *stats);
+int ezlda_reset_stats(ezlda_ctx *ctx);
+
+/* ---- state in / out (host arrays, caller-owned) */
+/* z of every token, in the doc-major order it was loaded in. */
+int ezlda_get_topics(ezlda_ctx *ctx, uint16_t *z);
+int ezlda_set_topics(ezlda_ctx *ctx, const uint16_t *z);
+int ezlda_set_sweep(ezlda_ctx *ctx, uint32_t sweep);
+int ezlda_get_nk(ezlda_ctx *ctx, int32_t *nk);
+/* Dense V x K word-topic counts, rows by input word id. */
+int ezlda_get_wt_dense(ezlda_ctx *ctx, int32_t *wt);
+/* DT rows as stored: entries[dt_offset(d) + i] = topic | count << 16 for
+ * i < nnz[d]; dt_offset(d) = sum_{e<d} min(n_e, K). */
+int ezlda_get_dt_raw(ezlda_ctx *ctx, uint32_t *entries, int32_t *nnz);
+/* Top-3 topics per word (3 x uint16 per input word id; 0xFFFF = none). */
+int ezlda_get_top(ezlda_ctx *ctx, uint16_t *top);
+/* Per-token n_dk1 | n_dk2 << 16, doc-major order. */
+int ezlda_get_aux(ezlda_ctx *ctx, uint32_t *aux);
+int ezlda_get_inv(ezlda_ctx *ctx, float *inv);
+
+/* ---- device self-tests (no reference counterpart; they pin the scan tree
+ * and the RNG the oracle restates to what the device computes) */
+/* out[64w + l] = inclusive canonical wave64 scan of in[64w .. 64w+63]. */
+int ezlda_selftest_scan64(const float *in, float *out, int32_t num_waves);
+/* out[4i..4i+3] = Philox4x32-10(ctr[4i..4i+3], key[2i..2i+1]). */
+int ezlda_selftest_philox(const uint32_t *ctr, const uint32_t *key, uint32_t *out, int32_t n);
+
+/* ---- checkpoint (the state is z plus the sweep counter) */
+int ezlda_save_checkpoint(ezlda_ctx *ctx, const char *path);
+int ezlda_load_checkpoint(ezlda_ctx *ctx, const char *path);
+
+/* ---- synthetic corpora (LDA generative process, Philox-keyed) */
+typedef struct ezlda_gen_params {
+    int32_t V, D, K_true;
+    int32_t len_dist;     /* 0 Poisson(len_mean); 1 log-normal(len_mean,
+                             len_sigma); 2 Pareto(pareto_shape, pareto_xm) */
+    double len_mean, len_sigma, pareto_shape, pareto_xm;
+    int32_t len_min, len_max;
+    double zipf_s;        /* base word distribution ~ rank^-s; 0 = uniform */
+    double phi_conc;      /* phi_k ~ Dir(phi_conc * V * base) */
+    double theta_conc;    /* theta_d ~ Dir(theta_conc) over K_true topics */
+    uint64_t seed;
+    int32_t threads;      /* host threads (0 = all) */
+} ezlda_gen_params;
+
+typedef struct ezlda_corpus ezlda_corpus;
+ezlda_corpus *ezlda_gen_corpus(const ezlda_gen_params *p);
+int64_t ezlda_corpus_num_tokens(const ezlda_corpus *c);
+int32_t ezlda_corpus_num_docs(const ezlda_corpus *c);
+int32_t ezlda_corpus_num_words(const ezlda_corpus *c);
+const int64_t *ezlda_corpus_doc_off(const ezlda_corpus *c);
+const int32_t *ezlda_corpus_words(const ezlda_corpus *c);
+/* Initial topics (0-based) when the source carried them, else NULL. */
+const uint16_t *ezlda_corpus_topics(const ezlda_corpus *c);
+void ezlda_corpus_free(ezlda_corpus *c);
+/* Copies caller arrays into a corpus handle (topics nullable). */
+ezlda_corpus *ezlda_corpus_from_arrays(int32_t V, int32_t D, const int64_t *doc_off, const int32_t *words,
+                                       const uint16_t *topics);
+/* UCI docword bag of words ("D W NNZ", then "doc word count"), the input
+ * preprocess/streamTestBalance.cpp:90-145 reads. */
+ezlda_corpus *ezlda_read_docword(const char *path);
+/* The chunked text folder of preprocess/streamTestBalance.cpp:356-547
+ * (Appendix format: lengthVec.txt, TLLength.txt, docLength.txt, DTLength.txt,
+ * WTLength.txt, TLSplit.txt, WTCountOffset.txt, chunk<c>/{TL, word2DocMap,
+ * doc2WordMap, wordCountOffset, docCountOffset, DTCountOffset,
+ * WTCountOffset}.txt).  Topics are written 1-based.  z0 nullable (Philox
+ * draws keyed by seed).  Returns a status. */
+int ezlda_write_chunked(const char *prefix, const ezlda_corpus *c, const uint16_t *z0, int32_t K,
+                        int32_t num_chunks, uint64_t seed);
+/* Reads that folder (DataChunk.cu:50-100, Doc.cu:45-108): docs of chunk 0
+ * first; word ids are the file's ranks; topics 0-based.  num_chunks <= 0
+ * reads the count from TLLength.txt.  NULL on failure. */
+ezlda_corpus *ezlda_read_chunked(const char *prefix, int32_t num_chunks);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* EZLDA_H */

@@ -1,0 +1,65 @@
+// Launchers of the gfx950 sweep kernels (ezlda_kernels.hip).  Plain device
+// pointers and sizes; every launcher enqueues on the given stream and returns
+// the hipError_t of its launch.
+#pragma once
+#include <hip/hip_runtime_api.h>
+#include <stdint.h>
+
+namespace ez {
+
+// Device image of one corpus + model.  Token ids are word-major positions;
+// N < 2^32.  Sizes: V words, VD head words, D docs, K topics.
+struct DevState {
+    int32_t K = 0, V = 0, VD = 0, D = 0;
+    uint32_t N = 0;
+    float alpha = 0.f, beta = 0.f, vbeta = 0.f;
+    uint64_t seed = 0;
+    // tokens, word-major
+    uint32_t *word_off = nullptr;   // V+1
+    uint32_t *tok_doc = nullptr;    // N
+    uint16_t *z = nullptr;          // N
+    uint32_t *aux = nullptr;        // N: n_dk1 | n_dk2 << 16
+    // tokens, doc-major
+    uint32_t *doc_off = nullptr;    // D+1
+    uint32_t *doc_tok = nullptr;    // N: doc-major position -> token id
+    uint32_t *doc_wrd = nullptr;    // N: doc-major position -> word
+    // doc-topic rows
+    uint2 *doc_info = nullptr;      // D: x = DT row offset, y = nnz | len << 16
+    uint32_t *dt = nullptr;         // sum_d min(n_d,K): topic | count << 16
+    // word-topic: dense head rows + sorted sparse tail rows
+    int32_t *wt_head = nullptr;     // VD*K
+    uint32_t *tail_off = nullptr;   // V-VD+1 (capacity min(n_w,K))
+    uint32_t *wt_tail = nullptr;    // topic | count << 16
+    uint32_t *tail_nnz = nullptr;   // V-VD
+    int32_t *nk = nullptr;          // K
+    float *inv = nullptr;           // K: 1/(n_k + V beta)
+    ushort4 *wtop = nullptr;        // V: k1, k2, k3 (0xFFFF = none)
+    // work items
+    uint4 *wt_segs = nullptr;       // head-word histogram segments (w, beg, end)
+    int32_t n_wt_segs = 0;
+    uint4 *smp_segs = nullptr;      // sampler segments (w, beg, end)
+    int32_t n_smp_segs = 0;
+    uint32_t *counters = nullptr;   // [0] sweep, [1] work queue, [2..] spare
+    unsigned long long *stats = nullptr;  // [0] full draws, [1] DT entries scanned
+    double *ll_part = nullptr;      // n_smp_segs
+    double *ll_out = nullptr;       // 1
+    int32_t block_dim = 256;
+    int32_t grid_dim = 0;           // 0 = one workgroup per sampler item
+};
+
+hipError_t launch_zero_counts(const DevState &s, hipStream_t st);
+hipError_t launch_build_wt(const DevState &s, hipStream_t st);
+hipError_t launch_topk(const DevState &s, hipStream_t st);
+hipError_t launch_build_dt(const DevState &s, hipStream_t st);
+hipError_t launch_sample(const DevState &s, bool shortcut, hipStream_t st);
+hipError_t launch_advance_sweep(const DevState &s, hipStream_t st);
+hipError_t launch_loglik(const DevState &s, hipStream_t st);
+hipError_t launch_set_u32(uint32_t *p, uint32_t v, hipStream_t st);
+
+hipError_t launch_selftest_scan(const float *in, float *out, int nwaves, hipStream_t st);
+hipError_t launch_selftest_philox(const uint32_t *ctr, const uint32_t *key, uint32_t *out, int n, hipStream_t st);
+
+// Bytes of dynamic LDS the sampler needs for K topics and block size bd.
+size_t sample_lds_bytes(int K, int bd);
+
+}  // namespace ez

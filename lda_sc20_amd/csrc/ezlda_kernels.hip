@@ -1,0 +1,674 @@
+// gfx950 kernels of one EZLDA Gibbs sweep.
+//
+// Stage order per sweep (reference EZLDA/mainKernel.cu:158-380):
+//   zero_counts -> topic_hist (n_k) -> wt_head / wt_tail (WT from z)
+//   -> inv + word_top (phi top-3) -> dt_build (DT rows + per-token aux)
+//   -> sample (three-bucket draw, writes z) -> advance_sweep.
+// Every float operation follows the canonical order of the CPU oracle
+// (oracle/ezlda_oracle.c); builds use -ffp-contract=off, so the topics agree
+// bit for bit.  Wave64 throughout: one wavefront owns a doc (dt_build), a
+// tail word (wt_tail, word_top) or a token's DT row (sample).
+#include <hip/hip_runtime.h>
+
+#include "ezlda_device.h"
+#include "ezlda_philox.h"
+
+namespace ez {
+
+static constexpr float kMargin = 0.9990234375f;  // 1 - 2^-10
+static constexpr uint32_t kNone = 0xFFFFu;
+
+// ---------------------------------------------------------------- helpers
+template <int CTRL, int ROWMASK>
+__device__ __forceinline__ float dpp_or_zero(float v) {
+    return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), CTRL, ROWMASK, 0xF, true));
+}
+
+// Inclusive wave64 scan in the canonical tree: Hillis-Steele inside each
+// 16-lane row (row_shr 1,2,4,8), then row_bcast:15 into rows 1,3 and
+// row_bcast:31 into rows 2,3.  Inputs are >= 0, so adding the 0 that a
+// masked or out-of-row lane reads leaves a value unchanged.
+__device__ __forceinline__ float wave_scan(float v) {
+    v = v + dpp_or_zero<0x111, 0xF>(v);
+    v = v + dpp_or_zero<0x112, 0xF>(v);
+    v = v + dpp_or_zero<0x114, 0xF>(v);
+    v = v + dpp_or_zero<0x118, 0xF>(v);
+    v = v + dpp_or_zero<0x142, 0xA>(v);
+    v = v + dpp_or_zero<0x143, 0xC>(v);
+    return v;
+}
+
+__device__ __forceinline__ float readlane_f(float v, int l) {
+    return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), l));
+}
+__device__ __forceinline__ uint32_t readlane_u(uint32_t v, int l) {
+    return (uint32_t)__builtin_amdgcn_readlane((int)v, l);
+}
+__device__ __forceinline__ int first_lane(uint64_t m) { return __ffsll((unsigned long long)m) - 1; }
+__device__ __forceinline__ int last_lane(uint64_t m) { return 63 - __clzll((long long)m); }
+__device__ __forceinline__ uint64_t lanemask_lt(int lane) { return (1ull << lane) - 1ull; }
+
+// Orders LDS traffic between the phases of one wavefront.
+__device__ __forceinline__ void wave_lds_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// 16-bit counters, two per LDS word (counts < 65536 by construction).
+__device__ __forceinline__ void h16_add(uint32_t *h, uint32_t k) {
+    atomicAdd(&h[k >> 1], 1u << ((k & 1u) << 4));
+}
+__device__ __forceinline__ uint32_t h16_get(const uint32_t *h, uint32_t k) {
+    return (h[k >> 1] >> ((k & 1u) << 4)) & 0xFFFFu;
+}
+
+// ------------------------------------------------------------ count reset
+__global__ void k_fill_u32(uint32_t *__restrict__ p, size_t n, uint32_t v) {
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
+        p[i] = v;
+}
+
+__global__ void k_set_u32(uint32_t *p, uint32_t v) { *p = v; }
+__global__ void k_inc_u32(uint32_t *p) { *p = *p + 1u; }
+
+// ------------------------------------------------------- n_k histogram
+// n_k = column sums of WT (WTDen_Sum_Update_Kernel utility.cu:1104-1126 plus
+// the tail's atomicAdd in sparseMatrixAdd :1132-1204): one LDS histogram of
+// z per workgroup, flushed with one atomic per nonzero topic.
+__global__ void k_topic_hist(const uint16_t *__restrict__ z, uint32_t N, int K, int32_t *__restrict__ nk) {
+    extern __shared__ uint32_t sh[];
+    for (int k = threadIdx.x; k < K; k += blockDim.x) sh[k] = 0;
+    __syncthreads();
+    const size_t stride = (size_t)gridDim.x * blockDim.x * 2;
+    const uint32_t *z2 = reinterpret_cast<const uint32_t *>(z);
+    for (size_t i = ((size_t)blockIdx.x * blockDim.x + threadIdx.x) * 2; i < N; i += stride) {
+        if (i + 1 < N) {
+            uint32_t p = z2[i >> 1];
+            atomicAdd(&sh[p & 0xFFFFu], 1u);
+            atomicAdd(&sh[p >> 16], 1u);
+        } else {
+            atomicAdd(&sh[z[i]], 1u);
+        }
+    }
+    __syncthreads();
+    for (int k = threadIdx.x; k < K; k += blockDim.x) {
+        uint32_t c = sh[k];
+        if (c) atomicAdd(&nk[k], (int32_t)c);
+    }
+}
+
+// ------------------------------------------------------------- head WT
+// Dense head rows (WTDen_Update_Kernel, utility.cu:1067-1102): one workgroup
+// per segment of a head word's tokens, LDS histogram, atomic flush into the
+// (pre-zeroed) row.
+__global__ void k_wt_head(const uint16_t *__restrict__ z, const uint4 *__restrict__ segs, int K,
+                          int32_t *__restrict__ wt_head) {
+    extern __shared__ uint32_t sh[];
+    const uint4 sg = segs[blockIdx.x];
+    for (int k = threadIdx.x; k < K; k += blockDim.x) sh[k] = 0;
+    __syncthreads();
+    for (uint32_t i = sg.y + threadIdx.x; i < sg.z; i += blockDim.x) atomicAdd(&sh[z[i]], 1u);
+    __syncthreads();
+    int32_t *row = wt_head + (size_t)sg.x * K;
+    for (int k = threadIdx.x; k < K; k += blockDim.x) {
+        uint32_t c = sh[k];
+        if (c) atomicAdd(&row[k], (int32_t)c);
+    }
+}
+
+// ------------------------------------------------------------- tail WT
+// Sorted sparse tail rows (WT_Update_Kernel, utility.cu:10-76): one
+// wavefront per tail word; 16-bit LDS histogram, then a ballot/popcount
+// compaction in ascending topic order, clearing the histogram as it goes.
+__global__ void k_wt_tail(const uint16_t *__restrict__ z, const uint32_t *__restrict__ word_off, int VD, int V,
+                          int K, const uint32_t *__restrict__ tail_off, uint32_t *__restrict__ wt_tail,
+                          uint32_t *__restrict__ tail_nnz) {
+    extern __shared__ uint32_t sh[];
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6, nwv = blockDim.x >> 6;
+    const int KH = (K + 1) >> 1;
+    uint32_t *h = sh + wv * KH;
+    for (int k = lane; k < KH; k += 64) h[k] = 0;
+    wave_lds_sync();
+    const int ntail = V - VD;
+    for (int tw = blockIdx.x * nwv + wv; tw < ntail; tw += gridDim.x * nwv) {
+        const int w = VD + tw;
+        const uint32_t b = word_off[w], e = word_off[w + 1];
+        if (b == e) {
+            if (lane == 0) tail_nnz[tw] = 0;
+            continue;
+        }
+        for (uint32_t i = b + lane; i < e; i += 64) h16_add(h, z[i]);
+        wave_lds_sync();
+        const uint32_t base = tail_off[tw];
+        uint32_t nz = 0;
+        for (int j = 0; j < K; j += 64) {
+            const int k = j + lane;
+            const uint32_t c = k < K ? h16_get(h, (uint32_t)k) : 0u;
+            const uint64_t m = __ballot(c != 0);
+            if (c) wt_tail[base + nz + __popcll(m & lanemask_lt(lane))] = (uint32_t)k | (c << 16);
+            nz += (uint32_t)__popcll(m);
+            wave_lds_sync();
+            if (k < K && !(k & 1)) h[k >> 1] = 0;
+        }
+        wave_lds_sync();
+        if (lane == 0) tail_nnz[tw] = nz;
+    }
+}
+
+// ----------------------------------------------------- inv and top-3
+__global__ void k_inv(const int32_t *__restrict__ nk, int K, float vbeta, float *__restrict__ inv) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k < K) inv[k] = 1.0f / ((float)nk[k] + vbeta);
+}
+
+struct Top3 {
+    float v0, v1, v2;
+    uint32_t k0, k1, k2;
+};
+
+__device__ __forceinline__ bool better(float va, uint32_t ka, float vb, uint32_t kb) {
+    return va > vb || (va == vb && ka < kb);
+}
+
+__device__ __forceinline__ void top3_insert(Top3 &t, float v, uint32_t k) {
+    if (!better(v, k, t.v2, t.k2)) return;
+    if (better(v, k, t.v1, t.k1)) {
+        t.v2 = t.v1; t.k2 = t.k1;
+        if (better(v, k, t.v0, t.k0)) {
+            t.v1 = t.v0; t.k1 = t.k0;
+            t.v0 = v; t.k0 = k;
+        } else {
+            t.v1 = v; t.k1 = k;
+        }
+    } else {
+        t.v2 = v; t.k2 = k;
+    }
+}
+
+// Per word the three largest phi_wk, ties to the lower topic
+// (MaxTopicDense/Sparse_Update_Kernel, utility.cu:305-678, :679-1062).  One
+// wavefront per word; a tail word's counts are scattered into a 16-bit LDS
+// row first.  Per-lane lists are merged with a butterfly; the order is total,
+// so the merge order does not matter.
+__global__ void k_word_top(int V, int VD, int K, float beta, const int32_t *__restrict__ wt_head,
+                           const uint32_t *__restrict__ tail_off, const uint32_t *__restrict__ wt_tail,
+                           const uint32_t *__restrict__ tail_nnz, const float *__restrict__ inv,
+                           ushort4 *__restrict__ wtop) {
+    extern __shared__ uint32_t sh[];
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6, nwv = blockDim.x >> 6;
+    const int KH = (K + 1) >> 1;
+    uint32_t *h = sh + wv * KH;
+    for (int k = lane; k < KH; k += 64) h[k] = 0;
+    wave_lds_sync();
+    for (int w = blockIdx.x * nwv + wv; w < V; w += gridDim.x * nwv) {
+        Top3 t{-1.f, -1.f, -1.f, kNone, kNone, kNone};
+        if (w < VD) {
+            const int32_t *row = wt_head + (size_t)w * K;
+            for (int k = lane; k < K; k += 64) top3_insert(t, ((float)row[k] + beta) * inv[k], (uint32_t)k);
+        } else {
+            const int tw = w - VD;
+            const uint32_t b = tail_off[tw], n = tail_nnz[tw];
+            for (uint32_t i = lane; i < n; i += 64) {
+                const uint32_t e = wt_tail[b + i];
+                atomicAdd(&h[(e & 0xFFFFu) >> 1], (e >> 16) << ((e & 1u) << 4));
+            }
+            wave_lds_sync();
+            for (int k = lane; k < K; k += 64)
+                top3_insert(t, ((float)h16_get(h, (uint32_t)k) + beta) * inv[k], (uint32_t)k);
+            wave_lds_sync();
+            for (uint32_t i = lane; i < n; i += 64) h[(wt_tail[b + i] & 0xFFFFu) >> 1] = 0;
+            wave_lds_sync();
+        }
+#pragma unroll
+        for (int off = 32; off >= 1; off >>= 1) {
+            Top3 o;
+            o.v0 = __shfl_xor(t.v0, off); o.k0 = __shfl_xor(t.k0, off);
+            o.v1 = __shfl_xor(t.v1, off); o.k1 = __shfl_xor(t.k1, off);
+            o.v2 = __shfl_xor(t.v2, off); o.k2 = __shfl_xor(t.k2, off);
+            top3_insert(t, o.v0, o.k0);
+            top3_insert(t, o.v1, o.k1);
+            top3_insert(t, o.v2, o.k2);
+        }
+        if (lane == 0) wtop[w] = make_ushort4((unsigned short)t.k0, (unsigned short)t.k1, (unsigned short)t.k2, 0);
+    }
+}
+
+// --------------------------------------------------- DT rows and aux
+// DT_Update_Kernel (utility.cu:79-284): one wavefront per doc gathers the
+// topics of its tokens into a 16-bit LDS histogram, writes each token's
+// (n_dk1, n_dk2) at its word's top-2 topics (the re-pack of :125-161), and
+// compacts the row in ascending topic order (:183-238).
+__global__ void k_dt_build(int D, int K, const uint32_t *__restrict__ doc_off, const uint32_t *__restrict__ doc_tok,
+                           const uint32_t *__restrict__ doc_wrd, const uint16_t *__restrict__ z,
+                           const ushort4 *__restrict__ wtop, uint32_t *__restrict__ aux,
+                           uint2 *__restrict__ doc_info, uint32_t *__restrict__ dt) {
+    extern __shared__ uint32_t sh[];
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6, nwv = blockDim.x >> 6;
+    const int KH = (K + 1) >> 1;
+    uint32_t *h = sh + wv * KH;
+    for (int k = lane; k < KH; k += 64) h[k] = 0;
+    wave_lds_sync();
+    for (int d = blockIdx.x * nwv + wv; d < D; d += gridDim.x * nwv) {
+        const uint32_t b = doc_off[d], e = doc_off[d + 1];
+        for (uint32_t i = b + lane; i < e; i += 64) h16_add(h, z[doc_tok[i]]);
+        wave_lds_sync();
+        for (uint32_t i = b + lane; i < e; i += 64) {
+            const ushort4 tp = wtop[doc_wrd[i]];
+            const uint32_t n1 = h16_get(h, tp.x);
+            const uint32_t n2 = tp.y != kNone ? h16_get(h, tp.y) : 0u;
+            aux[doc_tok[i]] = n1 | (n2 << 16);
+        }
+        wave_lds_sync();
+        const uint32_t base = doc_info[d].x;
+        uint32_t nz = 0;
+        if (e > b) {
+            for (int j = 0; j < K; j += 64) {
+                const int k = j + lane;
+                const uint32_t c = k < K ? h16_get(h, (uint32_t)k) : 0u;
+                const uint64_t m = __ballot(c != 0);
+                if (c) dt[base + nz + __popcll(m & lanemask_lt(lane))] = (uint32_t)k | (c << 16);
+                nz += (uint32_t)__popcll(m);
+                wave_lds_sync();
+                if (k < K && !(k & 1)) h[k >> 1] = 0;
+            }
+            wave_lds_sync();
+        }
+        if (lane == 0) doc_info[d].y = nz | ((e - b) << 16);
+    }
+}
+
+// ------------------------------------------------------------- sampler
+// One workgroup per sampler item (a word, or a token segment of a head
+// word): phi_w in LDS with phi[k1] = 0, the Q tree over alpha*phi, then each
+// wavefront takes 64 tokens at a time, one lane per token for the shortcut
+// test and the whole wavefront per token for the full draw.  Replaces
+// UpdateProbKernelTrainD1 (utility.cu:2680-3312), LDAKernelTrainD
+// (:1625-2103) and LDAKernelTrain (:1217-1611).
+template <bool SHORTCUT>
+__global__ void __launch_bounds__(1024) k_sample(
+    const uint4 *__restrict__ segs, int VD, int K, float alpha, float beta, const int32_t *__restrict__ wt_head,
+    const uint32_t *__restrict__ tail_off, const uint32_t *__restrict__ wt_tail,
+    const uint32_t *__restrict__ tail_nnz, const float *__restrict__ inv, const ushort4 *__restrict__ wtop,
+    const uint32_t *__restrict__ tok_doc, const uint32_t *__restrict__ aux, const uint2 *__restrict__ doc_info,
+    const uint32_t *__restrict__ dt, uint16_t *__restrict__ z, const uint32_t *__restrict__ counters,
+    unsigned long long *__restrict__ stats, uint64_t seed) {
+    extern __shared__ float smf[];
+    __shared__ float f123[3];
+    uint32_t n_full = 0, n_scanned = 0;  // per wavefront: full draws, DT entries read
+    const int NS = (K + 63) >> 6;
+    const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6, nwv = blockDim.x >> 6;
+    float *phi = smf;             // NS*64
+    float *qtot = phi + NS * 64;  // NS
+    float *qacc = qtot + NS;      // NS
+    float *stot = qacc + NS + wv * 2 * NS;
+    float *sacc = stot + NS;
+    const uint4 sg = segs[blockIdx.x];
+    const int w = (int)sg.x;
+    const ushort4 tp = wtop[w];
+    const uint32_t sweep = counters[0];
+
+    if (w < VD) {
+        const int32_t *row = wt_head + (size_t)w * K;
+        for (int k = tid; k < NS * 64; k += blockDim.x) phi[k] = k < K ? ((float)row[k] + beta) * inv[k] : 0.0f;
+    } else {
+        for (int k = tid; k < NS * 64; k += blockDim.x) phi[k] = k < K ? beta * inv[k] : 0.0f;
+        __syncthreads();
+        const int tw = w - VD;
+        const uint32_t b = tail_off[tw], n = tail_nnz[tw];
+        for (uint32_t i = tid; i < n; i += blockDim.x) {
+            const uint32_t e = wt_tail[b + i];
+            const uint32_t k = e & 0xFFFFu;
+            phi[k] = ((float)(e >> 16) + beta) * inv[k];
+        }
+    }
+    __syncthreads();
+    if (tid == 0) {
+        f123[0] = phi[tp.x];
+        f123[1] = tp.y != kNone ? phi[tp.y] : 0.0f;
+        f123[2] = tp.z != kNone ? phi[tp.z] : 0.0f;
+        phi[tp.x] = 0.0f;
+    }
+    __syncthreads();
+    for (int j = wv; j < NS; j += nwv) {
+        const float q = wave_scan(alpha * phi[j * 64 + lane]);
+        if (lane == 63) qtot[j] = q;
+    }
+    __syncthreads();
+    if (tid == 0) {
+        float acc = 0.0f;
+        for (int j = 0; j < NS; ++j) {
+            acc = acc + qtot[j];
+            qacc[j] = acc;
+        }
+    }
+    __syncthreads();
+    const float Q = qacc[NS - 1];
+    const float f1 = f123[0], f2 = f123[1], f3 = f123[2];
+    const uint32_t k1 = tp.x;
+
+    for (uint32_t base = sg.y + (uint32_t)wv * 64u; base < sg.z; base += (uint32_t)nwv * 64u) {
+        const uint32_t t = base + lane;
+        const bool valid = t < sg.z;
+        uint2 di = make_uint2(0u, 0u);
+        float u = 0.0f, P1 = 0.0f;
+        bool need = false;
+        if (valid) {
+            const uint32_t d = tok_doc[t];
+            const uint32_t a = aux[t];
+            di = doc_info[d];
+            u = ez_u24(ez_philox_x0(seed, t, sweep, EZ_TAG_SAMPLE));
+            const uint32_t n1 = a & 0xFFFFu, n2 = a >> 16, nd = di.y >> 16;
+            P1 = ((float)n1 + alpha) * f1;
+            need = true;
+            if (SHORTCUT) {
+                const float maxS = (float)n2 * f2 + (float)(nd - n1 - n2) * f3;
+                const float B = (P1 + maxS) + Q;
+                if (u * B < P1 * kMargin) need = false;
+            }
+        }
+        uint32_t newz = k1;
+        uint64_t todo = __ballot(need);
+        while (todo) {
+            const int L = first_lane(todo);
+            todo &= todo - 1ull;
+            const uint32_t off = readlane_u(di.x, L);
+            const uint32_t nnz = readlane_u(di.y, L) & 0xFFFFu;
+            const float uL = readlane_f(u, L), P1L = readlane_f(P1, L);
+            const int nss = (int)((nnz + 63u) >> 6);
+            n_full += 1u;
+            n_scanned += nnz;
+            float S = 0.0f;
+            for (int j = 0; j < nss; ++j) {
+                const uint32_t i = (uint32_t)j * 64u + lane;
+                float p = 0.0f;
+                if (i < nnz) {
+                    const uint32_t e = dt[off + i];
+                    p = (float)(e >> 16) * phi[e & 0xFFFFu];
+                }
+                const float tot = readlane_f(wave_scan(p), 63);
+                S = S + tot;
+                if (lane == 0) {
+                    stot[j] = tot;
+                    sacc[j] = S;
+                }
+            }
+            wave_lds_sync();
+            const float A = P1L + S;
+            const float total = A + Q;
+            const float target = uL * total;
+            uint32_t zz = k1;
+            if (target < P1L) {
+                zz = k1;
+            } else if (target < A) {
+                const float t2 = target - P1L;
+                int js = -1;
+                for (int j0 = 0; j0 < nss && js < 0; j0 += 64) {
+                    const int j = j0 + lane;
+                    const uint64_t m = __ballot(j < nss && sacc[j] > t2);
+                    if (m) js = j0 + first_lane(m);
+                }
+                if (js < 0)
+                    for (int j0 = ((nss - 1) >> 6) << 6; j0 >= 0 && js < 0; j0 -= 64) {
+                        const int j = j0 + lane;
+                        const uint64_t m = __ballot(j < nss && stot[j] > 0.0f);
+                        if (m) js = j0 + last_lane(m);
+                    }
+                if (js >= 0) {
+                    const float t3 = t2 - (js > 0 ? sacc[js - 1] : 0.0f);
+                    const uint32_t i = (uint32_t)js * 64u + lane;
+                    float p = 0.0f;
+                    uint32_t e = 0;
+                    if (i < nnz) {
+                        e = dt[off + i];
+                        p = (float)(e >> 16) * phi[e & 0xFFFFu];
+                    }
+                    const float sc = wave_scan(p);
+                    uint64_t m = __ballot(i < nnz && sc > t3 && p > 0.0f);
+                    int ls = m ? first_lane(m) : -1;
+                    if (ls < 0) {
+                        const uint64_t m2 = __ballot(p > 0.0f);
+                        ls = m2 ? last_lane(m2) : -1;
+                    }
+                    if (ls >= 0) zz = readlane_u(e, ls) & 0xFFFFu;
+                }
+            } else {
+                const float t2 = target - A;
+                int js = -1;
+                for (int j0 = 0; j0 < NS && js < 0; j0 += 64) {
+                    const int j = j0 + lane;
+                    const uint64_t m = __ballot(j < NS && qacc[j] > t2);
+                    if (m) js = j0 + first_lane(m);
+                }
+                if (js < 0)
+                    for (int j0 = ((NS - 1) >> 6) << 6; j0 >= 0 && js < 0; j0 -= 64) {
+                        const int j = j0 + lane;
+                        const uint64_t m = __ballot(j < NS && qtot[j] > 0.0f);
+                        if (m) js = j0 + last_lane(m);
+                    }
+                if (js >= 0) {
+                    const float t3 = t2 - (js > 0 ? qacc[js - 1] : 0.0f);
+                    const int k = js * 64 + lane;
+                    const float p = alpha * phi[k];
+                    const float sc = wave_scan(p);
+                    uint64_t m = __ballot(k < K && sc > t3 && p > 0.0f);
+                    int ls = m ? first_lane(m) : -1;
+                    if (ls < 0) {
+                        const uint64_t m2 = __ballot(p > 0.0f);
+                        ls = m2 ? last_lane(m2) : -1;
+                    }
+                    if (ls >= 0) zz = (uint32_t)(js * 64 + ls);
+                }
+            }
+            if (lane == L) newz = zz;
+            wave_lds_sync();
+        }
+        if (valid) z[t] = (uint16_t)newz;
+    }
+    if (lane == 0 && n_full) {
+        atomicAdd(&stats[0], (unsigned long long)n_full);
+        atomicAdd(&stats[1], (unsigned long long)n_scanned);
+    }
+}
+
+// ------------------------------------------------ FP64 log-likelihood
+// sum_t log( sum_k (n_dk + alpha)/(n_d + K alpha) * (n_wk + beta)/(n_k + V beta) )
+// per token, the term of profilePerp/utility.cu:2398,2810 reduced in FP64
+// (profilePerp/utility.cu:2858-2928).  One workgroup per sampler item; the
+// per-item partials are summed in fixed order by k_ll_final.
+__global__ void __launch_bounds__(1024) k_ll(const uint4 *__restrict__ segs, int VD, int K, double alpha, double beta,
+                                             double vbeta, const int32_t *__restrict__ wt_head,
+                                             const uint32_t *__restrict__ tail_off,
+                                             const uint32_t *__restrict__ wt_tail,
+                                             const uint32_t *__restrict__ tail_nnz, const int32_t *__restrict__ nk,
+                                             const uint32_t *__restrict__ tok_doc,
+                                             const uint2 *__restrict__ doc_info, const uint32_t *__restrict__ dt,
+                                             double *__restrict__ part) {
+    extern __shared__ double smd[];
+    __shared__ double red[16];
+    const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6, nwv = blockDim.x >> 6;
+    double *phi = smd;
+    const uint4 sg = segs[blockIdx.x];
+    const int w = (int)sg.x;
+    if (w < VD) {
+        const int32_t *row = wt_head + (size_t)w * K;
+        for (int k = tid; k < K; k += blockDim.x) phi[k] = ((double)row[k] + beta) / ((double)nk[k] + vbeta);
+    } else {
+        for (int k = tid; k < K; k += blockDim.x) phi[k] = beta / ((double)nk[k] + vbeta);
+        __syncthreads();
+        const int tw = w - VD;
+        const uint32_t b = tail_off[tw], n = tail_nnz[tw];
+        for (uint32_t i = tid; i < n; i += blockDim.x) {
+            const uint32_t e = wt_tail[b + i];
+            const uint32_t k = e & 0xFFFFu;
+            phi[k] = ((double)(e >> 16) + beta) / ((double)nk[k] + vbeta);
+        }
+    }
+    __syncthreads();
+    double q = 0.0;
+    for (int k = tid; k < K; k += blockDim.x) q += phi[k];
+    for (int off = 32; off >= 1; off >>= 1) q += __shfl_xor(q, off);
+    if (lane == 0) red[wv] = q;
+    __syncthreads();
+    q = 0.0;
+    for (int i = 0; i < nwv; ++i) q += red[i];
+    q *= alpha;
+    __syncthreads();
+    double acc = 0.0;
+    for (uint32_t t = sg.y + (uint32_t)wv; t < sg.z; t += (uint32_t)nwv) {
+        const uint2 di = doc_info[tok_doc[t]];
+        const uint32_t nnz = di.y & 0xFFFFu;
+        double s = 0.0;
+        for (uint32_t i = lane; i < nnz; i += 64) {
+            const uint32_t e = dt[di.x + i];
+            s += (double)(e >> 16) * phi[e & 0xFFFFu];
+        }
+        for (int off = 32; off >= 1; off >>= 1) s += __shfl_xor(s, off);
+        acc += log((s + q) / ((double)(di.y >> 16) + (double)K * alpha));
+    }
+    if (lane == 0) red[wv] = acc;
+    __syncthreads();
+    if (tid == 0) {
+        double tot = 0.0;
+        for (int i = 0; i < nwv; ++i) tot += red[i];
+        part[blockIdx.x] = tot;
+    }
+}
+
+__global__ void k_ll_final(const double *__restrict__ part, int n, double *__restrict__ out) {
+    __shared__ double red[256];
+    double s = 0.0;
+    for (int i = threadIdx.x; i < n; i += blockDim.x) s += part[i];
+    red[threadIdx.x] = s;
+    __syncthreads();
+    for (int h = blockDim.x / 2; h > 0; h >>= 1) {
+        if ((int)threadIdx.x < h) red[threadIdx.x] += red[threadIdx.x + h];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) *out = red[0];
+}
+
+// ------------------------------------------------------------ self-tests
+__global__ void k_selftest_scan(const float *__restrict__ in, float *__restrict__ out, int n) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const float v = i < n ? in[i] : 0.0f;
+    const float s = wave_scan(v);
+    if (i < n) out[i] = s;
+}
+
+__global__ void k_selftest_philox(const uint4 *__restrict__ ctr, const uint2 *__restrict__ key,
+                                  uint4 *__restrict__ out, int n) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const uint4 c = ctr[i];
+    const ez_u4 o = ez_philox4x32_10(ez_u4{c.x, c.y, c.z, c.w}, key[i].x, key[i].y);
+    out[i] = make_uint4(o.x, o.y, o.z, o.w);
+}
+
+hipError_t launch_selftest_scan(const float *in, float *out, int nwaves, hipStream_t st) {
+    hipLaunchKernelGGL(k_selftest_scan, dim3(nwaves), dim3(64), 0, st, in, out, nwaves * 64);
+    return hipGetLastError();
+}
+
+hipError_t launch_selftest_philox(const uint32_t *ctr, const uint32_t *key, uint32_t *out, int n, hipStream_t st) {
+    hipLaunchKernelGGL(k_selftest_philox, dim3((n + 255) / 256), dim3(256), 0, st,
+                       reinterpret_cast<const uint4 *>(ctr), reinterpret_cast<const uint2 *>(key),
+                       reinterpret_cast<uint4 *>(out), n);
+    return hipGetLastError();
+}
+
+// ------------------------------------------------------------ launchers
+static int grid_for(size_t n, int bd, int cap) {
+    size_t g = (n + (size_t)bd - 1) / (size_t)bd;
+    if (g < 1) g = 1;
+    if (g > (size_t)cap) g = (size_t)cap;
+    return (int)g;
+}
+
+static size_t wave_hist_bytes(int K) { return (size_t)((K + 1) >> 1) * sizeof(uint32_t); }
+
+hipError_t launch_set_u32(uint32_t *p, uint32_t v, hipStream_t st) {
+    hipLaunchKernelGGL(k_set_u32, dim3(1), dim3(1), 0, st, p, v);
+    return hipGetLastError();
+}
+
+hipError_t launch_zero_counts(const DevState &s, hipStream_t st) {
+    const size_t nh = (size_t)s.VD * s.K;
+    if (nh) hipLaunchKernelGGL(k_fill_u32, dim3(grid_for(nh, 256, 4096)), dim3(256), 0, st,
+                               reinterpret_cast<uint32_t *>(s.wt_head), nh, 0u);
+    hipLaunchKernelGGL(k_fill_u32, dim3(grid_for(s.K, 256, 64)), dim3(256), 0, st,
+                       reinterpret_cast<uint32_t *>(s.nk), (size_t)s.K, 0u);
+    return hipGetLastError();
+}
+
+hipError_t launch_build_wt(const DevState &s, hipStream_t st) {
+    hipError_t e = launch_zero_counts(s, st);
+    if (e != hipSuccess) return e;
+    const size_t ldsK = (size_t)s.K * sizeof(uint32_t);
+    if (s.N) hipLaunchKernelGGL(k_topic_hist, dim3(grid_for(s.N, 256 * 16, 2048)), dim3(256), ldsK, st, s.z, s.N, s.K, s.nk);
+    if (s.n_wt_segs)
+        hipLaunchKernelGGL(k_wt_head, dim3(s.n_wt_segs), dim3(256), ldsK, st, s.z, s.wt_segs, s.K, s.wt_head);
+    const int ntail = s.V - s.VD;
+    if (ntail > 0) {
+        const int nwv = 4;
+        hipLaunchKernelGGL(k_wt_tail, dim3(grid_for(ntail, nwv, 1 << 16)), dim3(64 * nwv), nwv * wave_hist_bytes(s.K), st,
+                           s.z, s.word_off, s.VD, s.V, s.K, s.tail_off, s.wt_tail, s.tail_nnz);
+    }
+    return hipGetLastError();
+}
+
+hipError_t launch_topk(const DevState &s, hipStream_t st) {
+    hipLaunchKernelGGL(k_inv, dim3(grid_for(s.K, 256, 1 << 16)), dim3(256), 0, st, s.nk, s.K, s.vbeta, s.inv);
+    const int nwv = 4;
+    hipLaunchKernelGGL(k_word_top, dim3(grid_for(s.V, nwv, 1 << 16)), dim3(64 * nwv), nwv * wave_hist_bytes(s.K), st,
+                       s.V, s.VD, s.K, s.beta, s.wt_head, s.tail_off, s.wt_tail, s.tail_nnz, s.inv, s.wtop);
+    return hipGetLastError();
+}
+
+hipError_t launch_build_dt(const DevState &s, hipStream_t st) {
+    if (s.D == 0) return hipSuccess;
+    const int nwv = 4;
+    hipLaunchKernelGGL(k_dt_build, dim3(grid_for(s.D, nwv, 1 << 18)), dim3(64 * nwv), nwv * wave_hist_bytes(s.K), st,
+                       s.D, s.K, s.doc_off, s.doc_tok, s.doc_wrd, s.z, s.wtop, s.aux, s.doc_info, s.dt);
+    return hipGetLastError();
+}
+
+size_t sample_lds_bytes(int K, int bd) {
+    const int NS = (K + 63) >> 6;
+    return ((size_t)NS * 64 + 2 * (size_t)NS + (size_t)(bd >> 6) * 2 * NS) * sizeof(float);
+}
+
+hipError_t launch_sample(const DevState &s, bool shortcut, hipStream_t st) {
+    if (s.n_smp_segs == 0) return hipSuccess;
+    const int bd = s.block_dim;
+    const size_t lds = sample_lds_bytes(s.K, bd);
+    if (shortcut)
+        hipLaunchKernelGGL(k_sample<true>, dim3(s.n_smp_segs), dim3(bd), lds, st, s.smp_segs, s.VD, s.K, s.alpha,
+                           s.beta, s.wt_head, s.tail_off, s.wt_tail, s.tail_nnz, s.inv, s.wtop, s.tok_doc, s.aux,
+                           s.doc_info, s.dt, s.z, s.counters, s.stats, s.seed);
+    else
+        hipLaunchKernelGGL(k_sample<false>, dim3(s.n_smp_segs), dim3(bd), lds, st, s.smp_segs, s.VD, s.K, s.alpha,
+                           s.beta, s.wt_head, s.tail_off, s.wt_tail, s.tail_nnz, s.inv, s.wtop, s.tok_doc, s.aux,
+                           s.doc_info, s.dt, s.z, s.counters, s.stats, s.seed);
+    return hipGetLastError();
+}
+
+hipError_t launch_advance_sweep(const DevState &s, hipStream_t st) {
+    hipLaunchKernelGGL(k_inc_u32, dim3(1), dim3(1), 0, st, s.counters);
+    return hipGetLastError();
+}
+
+hipError_t launch_loglik(const DevState &s, hipStream_t st) {
+    if (s.n_smp_segs == 0) {
+        hipLaunchKernelGGL(k_ll_final, dim3(1), dim3(256), 0, st, s.ll_part, 0, s.ll_out);
+        return hipGetLastError();
+    }
+    const int bd = 256;
+    hipLaunchKernelGGL(k_ll, dim3(s.n_smp_segs), dim3(bd), (size_t)s.K * sizeof(double), st, s.smp_segs, s.VD, s.K,
+                       (double)s.alpha, (double)s.beta, (double)s.V * (double)s.beta, s.wt_head, s.tail_off,
+                       s.wt_tail, s.tail_nnz, s.nk, s.tok_doc, s.doc_info, s.dt, s.ll_part);
+    hipLaunchKernelGGL(k_ll_final, dim3(1), dim3(256), 0, st, s.ll_part, s.n_smp_segs, s.ll_out);
+    return hipGetLastError();
+}
+
+}  // namespace ez

@@ -1,0 +1,751 @@
+// Host runtime behind include/ezlda.h: context, word-major corpus layout,
+// device allocation, stage and sweep entry points, hipGraph replay, timing,
+// checkpoints.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <thread>
+#include <vector>
+
+#include "../../include/ezlda.h"
+#include "ezlda_device.h"
+#include "ezlda_philox.h"
+
+namespace {
+
+thread_local std::string g_err;
+
+// Host-side threads for the layout builder (0 = hardware concurrency).
+int host_threads() {
+    unsigned n = std::thread::hardware_concurrency();
+    if (n == 0) n = 1;
+    if (n > 16) n = 16;
+    return (int)n;
+}
+
+template <class F>
+void parallel_for(int64_t n, F f) {
+    const int T = host_threads();
+    if (n < 65536 || T == 1) {
+        f(0, n, 0);
+        return;
+    }
+    std::vector<std::thread> th;
+    for (int t = 0; t < T; ++t) {
+        const int64_t b = n * t / T, e = n * (t + 1) / T;
+        th.emplace_back([=] { f(b, e, t); });
+    }
+    for (auto &x : th) x.join();
+}
+
+}  // namespace
+
+namespace ezc {
+void set_error(const std::string &m) { g_err = m; }
+}  // namespace ezc
+
+struct ezlda_ctx {
+    ezlda_args a{};
+    ez::DevState s{};
+    hipStream_t st = nullptr;
+    std::string err;
+    bool loaded = false;
+    bool counts_fresh = false;
+    uint32_t sweep = 0;
+    int64_t dev_bytes = 0;
+    int64_t dt_cap = 0, tail_cap = 0;
+    std::vector<void *> allocs;
+    // host mirrors for result mapping
+    std::vector<uint32_t> doc_tok;   // doc-major position -> token id
+    std::vector<int32_t> rank_of;    // input word -> rank
+    std::vector<uint32_t> word_off;  // V+1 by rank
+    std::vector<uint32_t> doc_off;   // D+1
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t graph_exec = nullptr;
+};
+
+namespace {
+
+int fail(ezlda_ctx *c, int code, const std::string &msg) {
+    if (c) c->err = msg;
+    g_err = msg;
+    return code;
+}
+
+#define EZ_TRY_HIP(ctx, call)                                                                    \
+    do {                                                                                         \
+        hipError_t e_ = (call);                                                                  \
+        if (e_ != hipSuccess)                                                                    \
+            return fail((ctx), EZLDA_EHIP, std::string(#call) + ": " + hipGetErrorString(e_));   \
+    } while (0)
+
+template <class T>
+int dalloc(ezlda_ctx *c, T **p, size_t n) {
+    *p = nullptr;
+    const size_t bytes = std::max<size_t>(n, 1) * sizeof(T);
+    hipError_t e = hipMalloc((void **)p, bytes);
+    if (e != hipSuccess) return fail(c, EZLDA_EHIP, std::string("hipMalloc: ") + hipGetErrorString(e));
+    c->allocs.push_back((void *)*p);
+    c->dev_bytes += (int64_t)bytes;
+    return EZLDA_OK;
+}
+
+template <class T>
+int upload(ezlda_ctx *c, T **p, const std::vector<T> &v) {
+    int r = dalloc(c, p, v.size());
+    if (r) return r;
+    if (!v.empty()) EZ_TRY_HIP(c, hipMemcpy(*p, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice));
+    return EZLDA_OK;
+}
+
+void free_device(ezlda_ctx *c) {
+    if (c->graph_exec) hipGraphExecDestroy(c->graph_exec);
+    if (c->graph) hipGraphDestroy(c->graph);
+    c->graph_exec = nullptr;
+    c->graph = nullptr;
+    for (void *p : c->allocs) hipFree(p);
+    c->allocs.clear();
+    c->dev_bytes = 0;
+    c->s = ez::DevState{};
+    c->loaded = false;
+}
+
+int need_loaded(ezlda_ctx *c) {
+    if (!c) return fail(nullptr, EZLDA_EINVAL, "null context");
+    if (!c->loaded) return fail(c, EZLDA_ESTATE, "no corpus loaded");
+    return EZLDA_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+void ezlda_default_args(ezlda_args *a) {
+    std::memset(a, 0, sizeof(*a));
+    a->K = 1024;             // Argument.cuh:6
+    a->alpha = 50.0f / 1024; // Argument.cuh:9
+    a->beta = 0.01f;         // Argument.cuh:8
+    a->block_dim = 256;      // Argument.cuh:4
+    a->grid_dim = 0;
+    a->num_chunks = 1;
+    a->num_streams = 1;
+    a->token_segment = 4096;
+    a->seed = 2002;
+    a->dense_words = -1;
+    a->dense_threshold = 0;  // K, as preprocess/streamTestBalance.cpp:291
+    a->shortcut = 1;
+    a->topic_bits = 16;
+}
+
+const char *ezlda_last_error(const ezlda_ctx *ctx) { return ctx ? ctx->err.c_str() : g_err.c_str(); }
+
+ezlda_ctx *ezlda_create(const ezlda_args *args) {
+    if (!args) {
+        fail(nullptr, EZLDA_EINVAL, "null args");
+        return nullptr;
+    }
+    ezlda_args a = *args;
+    if (a.K < 1 || a.K > 65535) {
+        fail(nullptr, EZLDA_EINVAL, "K must be in [1, 65535]");
+        return nullptr;
+    }
+    if (a.alpha <= 0.f) a.alpha = 50.0f / (float)a.K;
+    if (a.beta <= 0.f) a.beta = 0.01f;
+    if (a.block_dim <= 0) a.block_dim = 256;
+    if (a.block_dim % 64 != 0 || a.block_dim > 1024) {
+        fail(nullptr, EZLDA_EINVAL, "block_dim must be a multiple of 64 and <= 1024");
+        return nullptr;
+    }
+    if (a.token_segment <= 0) a.token_segment = 4096;
+    if (a.topic_bits == 0) a.topic_bits = 16;
+    if (a.topic_bits != 16) {
+        fail(nullptr, EZLDA_EINVAL, "only 16-bit topics are supported");
+        return nullptr;
+    }
+    if (a.grid_dim < 0) a.grid_dim = 0;
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) {
+        fail(nullptr, EZLDA_EHIP, "no HIP device");
+        return nullptr;
+    }
+    const size_t lds = ez::sample_lds_bytes(a.K, a.block_dim);
+    if (lds > 160 * 1024) {
+        fail(nullptr, EZLDA_EINVAL, "K too large for the sampler's LDS at this block_dim");
+        return nullptr;
+    }
+    ezlda_ctx *c = new ezlda_ctx();
+    c->a = a;
+    if (hipStreamCreateWithFlags(&c->st, hipStreamNonBlocking) != hipSuccess) {
+        fail(nullptr, EZLDA_EHIP, "hipStreamCreate failed");
+        delete c;
+        return nullptr;
+    }
+    return c;
+}
+
+void ezlda_destroy(ezlda_ctx *c) {
+    if (!c) return;
+    if (c->st) hipStreamSynchronize(c->st);
+    free_device(c);
+    if (c->st) hipStreamDestroy(c->st);
+    delete c;
+}
+
+int ezlda_load_docs(ezlda_ctx *c, int32_t V, int32_t D, const int64_t *doc_off, const int32_t *words,
+                    const uint16_t *z0) {
+    if (!c) return fail(nullptr, EZLDA_EINVAL, "null context");
+    if (V < 1 || D < 0 || !doc_off || (!words && D > 0 && doc_off[D] > 0))
+        return fail(c, EZLDA_EINVAL, "bad corpus arguments");
+    const int K = c->a.K;
+    const int64_t N64 = doc_off[D] - doc_off[0];
+    if (doc_off[0] != 0) return fail(c, EZLDA_EINVAL, "doc_off[0] must be 0");
+    if (N64 < 0 || N64 >= (int64_t)0xFFFFFFFFll) return fail(c, EZLDA_EINVAL, "token count must be below 2^32");
+    for (int d = 0; d < D; ++d) {
+        const int64_t n = doc_off[d + 1] - doc_off[d];
+        if (n < 0) return fail(c, EZLDA_EINVAL, "doc_off must be non-decreasing");
+        if (n > 65535) return fail(c, EZLDA_EINVAL, "a document has more than 65535 tokens (doc " + std::to_string(d) + ")");
+    }
+    const uint32_t N = (uint32_t)N64;
+    // word counts and the rank order (count desc, id asc)
+    std::vector<int64_t> cnt(V, 0);
+    for (uint32_t i = 0; i < N; ++i) {
+        const int32_t w = words[i];
+        if (w < 0 || w >= V) return fail(c, EZLDA_EINVAL, "word id out of range at token " + std::to_string(i));
+        cnt[w]++;
+    }
+    if (z0)
+        for (uint32_t i = 0; i < N; ++i)
+            if (z0[i] >= K) return fail(c, EZLDA_EINVAL, "initial topic out of range at token " + std::to_string(i));
+    free_device(c);
+    std::vector<int32_t> ids(V);
+    for (int w = 0; w < V; ++w) ids[w] = w;
+    std::sort(ids.begin(), ids.end(), [&](int32_t x, int32_t y) { return cnt[x] > cnt[y] || (cnt[x] == cnt[y] && x < y); });
+    c->rank_of.assign(V, 0);
+    for (int r = 0; r < V; ++r) c->rank_of[ids[r]] = r;
+    c->word_off.assign(V + 1, 0);
+    for (int r = 0; r < V; ++r) c->word_off[r + 1] = c->word_off[r] + (uint32_t)cnt[ids[r]];
+    c->doc_off.assign(D + 1, 0);
+    for (int d = 0; d <= D; ++d) c->doc_off[d] = (uint32_t)doc_off[d];
+
+    // placement: contiguous doc ranges per thread keep (doc, position) order
+    const int T = host_threads();
+    std::vector<int> dsplit(T + 1, 0);
+    for (int t = 0; t <= T; ++t) {
+        const int64_t target = N64 * t / T;
+        dsplit[t] = (int)(std::lower_bound(doc_off, doc_off + D + 1, target) - doc_off);
+        if (dsplit[t] > D) dsplit[t] = D;
+    }
+    dsplit[0] = 0;
+    dsplit[T] = D;
+    std::vector<std::vector<uint32_t>> tcnt(T, std::vector<uint32_t>(V, 0));
+    {
+        std::vector<std::thread> th;
+        for (int t = 0; t < T; ++t)
+            th.emplace_back([&, t] {
+                for (int64_t i = doc_off[dsplit[t]]; i < doc_off[dsplit[t + 1]]; ++i) tcnt[t][c->rank_of[words[i]]]++;
+            });
+        for (auto &x : th) x.join();
+    }
+    for (int r = 0; r < V; ++r) {
+        uint32_t cur = c->word_off[r];
+        for (int t = 0; t < T; ++t) {
+            const uint32_t n = tcnt[t][r];
+            tcnt[t][r] = cur;
+            cur += n;
+        }
+    }
+    std::vector<uint32_t> tok_doc(N), doc_wrd(N);
+    std::vector<uint16_t> z(N);
+    c->doc_tok.assign(N, 0);
+    {
+        std::vector<std::thread> th;
+        for (int t = 0; t < T; ++t)
+            th.emplace_back([&, t] {
+                std::vector<uint32_t> &cur = tcnt[t];
+                for (int d = dsplit[t]; d < dsplit[t + 1]; ++d)
+                    for (int64_t i = doc_off[d]; i < doc_off[d + 1]; ++i) {
+                        const int32_t r = c->rank_of[words[i]];
+                        const uint32_t tk = cur[r]++;
+                        tok_doc[tk] = (uint32_t)d;
+                        c->doc_tok[i] = tk;
+                        doc_wrd[i] = (uint32_t)r;
+                        if (z0) z[tk] = z0[i];
+                    }
+            });
+        for (auto &x : th) x.join();
+    }
+    tcnt.clear();
+    if (!z0) {
+        const uint64_t seed = c->a.seed;
+        parallel_for(N, [&](int64_t b, int64_t e, int) {
+            for (int64_t t = b; t < e; ++t)
+                z[t] = (uint16_t)(ez_philox_x0(seed, (uint64_t)t, 0u, EZ_TAG_INIT) % (uint32_t)K);
+        });
+    }
+    // head / tail split
+    int VD;
+    if (c->a.dense_words >= 0) {
+        VD = std::min(c->a.dense_words, V);
+    } else {
+        const int64_t thr = c->a.dense_threshold > 0 ? c->a.dense_threshold : K;
+        VD = 0;
+        while (VD < V && cnt[ids[VD]] >= thr) ++VD;
+    }
+    while (VD < V && cnt[ids[VD]] >= 65536) ++VD;  // tail counts are 16-bit
+    std::vector<uint32_t> tail_off(V - VD + 1, 0);
+    for (int r = VD; r < V; ++r) {
+        const uint32_t n = c->word_off[r + 1] - c->word_off[r];
+        tail_off[r - VD + 1] = tail_off[r - VD] + std::min<uint32_t>(n, (uint32_t)K);
+    }
+    std::vector<uint2> doc_info(D);
+    uint64_t dtc = 0;
+    for (int d = 0; d < D; ++d) {
+        const uint32_t n = c->doc_off[d + 1] - c->doc_off[d];
+        doc_info[d] = make_uint2((uint32_t)dtc, n << 16);
+        dtc += std::min<uint32_t>(n, (uint32_t)K);
+    }
+    // work items
+    const uint32_t WT_SEG = 32768;
+    std::vector<uint4> wt_segs, smp_segs;
+    for (int r = 0; r < VD; ++r)
+        for (uint32_t b = c->word_off[r]; b < c->word_off[r + 1]; b += WT_SEG)
+            wt_segs.push_back(make_uint4((uint32_t)r, b, std::min(b + WT_SEG, c->word_off[r + 1]), 0u));
+    const uint32_t SEG = (uint32_t)c->a.token_segment;
+    for (int r = 0; r < V; ++r)
+        for (uint32_t b = c->word_off[r]; b < c->word_off[r + 1]; b += SEG)
+            smp_segs.push_back(make_uint4((uint32_t)r, b, std::min(b + SEG, c->word_off[r + 1]), 0u));
+
+    ez::DevState &s = c->s;
+    s.K = K; s.V = V; s.VD = VD; s.D = D; s.N = N;
+    s.alpha = c->a.alpha; s.beta = c->a.beta;
+    s.vbeta = (float)((double)V * (double)c->a.beta);
+    s.seed = c->a.seed;
+    s.block_dim = c->a.block_dim;
+    s.grid_dim = c->a.grid_dim;
+    c->dt_cap = (int64_t)dtc;
+    c->tail_cap = tail_off.back();
+    int r = 0;
+    if ((r = upload(c, &s.word_off, c->word_off))) return r;
+    if ((r = upload(c, &s.tok_doc, tok_doc))) return r;
+    if ((r = upload(c, &s.z, z))) return r;
+    if ((r = dalloc(c, &s.aux, N))) return r;
+    if ((r = upload(c, &s.doc_off, c->doc_off))) return r;
+    if ((r = upload(c, &s.doc_tok, c->doc_tok))) return r;
+    if ((r = upload(c, &s.doc_wrd, doc_wrd))) return r;
+    if ((r = upload(c, &s.doc_info, doc_info))) return r;
+    if ((r = dalloc(c, &s.dt, dtc))) return r;
+    if ((r = dalloc(c, &s.wt_head, (size_t)VD * K))) return r;
+    if ((r = upload(c, &s.tail_off, tail_off))) return r;
+    if ((r = dalloc(c, &s.wt_tail, tail_off.back()))) return r;
+    if ((r = dalloc(c, &s.tail_nnz, V - VD))) return r;
+    if ((r = dalloc(c, &s.nk, K))) return r;
+    if ((r = dalloc(c, &s.inv, K))) return r;
+    if ((r = dalloc(c, &s.wtop, V))) return r;
+    if ((r = upload(c, &s.wt_segs, wt_segs))) return r;
+    s.n_wt_segs = (int32_t)wt_segs.size();
+    if ((r = upload(c, &s.smp_segs, smp_segs))) return r;
+    s.n_smp_segs = (int32_t)smp_segs.size();
+    if ((r = dalloc(c, &s.counters, 16))) return r;
+    if ((r = dalloc(c, &s.stats, 2))) return r;
+    if ((r = dalloc(c, &s.ll_part, smp_segs.size()))) return r;
+    if ((r = dalloc(c, &s.ll_out, 1))) return r;
+    EZ_TRY_HIP(c, hipMemsetAsync(s.counters, 0, 16 * sizeof(uint32_t), c->st));
+    EZ_TRY_HIP(c, hipMemsetAsync(s.stats, 0, 2 * sizeof(unsigned long long), c->st));
+    EZ_TRY_HIP(c, hipStreamSynchronize(c->st));
+    EZ_TRY_HIP(c, hipDeviceSynchronize());
+    c->sweep = 0;
+    c->counts_fresh = false;
+    c->loaded = true;
+    return EZLDA_OK;
+}
+
+int ezlda_load_chunked(ezlda_ctx *c, const char *prefix, int32_t num_chunks) {
+    if (!c) return fail(nullptr, EZLDA_EINVAL, "null context");
+    ezlda_corpus *cp = ezlda_read_chunked(prefix, num_chunks);
+    if (!cp) return fail(c, EZLDA_EIO, g_err);
+    const int r = ezlda_load_docs(c, ezlda_corpus_num_words(cp), ezlda_corpus_num_docs(cp), ezlda_corpus_doc_off(cp),
+                                  ezlda_corpus_words(cp), ezlda_corpus_topics(cp));
+    ezlda_corpus_free(cp);
+    return r;
+}
+
+int ezlda_get_info(ezlda_ctx *c, ezlda_info *info) {
+    if (int r = need_loaded(c)) return r;
+    info->num_tokens = c->s.N;
+    info->num_words = c->s.V;
+    info->num_docs = c->s.D;
+    info->num_head_words = c->s.VD;
+    info->K = c->s.K;
+    info->dt_capacity = c->dt_cap;
+    info->wt_tail_capacity = c->tail_cap;
+    info->num_sample_items = c->s.n_smp_segs;
+    info->sweep = c->sweep;
+    info->device_bytes = c->dev_bytes;
+    return EZLDA_OK;
+}
+
+int ezlda_build_wt(ezlda_ctx *c) {
+    if (int r = need_loaded(c)) return r;
+    EZ_TRY_HIP(c, ez::launch_build_wt(c->s, c->st));
+    EZ_TRY_HIP(c, hipStreamSynchronize(c->st));
+    c->counts_fresh = true;
+    return EZLDA_OK;
+}
+
+int ezlda_topk(ezlda_ctx *c) {
+    if (int r = need_loaded(c)) return r;
+    EZ_TRY_HIP(c, ez::launch_topk(c->s, c->st));
+    EZ_TRY_HIP(c, hipStreamSynchronize(c->st));
+    return EZLDA_OK;
+}
+
+int ezlda_build_dt(ezlda_ctx *c) {
+    if (int r = need_loaded(c)) return r;
+    EZ_TRY_HIP(c, ez::launch_build_dt(c->s, c->st));
+    EZ_TRY_HIP(c, hipStreamSynchronize(c->st));
+    return EZLDA_OK;
+}
+
+int ezlda_sample(ezlda_ctx *c) {
+    if (int r = need_loaded(c)) return r;
+    EZ_TRY_HIP(c, ez::launch_sample(c->s, c->a.shortcut != 0, c->st));
+    EZ_TRY_HIP(c, ez::launch_advance_sweep(c->s, c->st));
+    EZ_TRY_HIP(c, hipStreamSynchronize(c->st));
+    c->sweep++;
+    c->counts_fresh = false;
+    return EZLDA_OK;
+}
+
+// One sweep in stage order; ev (nullable) receives EZLDA_NUM_STAGES + 1
+// events that bracket the stages.
+static int enqueue_sweep(ezlda_ctx *c, hipEvent_t *ev) {
+    if (ev) EZ_TRY_HIP(c, hipEventRecord(ev[0], c->st));
+    EZ_TRY_HIP(c, ez::launch_build_wt(c->s, c->st));
+    if (ev) EZ_TRY_HIP(c, hipEventRecord(ev[1], c->st));
+    EZ_TRY_HIP(c, ez::launch_topk(c->s, c->st));
+    if (ev) EZ_TRY_HIP(c, hipEventRecord(ev[2], c->st));
+    EZ_TRY_HIP(c, ez::launch_build_dt(c->s, c->st));
+    if (ev) EZ_TRY_HIP(c, hipEventRecord(ev[3], c->st));
+    EZ_TRY_HIP(c, ez::launch_sample(c->s, c->a.shortcut != 0, c->st));
+    if (ev) EZ_TRY_HIP(c, hipEventRecord(ev[4], c->st));
+    EZ_TRY_HIP(c, ez::launch_advance_sweep(c->s, c->st));
+    return EZLDA_OK;
+}
+
+int ezlda_sweep(ezlda_ctx *c, int32_t n) {
+    if (int r = need_loaded(c)) return r;
+    for (int i = 0; i < n; ++i) {
+        if (int r = enqueue_sweep(c, nullptr)) return r;
+        c->sweep++;
+    }
+    EZ_TRY_HIP(c, hipStreamSynchronize(c->st));
+    c->counts_fresh = false;
+    return EZLDA_OK;
+}
+
+static int ensure_graph(ezlda_ctx *c) {
+    if (c->graph_exec) return EZLDA_OK;
+    EZ_TRY_HIP(c, hipStreamBeginCapture(c->st, hipStreamCaptureModeThreadLocal));
+    int r = enqueue_sweep(c, nullptr);
+    hipGraph_t g = nullptr;
+    hipError_t e = hipStreamEndCapture(c->st, &g);
+    if (r) {
+        if (g) hipGraphDestroy(g);
+        return r;
+    }
+    if (e != hipSuccess) return fail(c, EZLDA_EHIP, std::string("hipStreamEndCapture: ") + hipGetErrorString(e));
+    c->graph = g;
+    EZ_TRY_HIP(c, hipGraphInstantiate(&c->graph_exec, c->graph, nullptr, nullptr, 0));
+    return EZLDA_OK;
+}
+
+int ezlda_sweep_graph(ezlda_ctx *c, int32_t n) {
+    if (int r = need_loaded(c)) return r;
+    if (int r = ensure_graph(c)) return r;
+    for (int i = 0; i < n; ++i) {
+        EZ_TRY_HIP(c, hipGraphLaunch(c->graph_exec, c->st));
+        c->sweep++;
+    }
+    EZ_TRY_HIP(c, hipStreamSynchronize(c->st));
+    c->counts_fresh = false;
+    return EZLDA_OK;
+}
+
+// Back-to-back sweeps with no host synchronisation inside the timed region.
+// Eager: every stage is bracketed by events, all read after the last sweep.
+// Graph: one event pair around n replays (stage times are then -1).
+int ezlda_time_sweeps(ezlda_ctx *c, int32_t n, int32_t use_graph, double *total_ms, double *stage_ms) {
+    if (int r = need_loaded(c)) return r;
+    if (n < 1) return fail(c, EZLDA_EINVAL, "num_sweeps must be >= 1");
+    double st[EZLDA_NUM_STAGES] = {0, 0, 0, 0};
+    double tot = 0.0;
+    if (use_graph) {
+        if (int r = ensure_graph(c)) return r;
+    }
+    const int per = use_graph ? 2 : (EZLDA_NUM_STAGES + 1);
+    const size_t nev = use_graph ? 2 : (size_t)n * per;
+    std::vector<hipEvent_t> ev(nev, nullptr);
+    int rc = EZLDA_OK;
+    for (auto &e : ev)
+        if (hipEventCreate(&e) != hipSuccess) rc = fail(c, EZLDA_EHIP, "hipEventCreate failed");
+    if (rc == EZLDA_OK) {
+        if (use_graph) {
+            hipEventRecord(ev[0], c->st);
+            for (int i = 0; i < n && rc == EZLDA_OK; ++i) {
+                if (hipGraphLaunch(c->graph_exec, c->st) != hipSuccess) rc = fail(c, EZLDA_EHIP, "hipGraphLaunch failed");
+                c->sweep++;
+            }
+            hipEventRecord(ev[1], c->st);
+        } else {
+            for (int i = 0; i < n && rc == EZLDA_OK; ++i) {
+                rc = enqueue_sweep(c, &ev[(size_t)i * per]);
+                c->sweep++;
+            }
+        }
+    }
+    if (rc == EZLDA_OK && hipStreamSynchronize(c->st) != hipSuccess) rc = fail(c, EZLDA_EHIP, "sweep failed");
+    if (rc == EZLDA_OK) {
+        float ms = 0.f;
+        if (use_graph) {
+            hipEventElapsedTime(&ms, ev[0], ev[1]);
+            tot = ms;
+            for (auto &x : st) x = -1.0;
+        } else {
+            hipEventElapsedTime(&ms, ev[0], ev[nev - 1]);
+            tot = ms;
+            for (int i = 0; i < n; ++i)
+                for (int k = 0; k < EZLDA_NUM_STAGES; ++k) {
+                    hipEventElapsedTime(&ms, ev[(size_t)i * per + k], ev[(size_t)i * per + k + 1]);
+                    st[k] += ms;
+                }
+        }
+    }
+    for (auto &e : ev)
+        if (e) hipEventDestroy(e);
+    c->counts_fresh = false;
+    if (rc) return rc;
+    if (total_ms) *total_ms = tot;
+    if (stage_ms)
+        for (int k = 0; k < EZLDA_NUM_STAGES; ++k) stage_ms[k] = st[k];
+    return EZLDA_OK;
+}
+
+int ezlda_get_stats(ezlda_ctx *c, uint64_t *stats) {
+    if (int r = need_loaded(c)) return r;
+    unsigned long long v[2] = {0, 0};
+    EZ_TRY_HIP(c, hipMemcpy(v, c->s.stats, sizeof v, hipMemcpyDeviceToHost));
+    stats[0] = v[0];
+    stats[1] = v[1];
+    std::vector<uint32_t> tn(c->s.V - c->s.VD);
+    std::vector<uint2> di(c->s.D);
+    if (!tn.empty()) EZ_TRY_HIP(c, hipMemcpy(tn.data(), c->s.tail_nnz, tn.size() * 4, hipMemcpyDeviceToHost));
+    if (!di.empty()) EZ_TRY_HIP(c, hipMemcpy(di.data(), c->s.doc_info, di.size() * sizeof(uint2), hipMemcpyDeviceToHost));
+    uint64_t a = 0, b = 0;
+    for (uint32_t x : tn) a += x;
+    for (const uint2 &x : di) b += x.y & 0xFFFFu;
+    stats[2] = a;
+    stats[3] = b;
+    return EZLDA_OK;
+}
+
+int ezlda_reset_stats(ezlda_ctx *c) {
+    if (int r = need_loaded(c)) return r;
+    EZ_TRY_HIP(c, hipMemsetAsync(c->s.stats, 0, 2 * sizeof(unsigned long long), c->st));
+    EZ_TRY_HIP(c, hipStreamSynchronize(c->st));
+    return EZLDA_OK;
+}
+
+int ezlda_loglik(ezlda_ctx *c, double *ll) {
+    if (int r = need_loaded(c)) return r;
+    if (!c->counts_fresh) EZ_TRY_HIP(c, ez::launch_build_wt(c->s, c->st));
+    c->counts_fresh = true;
+    EZ_TRY_HIP(c, ez::launch_topk(c->s, c->st));
+    EZ_TRY_HIP(c, ez::launch_build_dt(c->s, c->st));
+    EZ_TRY_HIP(c, ez::launch_loglik(c->s, c->st));
+    EZ_TRY_HIP(c, hipMemcpyAsync(ll, c->s.ll_out, sizeof(double), hipMemcpyDeviceToHost, c->st));
+    EZ_TRY_HIP(c, hipStreamSynchronize(c->st));
+    return EZLDA_OK;
+}
+
+int ezlda_get_topics(ezlda_ctx *c, uint16_t *zout) {
+    if (int r = need_loaded(c)) return r;
+    std::vector<uint16_t> z(c->s.N);
+    EZ_TRY_HIP(c, hipMemcpy(z.data(), c->s.z, z.size() * sizeof(uint16_t), hipMemcpyDeviceToHost));
+    for (size_t i = 0; i < z.size(); ++i) zout[i] = z[c->doc_tok[i]];
+    return EZLDA_OK;
+}
+
+int ezlda_set_topics(ezlda_ctx *c, const uint16_t *zin) {
+    if (int r = need_loaded(c)) return r;
+    std::vector<uint16_t> z(c->s.N);
+    for (size_t i = 0; i < z.size(); ++i) {
+        if (zin[i] >= c->s.K) return fail(c, EZLDA_EINVAL, "topic out of range");
+        z[c->doc_tok[i]] = zin[i];
+    }
+    EZ_TRY_HIP(c, hipMemcpy(c->s.z, z.data(), z.size() * sizeof(uint16_t), hipMemcpyHostToDevice));
+    c->counts_fresh = false;
+    return EZLDA_OK;
+}
+
+int ezlda_set_sweep(ezlda_ctx *c, uint32_t sweep) {
+    if (int r = need_loaded(c)) return r;
+    EZ_TRY_HIP(c, ez::launch_set_u32(c->s.counters, sweep, c->st));
+    EZ_TRY_HIP(c, hipStreamSynchronize(c->st));
+    c->sweep = sweep;
+    return EZLDA_OK;
+}
+
+int ezlda_get_nk(ezlda_ctx *c, int32_t *nk) {
+    if (int r = need_loaded(c)) return r;
+    EZ_TRY_HIP(c, hipMemcpy(nk, c->s.nk, sizeof(int32_t) * c->s.K, hipMemcpyDeviceToHost));
+    return EZLDA_OK;
+}
+
+int ezlda_get_inv(ezlda_ctx *c, float *inv) {
+    if (int r = need_loaded(c)) return r;
+    EZ_TRY_HIP(c, hipMemcpy(inv, c->s.inv, sizeof(float) * c->s.K, hipMemcpyDeviceToHost));
+    return EZLDA_OK;
+}
+
+int ezlda_get_wt_dense(ezlda_ctx *c, int32_t *wt) {
+    if (int r = need_loaded(c)) return r;
+    const ez::DevState &s = c->s;
+    std::vector<int32_t> head((size_t)s.VD * s.K);
+    std::vector<uint32_t> toff(s.V - s.VD + 1), tnnz(s.V - s.VD), tail(c->tail_cap);
+    if (!head.empty()) EZ_TRY_HIP(c, hipMemcpy(head.data(), s.wt_head, head.size() * 4, hipMemcpyDeviceToHost));
+    EZ_TRY_HIP(c, hipMemcpy(toff.data(), s.tail_off, toff.size() * 4, hipMemcpyDeviceToHost));
+    if (!tnnz.empty()) EZ_TRY_HIP(c, hipMemcpy(tnnz.data(), s.tail_nnz, tnnz.size() * 4, hipMemcpyDeviceToHost));
+    if (!tail.empty()) EZ_TRY_HIP(c, hipMemcpy(tail.data(), s.wt_tail, tail.size() * 4, hipMemcpyDeviceToHost));
+    std::memset(wt, 0, sizeof(int32_t) * (size_t)s.V * s.K);
+    for (int w = 0; w < s.V; ++w) {
+        const int r = c->rank_of[w];
+        int32_t *out = wt + (size_t)w * s.K;
+        if (r < s.VD) {
+            std::memcpy(out, head.data() + (size_t)r * s.K, sizeof(int32_t) * s.K);
+        } else {
+            const int tw = r - s.VD;
+            for (uint32_t i = 0; i < tnnz[tw]; ++i) {
+                const uint32_t e = tail[toff[tw] + i];
+                out[e & 0xFFFFu] = (int32_t)(e >> 16);
+            }
+        }
+    }
+    return EZLDA_OK;
+}
+
+int ezlda_get_dt_raw(ezlda_ctx *c, uint32_t *entries, int32_t *nnz) {
+    if (int r = need_loaded(c)) return r;
+    const ez::DevState &s = c->s;
+    std::vector<uint2> di(s.D);
+    std::vector<uint32_t> dt(c->dt_cap);
+    if (s.D) EZ_TRY_HIP(c, hipMemcpy(di.data(), s.doc_info, di.size() * sizeof(uint2), hipMemcpyDeviceToHost));
+    if (!dt.empty()) EZ_TRY_HIP(c, hipMemcpy(dt.data(), s.dt, dt.size() * 4, hipMemcpyDeviceToHost));
+    for (int d = 0; d < s.D; ++d) {
+        nnz[d] = (int32_t)(di[d].y & 0xFFFFu);
+        std::memcpy(entries + di[d].x, dt.data() + di[d].x, sizeof(uint32_t) * nnz[d]);
+    }
+    return EZLDA_OK;
+}
+
+int ezlda_get_top(ezlda_ctx *c, uint16_t *top) {
+    if (int r = need_loaded(c)) return r;
+    std::vector<ushort4> t(c->s.V);
+    EZ_TRY_HIP(c, hipMemcpy(t.data(), c->s.wtop, t.size() * sizeof(ushort4), hipMemcpyDeviceToHost));
+    for (int w = 0; w < c->s.V; ++w) {
+        const ushort4 &x = t[c->rank_of[w]];
+        top[3 * (size_t)w] = x.x;
+        top[3 * (size_t)w + 1] = x.y;
+        top[3 * (size_t)w + 2] = x.z;
+    }
+    return EZLDA_OK;
+}
+
+int ezlda_get_aux(ezlda_ctx *c, uint32_t *aux) {
+    if (int r = need_loaded(c)) return r;
+    std::vector<uint32_t> a(c->s.N);
+    EZ_TRY_HIP(c, hipMemcpy(a.data(), c->s.aux, a.size() * 4, hipMemcpyDeviceToHost));
+    for (size_t i = 0; i < a.size(); ++i) aux[i] = a[c->doc_tok[i]];
+    return EZLDA_OK;
+}
+
+int ezlda_selftest_scan64(const float *in, float *out, int32_t nw) {
+    if (nw < 1 || !in || !out) return fail(nullptr, EZLDA_EINVAL, "bad selftest arguments");
+    float *di = nullptr, *dout = nullptr;
+    const size_t bytes = (size_t)nw * 64 * sizeof(float);
+    EZ_TRY_HIP(nullptr, hipMalloc(&di, bytes));
+    EZ_TRY_HIP(nullptr, hipMalloc(&dout, bytes));
+    EZ_TRY_HIP(nullptr, hipMemcpy(di, in, bytes, hipMemcpyHostToDevice));
+    EZ_TRY_HIP(nullptr, ez::launch_selftest_scan(di, dout, nw, nullptr));
+    EZ_TRY_HIP(nullptr, hipMemcpy(out, dout, bytes, hipMemcpyDeviceToHost));
+    hipFree(di);
+    hipFree(dout);
+    return EZLDA_OK;
+}
+
+int ezlda_selftest_philox(const uint32_t *ctr, const uint32_t *key, uint32_t *out, int32_t n) {
+    if (n < 1 || !ctr || !key || !out) return fail(nullptr, EZLDA_EINVAL, "bad selftest arguments");
+    uint32_t *dc = nullptr, *dk = nullptr, *dout = nullptr;
+    EZ_TRY_HIP(nullptr, hipMalloc(&dc, (size_t)n * 16));
+    EZ_TRY_HIP(nullptr, hipMalloc(&dk, (size_t)n * 8));
+    EZ_TRY_HIP(nullptr, hipMalloc(&dout, (size_t)n * 16));
+    EZ_TRY_HIP(nullptr, hipMemcpy(dc, ctr, (size_t)n * 16, hipMemcpyHostToDevice));
+    EZ_TRY_HIP(nullptr, hipMemcpy(dk, key, (size_t)n * 8, hipMemcpyHostToDevice));
+    EZ_TRY_HIP(nullptr, ez::launch_selftest_philox(dc, dk, dout, n, nullptr));
+    EZ_TRY_HIP(nullptr, hipMemcpy(out, dout, (size_t)n * 16, hipMemcpyDeviceToHost));
+    hipFree(dc);
+    hipFree(dk);
+    hipFree(dout);
+    return EZLDA_OK;
+}
+
+// Checkpoint: "EZLDACK1", then int64 K, V, D, N, uint64 seed, uint32 sweep,
+// then z (doc-major, uint16).  Restoring z and the sweep counter restores
+// the whole sampler state: counts are rebuilt from z and the draws are keyed
+// by (seed, token, sweep).
+int ezlda_save_checkpoint(ezlda_ctx *c, const char *path) {
+    if (int r = need_loaded(c)) return r;
+    std::vector<uint16_t> z(c->s.N);
+    if (int r = ezlda_get_topics(c, z.data())) return r;
+    FILE *f = std::fopen(path, "wb");
+    if (!f) return fail(c, EZLDA_EIO, std::string("cannot write ") + path);
+    const int64_t hdr[4] = {c->s.K, c->s.V, c->s.D, (int64_t)c->s.N};
+    const uint64_t seed = c->a.seed;
+    const uint32_t sweep = c->sweep;
+    bool ok = std::fwrite("EZLDACK1", 1, 8, f) == 8 && std::fwrite(hdr, sizeof hdr, 1, f) == 1 &&
+              std::fwrite(&seed, sizeof seed, 1, f) == 1 && std::fwrite(&sweep, sizeof sweep, 1, f) == 1 &&
+              (z.empty() || std::fwrite(z.data(), 2, z.size(), f) == z.size());
+    ok = (std::fclose(f) == 0) && ok;
+    return ok ? EZLDA_OK : fail(c, EZLDA_EIO, std::string("short write to ") + path);
+}
+
+int ezlda_load_checkpoint(ezlda_ctx *c, const char *path) {
+    if (int r = need_loaded(c)) return r;
+    FILE *f = std::fopen(path, "rb");
+    if (!f) return fail(c, EZLDA_EIO, std::string("cannot read ") + path);
+    char magic[8];
+    int64_t hdr[4];
+    uint64_t seed = 0;
+    uint32_t sweep = 0;
+    bool ok = std::fread(magic, 1, 8, f) == 8 && std::memcmp(magic, "EZLDACK1", 8) == 0 &&
+              std::fread(hdr, sizeof hdr, 1, f) == 1 && std::fread(&seed, sizeof seed, 1, f) == 1 &&
+              std::fread(&sweep, sizeof sweep, 1, f) == 1;
+    if (!ok) {
+        std::fclose(f);
+        return fail(c, EZLDA_EIO, std::string("not an ezlda checkpoint: ") + path);
+    }
+    if (hdr[0] != c->s.K || hdr[1] != c->s.V || hdr[2] != c->s.D || hdr[3] != (int64_t)c->s.N || seed != c->a.seed) {
+        std::fclose(f);
+        return fail(c, EZLDA_EIO, "checkpoint does not match the loaded corpus, K or seed");
+    }
+    std::vector<uint16_t> z(c->s.N);
+    ok = z.empty() || std::fread(z.data(), 2, z.size(), f) == z.size();
+    std::fclose(f);
+    if (!ok) return fail(c, EZLDA_EIO, std::string("truncated checkpoint: ") + path);
+    if (int r = ezlda_set_topics(c, z.data())) return r;
+    return ezlda_set_sweep(c, sweep);
+}
+
+}  // extern "C"
