@@ -78,12 +78,14 @@ def cpu_baseline(corpus, K: int, seed: int, max_docs: int, sweeps: int = 2):
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--steps", type=int, default=190)
     ap.add_argument("--warmup", type=int, default=10)
     ap.add_argument("--workload", default="cfg2", choices=sorted(WORKLOADS))
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-docs", type=int, default=30000)
     ap.add_argument("--no-graph", action="store_true")
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
+                    help="only run the plain copy kernel over BYTES (to scale HBM counters) and exit")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU: the Gibbs sweep here is single-GPU (--gpus {args.gpus} is not "
@@ -91,6 +93,12 @@ def main() -> int:
         return 2
 
     from lda_sc20_amd import ezlda
+
+    if args.copy_calibration:
+        ms = ezlda.selftest_copy(args.copy_calibration, 10)
+        print(json.dumps({"copy_bytes": args.copy_calibration, "copies": 11, "ms_per_copy": ms,
+                          "GBs": 2 * args.copy_calibration / ms / 1e6}), flush=True)
+        return 0
 
     wl = WORKLOADS[args.workload]
     t = time.perf_counter()

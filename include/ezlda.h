@@ -165,6 +165,10 @@ int ezlda_get_inv(ezlda_ctx *ctx, float *inv);
 int ezlda_selftest_scan64(const float *in, float *out, int32_t num_waves);
 /* out[4i..4i+3] = Philox4x32-10(ctr[4i..4i+3], key[2i..2i+1]). */
 int ezlda_selftest_philox(const uint32_t *ctr, const uint32_t *key, uint32_t *out, int32_t n);
+/* Streams `bytes` from one device buffer to another `reps` times with a
+ * plain copy kernel; *ms = mean time per copy.  The known-size workload
+ * that scales hardware traffic counters. */
+int ezlda_selftest_copy(int64_t bytes, int32_t reps, double *ms);
 
 /* ---- checkpoint (the state is z plus the sweep counter) */
 int ezlda_save_checkpoint(ezlda_ctx *ctx, const char *path);

@@ -58,6 +58,7 @@ hipError_t launch_set_u32(uint32_t *p, uint32_t v, hipStream_t st);
 
 hipError_t launch_selftest_scan(const float *in, float *out, int nwaves, hipStream_t st);
 hipError_t launch_selftest_philox(const uint32_t *ctr, const uint32_t *key, uint32_t *out, int n, hipStream_t st);
+hipError_t launch_copy(const void *src, void *dst, size_t bytes, hipStream_t st);
 
 // Bytes of dynamic LDS the sampler needs for K topics and block size bd.
 size_t sample_lds_bytes(int K, int bd);

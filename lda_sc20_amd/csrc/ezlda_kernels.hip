@@ -55,6 +55,15 @@ __device__ __forceinline__ void wave_lds_sync() {
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
+static int grid_for(size_t n, int bd, int cap) {
+    size_t g = (n + (size_t)bd - 1) / (size_t)bd;
+    if (g < 1) g = 1;
+    if (g > (size_t)cap) g = (size_t)cap;
+    return (int)g;
+}
+
+static size_t wave_hist_bytes(int K) { return (size_t)((K + 1) >> 1) * sizeof(uint32_t); }
+
 // 16-bit counters, two per LDS word (counts < 65536 by construction).
 __device__ __forceinline__ void h16_add(uint32_t *h, uint32_t k) {
     atomicAdd(&h[k >> 1], 1u << ((k & 1u) << 4));
@@ -279,6 +288,29 @@ __global__ void k_dt_build(int D, int K, const uint32_t *__restrict__ doc_off, c
 }
 
 // ------------------------------------------------------------- sampler
+// First four lane-strided DT entries of a row (0 beyond its end).
+struct Row4 {
+    uint32_t e0, e1, e2, e3;
+};
+
+// Branch-free, so the loads of two rows can be in flight together: indices
+// past the end read the row's last entry (row[0] when nnz == 0; the caller
+// passes a row with at least one entry) and are masked to 0.
+__device__ __forceinline__ Row4 load_row4(const uint32_t *__restrict__ row, uint32_t nnz, int lane) {
+    const uint32_t l = (uint32_t)lane;
+    const uint32_t last = nnz ? nnz - 1u : 0u;
+    const uint32_t a0 = row[min(l, last)];
+    const uint32_t a1 = row[min(l + 64u, last)];
+    const uint32_t a2 = row[min(l + 128u, last)];
+    const uint32_t a3 = row[min(l + 192u, last)];
+    Row4 r;
+    r.e0 = l < nnz ? a0 : 0u;
+    r.e1 = l + 64u < nnz ? a1 : 0u;
+    r.e2 = l + 128u < nnz ? a2 : 0u;
+    r.e3 = l + 192u < nnz ? a3 : 0u;
+    return r;
+}
+
 // One workgroup per sampler item (a word, or a token segment of a head
 // word): phi_w in LDS with phi[k1] = 0, the Q tree over alpha*phi, then each
 // wavefront takes 64 tokens at a time, one lane per token for the shortcut
@@ -301,8 +333,6 @@ __global__ void __launch_bounds__(1024) k_sample(
     float *phi = smf;             // NS*64
     float *qtot = phi + NS * 64;  // NS
     float *qacc = qtot + NS;      // NS
-    float *stot = qacc + NS + wv * 2 * NS;
-    float *sacc = stot + NS;
     const uint4 sg = segs[blockIdx.x];
     const int w = (int)sg.x;
     const ushort4 tp = wtop[w];
@@ -369,31 +399,43 @@ __global__ void __launch_bounds__(1024) k_sample(
         }
         uint32_t newz = k1;
         uint64_t todo = __ballot(need);
-        while (todo) {
-            const int L = first_lane(todo);
+        // Full draws one token at a time; the first 256 DT entries of the
+        // next token's row are loaded while the current token is drawn.
+        int L = todo ? first_lane(todo) : -1;
+        uint32_t off = 0, nnz = 0;
+        Row4 cur{0u, 0u, 0u, 0u};
+        if (L >= 0) {
+            off = readlane_u(di.x, L);
+            nnz = readlane_u(di.y, L) & 0xFFFFu;
+            cur = load_row4(dt + off, nnz, lane);
+        }
+        while (L >= 0) {
             todo &= todo - 1ull;
-            const uint32_t off = readlane_u(di.x, L);
-            const uint32_t nnz = readlane_u(di.y, L) & 0xFFFFu;
+            const int L2 = todo ? first_lane(todo) : -1;
+            // no next token: re-read the current row's first entry, masked
+            uint32_t off2 = off, nnz2 = 0;
+            if (L2 >= 0) {
+                off2 = readlane_u(di.x, L2);
+                nnz2 = readlane_u(di.y, L2) & 0xFFFFu;
+            }
+            const Row4 nxt = load_row4(dt + off2, nnz2, lane);
             const float uL = readlane_f(u, L), P1L = readlane_f(P1, L);
-            const int nss = (int)((nnz + 63u) >> 6);
             n_full += 1u;
             n_scanned += nnz;
-            float S = 0.0f;
-            for (int j = 0; j < nss; ++j) {
-                const uint32_t i = (uint32_t)j * 64u + lane;
-                float p = 0.0f;
-                if (i < nnz) {
-                    const uint32_t e = dt[off + i];
-                    p = (float)(e >> 16) * phi[e & 0xFFFFu];
-                }
-                const float tot = readlane_f(wave_scan(p), 63);
-                S = S + tot;
-                if (lane == 0) {
-                    stot[j] = tot;
-                    sacc[j] = S;
-                }
+            // S bucket, lane-major: lane l sums entries l, l+64, ... in order
+            // (a zero entry beyond the row adds +0 and changes nothing)
+            const uint32_t *row = dt + off;
+            float acc = 0.0f;
+            acc = acc + (float)(cur.e0 >> 16) * phi[cur.e0 & 0xFFFFu];
+            acc = acc + (float)(cur.e1 >> 16) * phi[cur.e1 & 0xFFFFu];
+            acc = acc + (float)(cur.e2 >> 16) * phi[cur.e2 & 0xFFFFu];
+            acc = acc + (float)(cur.e3 >> 16) * phi[cur.e3 & 0xFFFFu];
+            for (uint32_t i = (uint32_t)lane + 256u; i < nnz; i += 64u) {
+                const uint32_t e = row[i];
+                acc = acc + (float)(e >> 16) * phi[e & 0xFFFFu];
             }
-            wave_lds_sync();
+            const float sc = wave_scan(acc);
+            const float S = readlane_f(sc, 63);
             const float A = P1L + S;
             const float total = A + Q;
             const float target = uL * total;
@@ -402,35 +444,48 @@ __global__ void __launch_bounds__(1024) k_sample(
                 zz = k1;
             } else if (target < A) {
                 const float t2 = target - P1L;
-                int js = -1;
-                for (int j0 = 0; j0 < nss && js < 0; j0 += 64) {
-                    const int j = j0 + lane;
-                    const uint64_t m = __ballot(j < nss && sacc[j] > t2);
-                    if (m) js = j0 + first_lane(m);
+                const uint64_t m = __ballot(sc > t2 && acc > 0.0f);
+                int ls = m ? first_lane(m) : -1;
+                if (ls < 0) {
+                    const uint64_t m2 = __ballot(acc > 0.0f);
+                    ls = m2 ? last_lane(m2) : -1;
                 }
-                if (js < 0)
-                    for (int j0 = ((nss - 1) >> 6) << 6; j0 >= 0 && js < 0; j0 -= 64) {
-                        const int j = j0 + lane;
-                        const uint64_t m = __ballot(j < nss && stot[j] > 0.0f);
-                        if (m) js = j0 + last_lane(m);
+                if (ls >= 0) {
+                    const float t3 = t2 - (ls > 0 ? readlane_f(sc, ls - 1) : 0.0f);
+                    // walk lane ls's entries ls, ls+64, ... left to right; lane j
+                    // of each pass holds entry ls + 64*(pass*64 + j)
+                    const int cnt = (int)((nnz - (uint32_t)ls + 63u) >> 6);
+                    float run = 0.0f;
+                    int sel = -1;
+                    uint32_t sel_e = 0, last_e = 0;
+                    bool have_last = false;
+                    for (int j0 = 0; j0 < cnt && sel < 0; j0 += 64) {
+                        const uint32_t i = (uint32_t)ls + 64u * (uint32_t)(j0 + lane);
+                        uint32_t e = 0;
+                        float p = 0.0f;
+                        if (i < nnz) {
+                            e = row[i];
+                            p = (float)(e >> 16) * phi[e & 0xFFFFu];
+                        }
+                        const int n = min(64, cnt - j0);
+                        for (int j = 0; j < n; ++j) {
+                            const float pj = readlane_f(p, j);
+                            run = run + pj;
+                            if (pj > 0.0f) {
+                                if (run > t3) {
+                                    sel = j;
+                                    sel_e = readlane_u(e, j);
+                                    break;
+                                }
+                                last_e = readlane_u(e, j);
+                                have_last = true;
+                            }
+                        }
                     }
-                if (js >= 0) {
-                    const float t3 = t2 - (js > 0 ? sacc[js - 1] : 0.0f);
-                    const uint32_t i = (uint32_t)js * 64u + lane;
-                    float p = 0.0f;
-                    uint32_t e = 0;
-                    if (i < nnz) {
-                        e = dt[off + i];
-                        p = (float)(e >> 16) * phi[e & 0xFFFFu];
-                    }
-                    const float sc = wave_scan(p);
-                    uint64_t m = __ballot(i < nnz && sc > t3 && p > 0.0f);
-                    int ls = m ? first_lane(m) : -1;
-                    if (ls < 0) {
-                        const uint64_t m2 = __ballot(p > 0.0f);
-                        ls = m2 ? last_lane(m2) : -1;
-                    }
-                    if (ls >= 0) zz = readlane_u(e, ls) & 0xFFFFu;
+                    if (sel >= 0)
+                        zz = sel_e & 0xFFFFu;
+                    else if (have_last)
+                        zz = last_e & 0xFFFFu;
                 }
             } else {
                 const float t2 = target - A;
@@ -461,7 +516,10 @@ __global__ void __launch_bounds__(1024) k_sample(
                 }
             }
             if (lane == L) newz = zz;
-            wave_lds_sync();
+            L = L2;
+            off = off2;
+            nnz = nnz2;
+            cur = nxt;
         }
         if (valid) z[t] = (uint16_t)newz;
     }
@@ -565,6 +623,20 @@ __global__ void k_selftest_philox(const uint4 *__restrict__ ctr, const uint2 *__
     out[i] = make_uint4(o.x, o.y, o.z, o.w);
 }
 
+// Plain streaming copy (16 B per lane per step): the known-size reference
+// for scaling HBM counters.
+__global__ void k_copy_u4(const uint4 *__restrict__ src, uint4 *__restrict__ dst, size_t n) {
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
+        dst[i] = src[i];
+}
+
+hipError_t launch_copy(const void *src, void *dst, size_t bytes, hipStream_t st) {
+    const size_t n = bytes / 16;
+    hipLaunchKernelGGL(k_copy_u4, dim3(grid_for(n, 256, 8192)), dim3(256), 0, st,
+                       reinterpret_cast<const uint4 *>(src), reinterpret_cast<uint4 *>(dst), n);
+    return hipGetLastError();
+}
+
 hipError_t launch_selftest_scan(const float *in, float *out, int nwaves, hipStream_t st) {
     hipLaunchKernelGGL(k_selftest_scan, dim3(nwaves), dim3(64), 0, st, in, out, nwaves * 64);
     return hipGetLastError();
@@ -578,14 +650,6 @@ hipError_t launch_selftest_philox(const uint32_t *ctr, const uint32_t *key, uint
 }
 
 // ------------------------------------------------------------ launchers
-static int grid_for(size_t n, int bd, int cap) {
-    size_t g = (n + (size_t)bd - 1) / (size_t)bd;
-    if (g < 1) g = 1;
-    if (g > (size_t)cap) g = (size_t)cap;
-    return (int)g;
-}
-
-static size_t wave_hist_bytes(int K) { return (size_t)((K + 1) >> 1) * sizeof(uint32_t); }
 
 hipError_t launch_set_u32(uint32_t *p, uint32_t v, hipStream_t st) {
     hipLaunchKernelGGL(k_set_u32, dim3(1), dim3(1), 0, st, p, v);
@@ -635,7 +699,8 @@ hipError_t launch_build_dt(const DevState &s, hipStream_t st) {
 
 size_t sample_lds_bytes(int K, int bd) {
     const int NS = (K + 63) >> 6;
-    return ((size_t)NS * 64 + 2 * (size_t)NS + (size_t)(bd >> 6) * 2 * NS) * sizeof(float);
+    (void)bd;
+    return ((size_t)NS * 64 + 2 * (size_t)NS) * sizeof(float);
 }
 
 hipError_t launch_sample(const DevState &s, bool shortcut, hipStream_t st) {

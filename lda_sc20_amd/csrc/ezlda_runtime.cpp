@@ -701,6 +701,32 @@ int ezlda_selftest_philox(const uint32_t *ctr, const uint32_t *key, uint32_t *ou
     return EZLDA_OK;
 }
 
+int ezlda_selftest_copy(int64_t bytes, int32_t reps, double *ms) {
+    if (bytes < 16 || reps < 1) return fail(nullptr, EZLDA_EINVAL, "bad copy calibration arguments");
+    bytes &= ~(int64_t)15;
+    void *a = nullptr, *b = nullptr;
+    EZ_TRY_HIP(nullptr, hipMalloc(&a, (size_t)bytes));
+    EZ_TRY_HIP(nullptr, hipMalloc(&b, (size_t)bytes));
+    EZ_TRY_HIP(nullptr, hipMemset(a, 1, (size_t)bytes));
+    EZ_TRY_HIP(nullptr, hipDeviceSynchronize());
+    hipEvent_t e0, e1;
+    EZ_TRY_HIP(nullptr, hipEventCreate(&e0));
+    EZ_TRY_HIP(nullptr, hipEventCreate(&e1));
+    EZ_TRY_HIP(nullptr, ez::launch_copy(a, b, (size_t)bytes, nullptr));
+    EZ_TRY_HIP(nullptr, hipEventRecord(e0, nullptr));
+    for (int i = 0; i < reps; ++i) EZ_TRY_HIP(nullptr, ez::launch_copy(a, b, (size_t)bytes, nullptr));
+    EZ_TRY_HIP(nullptr, hipEventRecord(e1, nullptr));
+    EZ_TRY_HIP(nullptr, hipEventSynchronize(e1));
+    float t = 0.f;
+    hipEventElapsedTime(&t, e0, e1);
+    if (ms) *ms = t / reps;
+    hipEventDestroy(e0);
+    hipEventDestroy(e1);
+    hipFree(a);
+    hipFree(b);
+    return EZLDA_OK;
+}
+
 // Checkpoint: "EZLDACK1", then int64 K, V, D, N, uint64 seed, uint32 sweep,
 // then z (doc-major, uint16).  Restoring z and the sweep counter restores
 // the whole sampler state: counts are rebuilt from z and the draws are keyed

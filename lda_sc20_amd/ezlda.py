@@ -28,7 +28,7 @@ EXPORTS = (
     "ezlda_load_checkpoint", "ezlda_gen_corpus", "ezlda_corpus_num_tokens", "ezlda_corpus_num_docs",
     "ezlda_corpus_num_words", "ezlda_corpus_doc_off", "ezlda_corpus_words", "ezlda_corpus_topics",
     "ezlda_corpus_free", "ezlda_corpus_from_arrays", "ezlda_read_docword", "ezlda_write_chunked",
-    "ezlda_read_chunked", "ezlda_selftest_scan64", "ezlda_selftest_philox", "ezlda_get_stats",
+    "ezlda_read_chunked", "ezlda_selftest_scan64", "ezlda_selftest_philox", "ezlda_selftest_copy", "ezlda_get_stats",
     "ezlda_reset_stats",
 )
 
@@ -114,6 +114,7 @@ def lib() -> C.CDLL:
         "ezlda_read_chunked": (P, [C.c_char_p, C.c_int32]),
         "ezlda_selftest_scan64": (C.c_int, [P, P, C.c_int32]),
         "ezlda_selftest_philox": (C.c_int, [P, P, P, C.c_int32]),
+        "ezlda_selftest_copy": (C.c_int, [C.c_int64, C.c_int32, C.POINTER(C.c_double)]),
         "ezlda_get_stats": (C.c_int, [P, P]),
         "ezlda_reset_stats": (C.c_int, [P]),
     }
@@ -373,6 +374,15 @@ def selftest_philox(ctr: np.ndarray, key: np.ndarray) -> np.ndarray:
     if r:
         raise EzldaError(r, last_error())
     return out
+
+
+def selftest_copy(nbytes: int, reps: int = 10) -> float:
+    """Mean ms of a plain device-to-device copy kernel over nbytes."""
+    ms = C.c_double()
+    r = lib().ezlda_selftest_copy(nbytes, reps, C.byref(ms))
+    if r:
+        raise EzldaError(r, last_error())
+    return ms.value
 
 
 def dt_offsets(doc_off: Sequence[int], K: int) -> np.ndarray:

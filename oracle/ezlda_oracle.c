@@ -36,10 +36,12 @@
  *      (replaces curand_uniform, utility.cu:1353).  target = u * total; the
  *      first element of positive mass whose inclusive prefix exceeds the
  *      target wins (a tree scan is not monotone in f32, so the mass test
- *      keeps zero-mass elements out).  Sums use the wave64 scan tree of
- *      scan64() per 64-element slab, slabs summed left to right.  When
- *      rounding leaves no prefix above the target, the last element of
- *      positive mass wins.
+ *      keeps zero-mass elements out).  S is summed lane-major: DT entry i
+ *      goes to lane i mod 64, each lane adds its entries left to right and
+ *      the 64 lane sums go through the wave64 scan tree of scan64(); the Q
+ *      bucket is scanned per 64-topic slab and the slabs summed left to
+ *      right.  When rounding leaves no prefix above the target, the last
+ *      element of positive mass wins.
  *   6. max-topic shortcut (UpdateProbKernelTrainD1, utility.cu:2680-3312;
  *      inline tail test utility.cu:1341-1373): accept k1 when
  *      u * (P1 + maxS + Q) < P1 * (1 - 2^-10), maxS = n_dk2 phi_k2 +
@@ -321,54 +323,51 @@ void orc_build_dt(void *h) {
 
 /* ------------------------------------------------------- 5-6. sampler */
 /* Full three-bucket draw for one token (LDAKernelTrain(D), utility.cu:
- * 1217-1611, :1625-2103; bucket search :1878-1992).  phi has phi[k1] = 0. */
+ * 1217-1611, :1625-2103; bucket search :1878-1992).  phi has phi[k1] = 0.
+ * S bucket in lane-major order: lane l (0..63) sums the products of DT
+ * entries l, l+64, l+128, ... left to right; the 64 lane sums go through
+ * scan64(); the draw picks the lane whose prefix first exceeds the target,
+ * then walks that lane's entries in the same left-to-right order. */
 static int draw_full(const orc_t *o, const float *phi, const float *qtot, const float *qacc,
-                     int nsq, int k1, float P1, float Q, float u, int d,
-                     float *stot, float *sacc) {
+                     int nsq, int k1, float P1, float Q, float u, int d) {
     const uint32_t *row = o->dt + o->dt_off[d];
     const int nnz = o->dt_nnz[d];
-    const int nss = (nnz + 63) / 64;
-    float v[64];
-    float S = 0.0f;
-    for (int j = 0; j < nss; ++j) {
-        for (int l = 0; l < 64; ++l) {
-            int i = 64 * j + l;
-            v[l] = 0.0f;
-            if (i < nnz) v[l] = (float)(row[i] >> 16) * phi[row[i] & 0xFFFFu];
-        }
-        orc_scan64(v);
-        stot[j] = v[63];
-        S = S + v[63];
-        sacc[j] = S;
+    float a[64], sc[64];
+    for (int l = 0; l < 64; ++l) {
+        float acc = 0.0f;
+        for (int i = l; i < nnz; i += 64) acc = acc + (float)(row[i] >> 16) * phi[row[i] & 0xFFFFu];
+        a[l] = acc;
+        sc[l] = acc;
     }
+    orc_scan64(sc);
+    const float S = sc[63];
     const float A = P1 + S;
     const float total = A + Q;
     const float target = u * total;
     if (target < P1) return k1;
     if (target < A) {
         const float t2 = target - P1;
-        int js = -1;
-        for (int j = 0; j < nss; ++j)
-            if (sacc[j] > t2) { js = j; break; }
-        if (js < 0)
-            for (int j = nss - 1; j >= 0; --j)
-                if (stot[j] > 0.0f) { js = j; break; }
-        if (js < 0) return k1;
-        const float t3 = t2 - (js > 0 ? sacc[js - 1] : 0.0f);
-        float p[64];
-        for (int l = 0; l < 64; ++l) {
-            int i = 64 * js + l;
-            p[l] = 0.0f;
-            if (i < nnz) p[l] = (float)(row[i] >> 16) * phi[row[i] & 0xFFFFu];
-            v[l] = p[l];
-        }
-        orc_scan64(v);
+        int ls = -1;
         for (int l = 0; l < 64; ++l)
-            if (64 * js + l < nnz && v[l] > t3 && p[l] > 0.0f) return (int)(row[64 * js + l] & 0xFFFFu);
-        for (int l = 63; l >= 0; --l)
-            if (p[l] > 0.0f) return (int)(row[64 * js + l] & 0xFFFFu);
-        return k1;
+            if (sc[l] > t2 && a[l] > 0.0f) { ls = l; break; }
+        if (ls < 0)
+            for (int l = 63; l >= 0; --l)
+                if (a[l] > 0.0f) { ls = l; break; }
+        if (ls < 0) return k1;
+        const float t3 = t2 - (ls > 0 ? sc[ls - 1] : 0.0f);
+        float run = 0.0f;
+        int last = -1;
+        for (int i = ls; i < nnz; i += 64) {
+            const float p = (float)(row[i] >> 16) * phi[row[i] & 0xFFFFu];
+            run = run + p;
+            if (p > 0.0f) {
+                if (run > t3) return (int)(row[i] & 0xFFFFu);
+                last = i;
+            }
+        }
+        return last >= 0 ? (int)(row[last] & 0xFFFFu) : k1;
     }
+    float v[64];
     const float t2 = target - A;
     int js = -1;
     for (int j = 0; j < nsq; ++j)
@@ -402,8 +401,6 @@ void orc_sample(void *h) {
         float *phi = (float *)xcalloc(K, sizeof(float));
         float *qtot = (float *)xcalloc(nsq, sizeof(float));
         float *qacc = (float *)xcalloc(nsq, sizeof(float));
-        float *stot = (float *)xcalloc(nsq, sizeof(float));
-        float *sacc = (float *)xcalloc(nsq, sizeof(float));
         float v[64];
 #pragma omp for schedule(dynamic, 16)
         for (int w = 0; w < o->V; ++w) {
@@ -442,10 +439,10 @@ void orc_sample(void *h) {
                         continue;
                     }
                 }
-                o->z[t] = (uint16_t)draw_full(o, phi, qtot, qacc, nsq, k1, P1, Q, u, d, stot, sacc);
+                o->z[t] = (uint16_t)draw_full(o, phi, qtot, qacc, nsq, k1, P1, Q, u, d);
             }
         }
-        free(phi); free(qtot); free(qacc); free(stot); free(sacc);
+        free(phi); free(qtot); free(qacc);
     }
     o->sweep++;
     o->counts_fresh = 0;

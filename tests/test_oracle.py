@@ -217,11 +217,19 @@ def _fp64_draw(u, k1, P1, s_terms, q_terms):
     return labels[i], gap
 
 
-@pytest.mark.parametrize("K,shortcut", [(7, True), (64, True), (64, False), (100, True)])
-def test_sampler_matches_fp64_inverse_cdf(K, shortcut):
+def _lane_major(entries):
+    """DT entries (ascending topic) in the S bucket's lane-major order:
+    entry i goes to lane i mod 64, lanes in order, each lane left to right."""
+    idx = sorted(range(len(entries)), key=lambda i: (i % 64, i // 64))
+    return [entries[i] for i in idx]
+
+
+@pytest.mark.parametrize("K,shortcut,mean_len", [(7, True, 30), (64, True, 30), (64, False, 30), (100, True, 30),
+                                                 (300, True, 400), (300, False, 400)])
+def test_sampler_matches_fp64_inverse_cdf(K, shortcut, mean_len):
     """Every draw equals an FP64 inverse CDF over the canonical bucket order
     with the same u, except within rounding distance of a bucket boundary."""
-    off, words = make_corpus(5, 80, 60, 30)
+    off, words = make_corpus(5, 80 if mean_len < 100 else 20, 60, mean_len)
     V = 60
     seed = 99
     o = _oracle(off, words, K, seed=seed, shortcut=shortcut, V=V)
@@ -244,7 +252,8 @@ def test_sampler_matches_fp64_inverse_cdf(K, shortcut):
         k1 = int(top[w, 0])
         u = ob.uniform(seed, int(doc_tok[i]), sweep)
         P1 = (dt[d, k1] + alpha) * phi[w, k1]
-        s_terms = [(k, dt[d, k] * phi[w, k]) for k in range(K) if dt[d, k] > 0 and k != k1]
+        row = [k for k in range(K) if dt[d, k] > 0]
+        s_terms = [(k, dt[d, k] * phi[w, k]) for k in _lane_major(row) if k != k1]
         q_terms = [(k, alpha * phi[w, k]) for k in range(K) if k != k1]
         exp, gap = _fp64_draw(u, k1, P1, s_terms, q_terms)
         if gap < 1e-5:
