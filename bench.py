@@ -84,6 +84,8 @@ def main() -> int:
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-docs", type=int, default=30000)
     ap.add_argument("--no-graph", action="store_true")
+    ap.add_argument("--doc-tiles", type=int, default=0,
+                    help="doc tiles for head-word items (0 auto, <0 off); topics do not depend on it")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
                     help="only run the plain copy kernel over BYTES (to scale HBM counters) and exit")
     args = ap.parse_args()
@@ -106,12 +108,14 @@ def main() -> int:
     log(f"generated {corpus.N} tokens in {time.perf_counter() - t:.1f}s")
     K = wl["K"]
     dense = -1 if wl["head_frac"] is None else int(math.ceil(wl["head_frac"] * corpus.V))
-    ctx = ezlda.Context(K=K, alpha=50.0 / K, beta=0.01, seed=wl["seed"], dense_words=dense, shortcut=1)
+    ctx = ezlda.Context(K=K, alpha=50.0 / K, beta=0.01, seed=wl["seed"], dense_words=dense, shortcut=1,
+                        doc_tiles=args.doc_tiles)
     t = time.perf_counter()
     ctx.load(corpus)
     info = ctx.info()
     log(f"loaded in {time.perf_counter() - t:.1f}s: N={info.num_tokens} V={info.num_words} D={info.num_docs} "
-        f"V_D={info.num_head_words} items={info.num_sample_items} device={info.device_bytes / 2**30:.2f} GiB")
+        f"V_D={info.num_head_words} items={info.num_sample_items} doc_tiles={info.doc_tiles} xcd={info.num_xcd} "
+        f"device={info.device_bytes / 2**30:.2f} GiB")
 
     t = time.perf_counter()
     ctx.sweep(args.warmup)
@@ -163,7 +167,8 @@ def main() -> int:
         "dtype": "f32",
         "data": "synthetic (LDA generative process, Philox-seeded)",
         "config": {"workload": f"{args.workload}: {wl['desc']}", "num_tokens": N, "K": K,
-                   "num_head_words": VD, "shortcut": True, "sweeps_before_timing": args.warmup},
+                   "num_head_words": VD, "shortcut": True, "sweeps_before_timing": args.warmup,
+                   "doc_tiles": int(info.doc_tiles)},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": None, "kernel": "k_sample",
                      "bytes_per_launch": b_sample, "avg_launch_ms": t_sample * 1e3},

@@ -38,7 +38,7 @@ typedef struct ezlda_ctx ezlda_ctx;
  * (EZLDA/Argument.cuh:3-9) plus the constants main() hard-codes
  * (mainKernel.cu:50-51) and Doc.cuh:30 tokenSegment. */
 typedef struct ezlda_args {
-    int32_t K;               /* topics (Argument.cuh:6); 1 <= K <= 65535 */
+    int32_t K;               /* topics (Argument.cuh:6); 1 <= K <= 16383 */
     float alpha;             /* Argument.cuh:9; <= 0 means 50/K */
     float beta;              /* Argument.cuh:8; <= 0 means 0.01 */
     int32_t block_dim;       /* Argument.cuh:4 BlockDim: threads per workgroup
@@ -59,6 +59,10 @@ typedef struct ezlda_args {
     int32_t shortcut;        /* 1 = max-topic shortcut, 0 = plain sampler;
                                 both give identical topics */
     int32_t topic_bits;      /* width of a stored topic; only 16 */
+    int32_t doc_tiles;       /* head-word sampler items cut by doc tiles for
+                                L2 locality, dealt to XCDs: 0 = auto,
+                                > 0 = that many tiles, < 0 = off; topics do
+                                not depend on it */
 } ezlda_args;
 
 typedef struct ezlda_info {
@@ -72,6 +76,8 @@ typedef struct ezlda_info {
     int32_t num_sample_items;/* sampler work items (word segments) */
     uint32_t sweep;          /* sweeps done (the Philox sweep counter) */
     int64_t device_bytes;    /* device memory held by the context */
+    int32_t doc_tiles;       /* doc tiles used for head-word items */
+    int32_t num_xcd;         /* XCDs of the device */
 } ezlda_info;
 
 void ezlda_default_args(ezlda_args *args);

@@ -64,6 +64,14 @@ static int grid_for(size_t n, int bd, int cap) {
 
 static size_t wave_hist_bytes(int K) { return (size_t)((K + 1) >> 1) * sizeof(uint32_t); }
 
+// A DT entry is (topic << 2) | (count << 16): its low half is the byte
+// offset of phi[topic] in a float array, so a gather needs one AND.
+__device__ __forceinline__ uint32_t dt_topic(uint32_t e) { return (e & 0xFFFFu) >> 2; }
+__device__ __forceinline__ float dt_count(uint32_t e) { return (float)(e >> 16); }
+__device__ __forceinline__ float phi_of(const float *phi, uint32_t e) {
+    return *reinterpret_cast<const float *>(reinterpret_cast<const char *>(phi) + (e & 0xFFFCu));
+}
+
 // 16-bit counters, two per LDS word (counts < 65536 by construction).
 __device__ __forceinline__ void h16_add(uint32_t *h, uint32_t k) {
     atomicAdd(&h[k >> 1], 1u << ((k & 1u) << 4));
@@ -276,7 +284,7 @@ __global__ void k_dt_build(int D, int K, const uint32_t *__restrict__ doc_off, c
                 const int k = j + lane;
                 const uint32_t c = k < K ? h16_get(h, (uint32_t)k) : 0u;
                 const uint64_t m = __ballot(c != 0);
-                if (c) dt[base + nz + __popcll(m & lanemask_lt(lane))] = (uint32_t)k | (c << 16);
+                if (c) dt[base + nz + __popcll(m & lanemask_lt(lane))] = ((uint32_t)k << 2) | (c << 16);
                 nz += (uint32_t)__popcll(m);
                 wave_lds_sync();
                 if (k < K && !(k & 1)) h[k >> 1] = 0;
@@ -288,35 +296,29 @@ __global__ void k_dt_build(int D, int K, const uint32_t *__restrict__ doc_off, c
 }
 
 // ------------------------------------------------------------- sampler
+// A DT row as a buffer resource whose range is the row itself: lanes that
+// index past the row read 0 in hardware, with no clamp, mask or branch.
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t row_rsrc(const uint32_t *row, uint32_t nnz) {
+    return __builtin_amdgcn_make_buffer_rsrc(const_cast<uint32_t *>(row), (short)0, (int)(nnz * 4u), 0x00020000);
+}
+__device__ __forceinline__ uint32_t row_ld(__amdgpu_buffer_rsrc_t r, uint32_t byte_off) {
+    return (uint32_t)__builtin_amdgcn_raw_buffer_load_b32(r, (int)byte_off, 0, 0);
+}
+
 // First four lane-strided DT entries of a row (0 beyond its end).
 struct Row4 {
     uint32_t e0, e1, e2, e3;
 };
 
-// Branch-free, so the loads of two rows can be in flight together: indices
-// past the end read the row's last entry (row[0] when nnz == 0; the caller
-// passes a row with at least one entry) and are masked to 0.
-__device__ __forceinline__ Row4 load_row4(const uint32_t *__restrict__ row, uint32_t nnz, int lane) {
-    const uint32_t l = (uint32_t)lane;
-    const uint32_t last = nnz ? nnz - 1u : 0u;
-    const uint32_t a0 = row[min(l, last)];
-    const uint32_t a1 = row[min(l + 64u, last)];
-    const uint32_t a2 = row[min(l + 128u, last)];
-    const uint32_t a3 = row[min(l + 192u, last)];
-    Row4 r;
-    r.e0 = l < nnz ? a0 : 0u;
-    r.e1 = l + 64u < nnz ? a1 : 0u;
-    r.e2 = l + 128u < nnz ? a2 : 0u;
-    r.e3 = l + 192u < nnz ? a3 : 0u;
-    return r;
+__device__ __forceinline__ Row4 load_row4(__amdgpu_buffer_rsrc_t r, uint32_t lane_bytes) {
+    Row4 x;
+    x.e0 = row_ld(r, lane_bytes);
+    x.e1 = row_ld(r, lane_bytes + 256u);
+    x.e2 = row_ld(r, lane_bytes + 512u);
+    x.e3 = row_ld(r, lane_bytes + 768u);
+    return x;
 }
 
-// One workgroup per sampler item (a word, or a token segment of a head
-// word): phi_w in LDS with phi[k1] = 0, the Q tree over alpha*phi, then each
-// wavefront takes 64 tokens at a time, one lane per token for the shortcut
-// test and the whole wavefront per token for the full draw.  Replaces
-// UpdateProbKernelTrainD1 (utility.cu:2680-3312), LDAKernelTrainD
-// (:1625-2103) and LDAKernelTrain (:1217-1611).
 template <bool SHORTCUT>
 __global__ void __launch_bounds__(1024) k_sample(
     const uint4 *__restrict__ segs, int VD, int K, float alpha, float beta, const int32_t *__restrict__ wt_head,
@@ -334,6 +336,7 @@ __global__ void __launch_bounds__(1024) k_sample(
     float *qtot = phi + NS * 64;  // NS
     float *qacc = qtot + NS;      // NS
     const uint4 sg = segs[blockIdx.x];
+    if (sg.x == 0xFFFFFFFFu) return;  // padding item
     const int w = (int)sg.x;
     const ushort4 tp = wtop[w];
     const uint32_t sweep = counters[0];
@@ -401,38 +404,39 @@ __global__ void __launch_bounds__(1024) k_sample(
         uint64_t todo = __ballot(need);
         // Full draws one token at a time; the first 256 DT entries of the
         // next token's row are loaded while the current token is drawn.
+        const uint32_t lane_bytes = (uint32_t)lane * 4u;
         int L = todo ? first_lane(todo) : -1;
         uint32_t off = 0, nnz = 0;
         Row4 cur{0u, 0u, 0u, 0u};
         if (L >= 0) {
             off = readlane_u(di.x, L);
             nnz = readlane_u(di.y, L) & 0xFFFFu;
-            cur = load_row4(dt + off, nnz, lane);
+            cur = load_row4(row_rsrc(dt + off, nnz), lane_bytes);
         }
         while (L >= 0) {
             todo &= todo - 1ull;
             const int L2 = todo ? first_lane(todo) : -1;
-            // no next token: re-read the current row's first entry, masked
-            uint32_t off2 = off, nnz2 = 0;
+            uint32_t off2 = 0, nnz2 = 0;  // no next token: an empty range reads zeros
             if (L2 >= 0) {
                 off2 = readlane_u(di.x, L2);
                 nnz2 = readlane_u(di.y, L2) & 0xFFFFu;
             }
-            const Row4 nxt = load_row4(dt + off2, nnz2, lane);
+            const Row4 nxt = load_row4(row_rsrc(dt + off2, nnz2), lane_bytes);
             const float uL = readlane_f(u, L), P1L = readlane_f(P1, L);
             n_full += 1u;
             n_scanned += nnz;
             // S bucket, lane-major: lane l sums entries l, l+64, ... in order
             // (a zero entry beyond the row adds +0 and changes nothing)
-            const uint32_t *row = dt + off;
-            float acc = 0.0f;
-            acc = acc + (float)(cur.e0 >> 16) * phi[cur.e0 & 0xFFFFu];
-            acc = acc + (float)(cur.e1 >> 16) * phi[cur.e1 & 0xFFFFu];
-            acc = acc + (float)(cur.e2 >> 16) * phi[cur.e2 & 0xFFFFu];
-            acc = acc + (float)(cur.e3 >> 16) * phi[cur.e3 & 0xFFFFu];
-            for (uint32_t i = (uint32_t)lane + 256u; i < nnz; i += 64u) {
-                const uint32_t e = row[i];
-                acc = acc + (float)(e >> 16) * phi[e & 0xFFFFu];
+            off = (uint32_t)__builtin_amdgcn_readfirstlane((int)off);
+            nnz = (uint32_t)__builtin_amdgcn_readfirstlane((int)nnz);
+            const __amdgpu_buffer_rsrc_t rr = row_rsrc(dt + off, nnz);
+            float acc = __builtin_fmaf(dt_count(cur.e0), phi_of(phi, cur.e0), 0.0f);
+            acc = __builtin_fmaf(dt_count(cur.e1), phi_of(phi, cur.e1), acc);
+            acc = __builtin_fmaf(dt_count(cur.e2), phi_of(phi, cur.e2), acc);
+            acc = __builtin_fmaf(dt_count(cur.e3), phi_of(phi, cur.e3), acc);
+            for (uint32_t b = 1024u; b < nnz * 4u; b += 256u) {
+                const uint32_t e = row_ld(rr, lane_bytes + b);
+                acc = __builtin_fmaf(dt_count(e), phi_of(phi, e), acc);
             }
             const float sc = wave_scan(acc);
             const float S = readlane_f(sc, 63);
@@ -460,18 +464,13 @@ __global__ void __launch_bounds__(1024) k_sample(
                     uint32_t sel_e = 0, last_e = 0;
                     bool have_last = false;
                     for (int j0 = 0; j0 < cnt && sel < 0; j0 += 64) {
-                        const uint32_t i = (uint32_t)ls + 64u * (uint32_t)(j0 + lane);
-                        uint32_t e = 0;
-                        float p = 0.0f;
-                        if (i < nnz) {
-                            e = row[i];
-                            p = (float)(e >> 16) * phi[e & 0xFFFFu];
-                        }
+                        const uint32_t e = row_ld(rr, ((uint32_t)ls + 64u * (uint32_t)(j0 + lane)) * 4u);
+                        const float c = dt_count(e), f = phi_of(phi, e);
                         const int n = min(64, cnt - j0);
                         for (int j = 0; j < n; ++j) {
-                            const float pj = readlane_f(p, j);
-                            run = run + pj;
-                            if (pj > 0.0f) {
+                            const float cj = readlane_f(c, j), fj = readlane_f(f, j);
+                            run = __builtin_fmaf(cj, fj, run);
+                            if (cj > 0.0f && fj > 0.0f) {
                                 if (run > t3) {
                                     sel = j;
                                     sel_e = readlane_u(e, j);
@@ -483,9 +482,9 @@ __global__ void __launch_bounds__(1024) k_sample(
                         }
                     }
                     if (sel >= 0)
-                        zz = sel_e & 0xFFFFu;
+                        zz = dt_topic(sel_e);
                     else if (have_last)
-                        zz = last_e & 0xFFFFu;
+                        zz = dt_topic(last_e);
                 }
             } else {
                 const float t2 = target - A;
@@ -547,6 +546,10 @@ __global__ void __launch_bounds__(1024) k_ll(const uint4 *__restrict__ segs, int
     const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6, nwv = blockDim.x >> 6;
     double *phi = smd;
     const uint4 sg = segs[blockIdx.x];
+    if (sg.x == 0xFFFFFFFFu) {  // padding item
+        if (tid == 0) part[blockIdx.x] = 0.0;
+        return;
+    }
     const int w = (int)sg.x;
     if (w < VD) {
         const int32_t *row = wt_head + (size_t)w * K;
@@ -579,7 +582,7 @@ __global__ void __launch_bounds__(1024) k_ll(const uint4 *__restrict__ segs, int
         double s = 0.0;
         for (uint32_t i = lane; i < nnz; i += 64) {
             const uint32_t e = dt[di.x + i];
-            s += (double)(e >> 16) * phi[e & 0xFFFFu];
+            s += (double)(e >> 16) * phi[dt_topic(e)];
         }
         for (int off = 32; off >= 1; off >>= 1) s += __shfl_xor(s, off);
         acc += log((s + q) / ((double)(di.y >> 16) + (double)K * alpha));

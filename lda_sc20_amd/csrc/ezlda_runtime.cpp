@@ -4,6 +4,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <array>
 #include <cstdio>
 #include <cstring>
 #include <string>
@@ -65,6 +66,8 @@ struct ezlda_ctx {
     std::vector<uint32_t> doc_off;   // D+1
     hipGraph_t graph = nullptr;
     hipGraphExec_t graph_exec = nullptr;
+    int n_xcd = 1;      // XCDs of the device (round-robin workgroup dispatch)
+    int doc_tiles = 0;  // doc tiles used for head-word items (0 = none)
 };
 
 namespace {
@@ -119,6 +122,91 @@ int need_loaded(ezlda_ctx *c) {
     return EZLDA_OK;
 }
 
+// Sampler work items (word, first token, end token).  Head words: docs are
+// cut into T tiles of equal head-token work, T = n_xcd * 2^a chosen so one
+// tile's DT rows fit in an XCD's L2; XCD x owns tiles [x*T/n_xcd, ...).  A
+// head word gets one item per group of tiles (finer for frequent words, so
+// an item keeps >= kMinItemTokens tokens); each XCD's items are sorted by
+// (tile, word) and the lists are interleaved so that workgroup 8j+x, which
+// round-robin dispatch sends to XCD x, is XCD x's j-th item.  Short lists
+// are padded with empty items (word 0xFFFFFFFF).  Tail words follow, one
+// item per word.  Items longer than token_segment are split.  The order of
+// items changes which workgroup draws a token, never the draw.
+std::vector<uint4> build_sample_items(const ezlda_args &a, const std::vector<uint32_t> &word_off,
+                                      const std::vector<uint32_t> &tok_doc, const std::vector<uint32_t> &doc_wrd,
+                                      const std::vector<uint32_t> &doc_off, int VD, int K, int n_xcd,
+                                      int *tiles_used) {
+    const uint32_t SEG = (uint32_t)a.token_segment;
+    const int V = (int)word_off.size() - 1;
+    const int D = (int)doc_off.size() - 1;
+    std::vector<uint4> out;
+    auto push_split = [&](std::vector<uint4> &v, uint32_t w, uint32_t b, uint32_t e) {
+        for (uint32_t x = b; x < e; x += SEG) v.push_back(make_uint4(w, x, std::min(x + SEG, e), 0u));
+    };
+    int T = 0;
+    uint64_t head_tokens = word_off[VD];
+    if (a.doc_tiles >= 0 && VD > 0 && D > 0 && n_xcd > 0 && head_tokens > 0) {
+        // DT bytes of the whole corpus -> tiles of ~2 MiB
+        uint64_t dt_bytes = 0;
+        for (int d = 0; d < D; ++d) dt_bytes += 4ull * std::min<uint32_t>(doc_off[d + 1] - doc_off[d], (uint32_t)K);
+        if (a.doc_tiles > 0) {
+            T = a.doc_tiles;
+        } else {
+            T = n_xcd;
+            while ((uint64_t)T * (2u << 20) < dt_bytes && T < 4096) T *= 2;
+        }
+        T = std::max(n_xcd, (T / n_xcd) * n_xcd);
+    }
+    if (T > 0) {
+        // tile boundaries: equal head-token work per tile
+        std::vector<uint64_t> hw(D + 1, 0);
+        for (int d = 0; d < D; ++d) {
+            uint64_t h = 0;
+            for (uint32_t i = doc_off[d]; i < doc_off[d + 1]; ++i) h += doc_wrd[i] < (uint32_t)VD;
+            hw[d + 1] = hw[d] + h;
+        }
+        std::vector<uint32_t> tile_doc(T + 1, 0);
+        for (int t = 0; t <= T; ++t) {
+            const uint64_t target = hw[D] * (uint64_t)t / (uint64_t)T;
+            tile_doc[t] = (uint32_t)(std::lower_bound(hw.begin(), hw.end(), target) - hw.begin());
+        }
+        tile_doc[0] = 0;
+        tile_doc[T] = (uint32_t)D;
+        const int per_xcd = T / n_xcd;
+        const uint32_t kMinItemTokens = 512;
+        // per XCD: (tile, word, first, end)
+        std::vector<std::vector<std::array<uint32_t, 4>>> lists(n_xcd);
+        for (int w = 0; w < VD; ++w) {
+            const uint32_t b = word_off[w], e = word_off[w + 1];
+            const uint32_t n = e - b;
+            if (n == 0) continue;
+            int g = per_xcd;  // tiles per item: the coarsest that still splits per XCD
+            while (g > 1 && (uint64_t)n * (uint64_t)g / (uint64_t)T >= 2ull * kMinItemTokens) g /= 2;
+            for (int t = 0; t < T; t += g) {
+                const uint32_t d0 = tile_doc[t], d1 = tile_doc[std::min(T, t + g)];
+                const uint32_t lo = (uint32_t)(std::lower_bound(tok_doc.begin() + b, tok_doc.begin() + e, d0) - tok_doc.begin());
+                const uint32_t hi = (uint32_t)(std::lower_bound(tok_doc.begin() + b, tok_doc.begin() + e, d1) - tok_doc.begin());
+                if (hi > lo) lists[t / per_xcd].push_back({(uint32_t)t, (uint32_t)w, lo, hi});
+            }
+        }
+        std::vector<std::vector<uint4>> split(n_xcd);
+        size_t longest = 0;
+        for (int x = 0; x < n_xcd; ++x) {
+            std::sort(lists[x].begin(), lists[x].end());
+            for (auto &it : lists[x]) push_split(split[x], it[1], it[2], it[3]);
+            longest = std::max(longest, split[x].size());
+        }
+        for (size_t j = 0; j < longest; ++j)
+            for (int x = 0; x < n_xcd; ++x)
+                out.push_back(j < split[x].size() ? split[x][j] : make_uint4(0xFFFFFFFFu, 0u, 0u, 0u));
+    } else {
+        for (int w = 0; w < VD; ++w) push_split(out, (uint32_t)w, word_off[w], word_off[w + 1]);
+    }
+    for (int w = VD; w < V; ++w) push_split(out, (uint32_t)w, word_off[w], word_off[w + 1]);
+    if (tiles_used) *tiles_used = T;
+    return out;
+}
+
 }  // namespace
 
 extern "C" {
@@ -148,8 +236,8 @@ ezlda_ctx *ezlda_create(const ezlda_args *args) {
         return nullptr;
     }
     ezlda_args a = *args;
-    if (a.K < 1 || a.K > 65535) {
-        fail(nullptr, EZLDA_EINVAL, "K must be in [1, 65535]");
+    if (a.K < 1 || a.K > 16383) {
+        fail(nullptr, EZLDA_EINVAL, "K must be in [1, 16383]");
         return nullptr;
     }
     if (a.alpha <= 0.f) a.alpha = 50.0f / (float)a.K;
@@ -178,6 +266,12 @@ ezlda_ctx *ezlda_create(const ezlda_args *args) {
     }
     ezlda_ctx *c = new ezlda_ctx();
     c->a = a;
+    {
+        int dev = 0, nx = 1;
+        if (hipGetDevice(&dev) == hipSuccess &&
+            hipDeviceGetAttribute(&nx, hipDeviceAttributeNumberOfXccs, dev) == hipSuccess && nx > 0)
+            c->n_xcd = nx;
+    }
     if (hipStreamCreateWithFlags(&c->st, hipStreamNonBlocking) != hipSuccess) {
         fail(nullptr, EZLDA_EHIP, "hipStreamCreate failed");
         delete c;
@@ -313,10 +407,8 @@ int ezlda_load_docs(ezlda_ctx *c, int32_t V, int32_t D, const int64_t *doc_off, 
     for (int r = 0; r < VD; ++r)
         for (uint32_t b = c->word_off[r]; b < c->word_off[r + 1]; b += WT_SEG)
             wt_segs.push_back(make_uint4((uint32_t)r, b, std::min(b + WT_SEG, c->word_off[r + 1]), 0u));
-    const uint32_t SEG = (uint32_t)c->a.token_segment;
-    for (int r = 0; r < V; ++r)
-        for (uint32_t b = c->word_off[r]; b < c->word_off[r + 1]; b += SEG)
-            smp_segs.push_back(make_uint4((uint32_t)r, b, std::min(b + SEG, c->word_off[r + 1]), 0u));
+    smp_segs = build_sample_items(c->a, c->word_off, tok_doc, doc_wrd, c->doc_off, VD, K, c->n_xcd,
+                                  &c->doc_tiles);
 
     ez::DevState &s = c->s;
     s.K = K; s.V = V; s.VD = VD; s.D = D; s.N = N;
@@ -336,7 +428,9 @@ int ezlda_load_docs(ezlda_ctx *c, int32_t V, int32_t D, const int64_t *doc_off, 
     if ((r = upload(c, &s.doc_tok, c->doc_tok))) return r;
     if ((r = upload(c, &s.doc_wrd, doc_wrd))) return r;
     if ((r = upload(c, &s.doc_info, doc_info))) return r;
-    if ((r = dalloc(c, &s.dt, dtc))) return r;
+    // 256 entries of zero padding past the last row
+    if ((r = dalloc(c, &s.dt, dtc + 256))) return r;
+    EZ_TRY_HIP(c, hipMemset(s.dt, 0, (dtc + 256) * sizeof(uint32_t)));
     if ((r = dalloc(c, &s.wt_head, (size_t)VD * K))) return r;
     if ((r = upload(c, &s.tail_off, tail_off))) return r;
     if ((r = dalloc(c, &s.wt_tail, tail_off.back()))) return r;
@@ -384,6 +478,8 @@ int ezlda_get_info(ezlda_ctx *c, ezlda_info *info) {
     info->num_sample_items = c->s.n_smp_segs;
     info->sweep = c->sweep;
     info->device_bytes = c->dev_bytes;
+    info->doc_tiles = c->doc_tiles;
+    info->num_xcd = c->n_xcd;
     return EZLDA_OK;
 }
 
@@ -645,7 +741,10 @@ int ezlda_get_dt_raw(ezlda_ctx *c, uint32_t *entries, int32_t *nnz) {
     if (!dt.empty()) EZ_TRY_HIP(c, hipMemcpy(dt.data(), s.dt, dt.size() * 4, hipMemcpyDeviceToHost));
     for (int d = 0; d < s.D; ++d) {
         nnz[d] = (int32_t)(di[d].y & 0xFFFFu);
-        std::memcpy(entries + di[d].x, dt.data() + di[d].x, sizeof(uint32_t) * nnz[d]);
+        for (int32_t i = 0; i < nnz[d]; ++i) {  // device rows hold topic << 2
+            const uint32_t e = dt[di[d].x + i];
+            entries[di[d].x + i] = ((e & 0xFFFFu) >> 2) | (e & 0xFFFF0000u);
+        }
     }
     return EZLDA_OK;
 }

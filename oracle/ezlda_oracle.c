@@ -37,7 +37,8 @@
  *      first element of positive mass whose inclusive prefix exceeds the
  *      target wins (a tree scan is not monotone in f32, so the mass test
  *      keeps zero-mass elements out).  S is summed lane-major: DT entry i
- *      goes to lane i mod 64, each lane adds its entries left to right and
+ *      goes to lane i mod 64, each lane accumulates its entries left to
+ *      right with fmaf (C99, correctly rounded: the device FMA) and
  *      the 64 lane sums go through the wave64 scan tree of scan64(); the Q
  *      bucket is scanned per 64-topic slab and the slabs summed left to
  *      right.  When rounding leaves no prefix above the target, the last
@@ -324,8 +325,9 @@ void orc_build_dt(void *h) {
 /* ------------------------------------------------------- 5-6. sampler */
 /* Full three-bucket draw for one token (LDAKernelTrain(D), utility.cu:
  * 1217-1611, :1625-2103; bucket search :1878-1992).  phi has phi[k1] = 0.
- * S bucket in lane-major order: lane l (0..63) sums the products of DT
- * entries l, l+64, l+128, ... left to right; the 64 lane sums go through
+ * S bucket in lane-major order: lane l (0..63) accumulates the products of
+ * DT entries l, l+64, l+128, ... left to right with fused multiply-adds
+ * (acc = fma(n_dk, phi_k, acc), one rounding each); the 64 lane sums go through
  * scan64(); the draw picks the lane whose prefix first exceeds the target,
  * then walks that lane's entries in the same left-to-right order. */
 static int draw_full(const orc_t *o, const float *phi, const float *qtot, const float *qacc,
@@ -335,7 +337,7 @@ static int draw_full(const orc_t *o, const float *phi, const float *qtot, const 
     float a[64], sc[64];
     for (int l = 0; l < 64; ++l) {
         float acc = 0.0f;
-        for (int i = l; i < nnz; i += 64) acc = acc + (float)(row[i] >> 16) * phi[row[i] & 0xFFFFu];
+        for (int i = l; i < nnz; i += 64) acc = fmaf((float)(row[i] >> 16), phi[row[i] & 0xFFFFu], acc);
         a[l] = acc;
         sc[l] = acc;
     }
@@ -358,9 +360,9 @@ static int draw_full(const orc_t *o, const float *phi, const float *qtot, const 
         float run = 0.0f;
         int last = -1;
         for (int i = ls; i < nnz; i += 64) {
-            const float p = (float)(row[i] >> 16) * phi[row[i] & 0xFFFFu];
-            run = run + p;
-            if (p > 0.0f) {
+            const float c = (float)(row[i] >> 16), f = phi[row[i] & 0xFFFFu];
+            run = fmaf(c, f, run);
+            if (c > 0.0f && f > 0.0f) {
                 if (run > t3) return (int)(row[i] & 0xFFFFu);
                 last = i;
             }
