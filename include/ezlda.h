@@ -165,10 +165,8 @@ int ezlda_get_top(ezlda_ctx *ctx, uint16_t *top);
 int ezlda_get_aux(ezlda_ctx *ctx, uint32_t *aux);
 int ezlda_get_inv(ezlda_ctx *ctx, float *inv);
 
-/* ---- device self-tests (no reference counterpart; they pin the scan tree
- * and the RNG the oracle restates to what the device computes) */
-/* out[64w + l] = inclusive canonical wave64 scan of in[64w .. 64w+63]. */
-int ezlda_selftest_scan64(const float *in, float *out, int32_t num_waves);
+/* ---- device self-tests (no reference counterpart; they pin the RNG the
+ * oracle restates to what the device computes, and time a plain copy) */
 /* out[4i..4i+3] = Philox4x32-10(ctr[4i..4i+3], key[2i..2i+1]). */
 int ezlda_selftest_philox(const uint32_t *ctr, const uint32_t *key, uint32_t *out, int32_t n);
 /* Streams `bytes` from one device buffer to another `reps` times with a

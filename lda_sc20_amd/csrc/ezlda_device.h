@@ -56,7 +56,6 @@ hipError_t launch_advance_sweep(const DevState &s, hipStream_t st);
 hipError_t launch_loglik(const DevState &s, hipStream_t st);
 hipError_t launch_set_u32(uint32_t *p, uint32_t v, hipStream_t st);
 
-hipError_t launch_selftest_scan(const float *in, float *out, int nwaves, hipStream_t st);
 hipError_t launch_selftest_philox(const uint32_t *ctr, const uint32_t *key, uint32_t *out, int n, hipStream_t st);
 hipError_t launch_copy(const void *src, void *dst, size_t bytes, hipStream_t st);
 

@@ -6,8 +6,8 @@
 //   -> sample (three-bucket draw, writes z) -> advance_sweep.
 // Every float operation follows the canonical order of the CPU oracle
 // (oracle/ezlda_oracle.c); builds use -ffp-contract=off, so the topics agree
-// bit for bit.  Wave64 throughout: one wavefront owns a doc (dt_build), a
-// tail word (wt_tail, word_top) or a token's DT row (sample).
+// bit for bit.  Wave64: one wavefront owns a doc (dt_build) or a tail word
+// (wt_tail, word_top); in the sampler one lane owns a token.
 #include <hip/hip_runtime.h>
 
 #include "ezlda_device.h"
@@ -15,29 +15,10 @@
 
 namespace ez {
 
-static constexpr float kMargin = 0.9990234375f;  // 1 - 2^-10
+static constexpr float kMargin = 0.998046875f;  // 1 - 2^-9
 static constexpr uint32_t kNone = 0xFFFFu;
 
 // ---------------------------------------------------------------- helpers
-template <int CTRL, int ROWMASK>
-__device__ __forceinline__ float dpp_or_zero(float v) {
-    return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), CTRL, ROWMASK, 0xF, true));
-}
-
-// Inclusive wave64 scan in the canonical tree: Hillis-Steele inside each
-// 16-lane row (row_shr 1,2,4,8), then row_bcast:15 into rows 1,3 and
-// row_bcast:31 into rows 2,3.  Inputs are >= 0, so adding the 0 that a
-// masked or out-of-row lane reads leaves a value unchanged.
-__device__ __forceinline__ float wave_scan(float v) {
-    v = v + dpp_or_zero<0x111, 0xF>(v);
-    v = v + dpp_or_zero<0x112, 0xF>(v);
-    v = v + dpp_or_zero<0x114, 0xF>(v);
-    v = v + dpp_or_zero<0x118, 0xF>(v);
-    v = v + dpp_or_zero<0x142, 0xA>(v);
-    v = v + dpp_or_zero<0x143, 0xC>(v);
-    return v;
-}
-
 __device__ __forceinline__ float readlane_f(float v, int l) {
     return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), l));
 }
@@ -255,7 +236,8 @@ __global__ void k_word_top(int V, int VD, int K, float beta, const int32_t *__re
 // DT_Update_Kernel (utility.cu:79-284): one wavefront per doc gathers the
 // topics of its tokens into a 16-bit LDS histogram, writes each token's
 // (n_dk1, n_dk2) at its word's top-2 topics (the re-pack of :125-161), and
-// compacts the row in ascending topic order (:183-238).
+// compacts the row in ascending topic order (:183-238).  A row starts on a
+// 16-byte boundary and is zero-padded to a multiple of 4 entries.
 __global__ void k_dt_build(int D, int K, const uint32_t *__restrict__ doc_off, const uint32_t *__restrict__ doc_tok,
                            const uint32_t *__restrict__ doc_wrd, const uint16_t *__restrict__ z,
                            const ushort4 *__restrict__ wtop, uint32_t *__restrict__ aux,
@@ -290,35 +272,25 @@ __global__ void k_dt_build(int D, int K, const uint32_t *__restrict__ doc_off, c
                 if (k < K && !(k & 1)) h[k >> 1] = 0;
             }
             wave_lds_sync();
+            // zero the row up to the next multiple of 4 entries: the sampler
+            // reads rows 16 bytes at a time and a zero entry adds nothing
+            if ((uint32_t)lane < ((4u - (nz & 3u)) & 3u)) dt[base + nz + lane] = 0u;
         }
         if (lane == 0) doc_info[d].y = nz | ((e - b) << 16);
     }
 }
 
 // ------------------------------------------------------------- sampler
-// A DT row as a buffer resource whose range is the row itself: lanes that
-// index past the row read 0 in hardware, with no clamp, mask or branch.
-__device__ __forceinline__ __amdgpu_buffer_rsrc_t row_rsrc(const uint32_t *row, uint32_t nnz) {
-    return __builtin_amdgcn_make_buffer_rsrc(const_cast<uint32_t *>(row), (short)0, (int)(nnz * 4u), 0x00020000);
-}
-__device__ __forceinline__ uint32_t row_ld(__amdgpu_buffer_rsrc_t r, uint32_t byte_off) {
-    return (uint32_t)__builtin_amdgcn_raw_buffer_load_b32(r, (int)byte_off, 0, 0);
-}
-
-// First four lane-strided DT entries of a row (0 beyond its end).
-struct Row4 {
-    uint32_t e0, e1, e2, e3;
-};
-
-__device__ __forceinline__ Row4 load_row4(__amdgpu_buffer_rsrc_t r, uint32_t lane_bytes) {
-    Row4 x;
-    x.e0 = row_ld(r, lane_bytes);
-    x.e1 = row_ld(r, lane_bytes + 256u);
-    x.e2 = row_ld(r, lane_bytes + 512u);
-    x.e3 = row_ld(r, lane_bytes + 768u);
-    return x;
-}
-
+// One workgroup per sampler item (a word, or a doc tile / token segment of a
+// head word).  The workgroup stages phi_w in LDS with phi[k1] = 0, and the Q
+// bucket's running sums: qs[k] inside each 64-topic slab, qacc[j] over the
+// slab totals.  Then one lane per token: the max-topic shortcut test, and for
+// the other tokens one pass over the doc's DT row (16 bytes per load) for S,
+// the bucket choice, and either a second pass to the first running sum above
+// the target (S) or a bisection of the monotone Q running sums (Q).
+// Replaces UpdateProbKernelTrainD1 (utility.cu:2680-3312), LDAKernelTrainD
+// (:1625-2103) and LDAKernelTrain (:1217-1611); one kernel serves head and
+// tail words, which differ only in how phi_w is staged.
 template <bool SHORTCUT>
 __global__ void __launch_bounds__(1024) k_sample(
     const uint4 *__restrict__ segs, int VD, int K, float alpha, float beta, const int32_t *__restrict__ wt_head,
@@ -329,12 +301,12 @@ __global__ void __launch_bounds__(1024) k_sample(
     unsigned long long *__restrict__ stats, uint64_t seed) {
     extern __shared__ float smf[];
     __shared__ float f123[3];
-    uint32_t n_full = 0, n_scanned = 0;  // per wavefront: full draws, DT entries read
     const int NS = (K + 63) >> 6;
     const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6, nwv = blockDim.x >> 6;
-    float *phi = smf;             // NS*64
-    float *qtot = phi + NS * 64;  // NS
-    float *qacc = qtot + NS;      // NS
+    float *phi = smf;              // NS*64
+    float *qs = phi + NS * 64;     // NS*64: running sums inside each slab
+    float *qt = qs + NS * 64;      // NS: slab totals
+    float *qacc = qt + NS;         // NS: running sum of the slab totals
     const uint4 sg = segs[blockIdx.x];
     if (sg.x == 0xFFFFFFFFu) return;  // padding item
     const int w = (int)sg.x;
@@ -363,15 +335,20 @@ __global__ void __launch_bounds__(1024) k_sample(
         phi[tp.x] = 0.0f;
     }
     __syncthreads();
-    for (int j = wv; j < NS; j += nwv) {
-        const float q = wave_scan(alpha * phi[j * 64 + lane]);
-        if (lane == 63) qtot[j] = q;
+    for (int j = tid; j < NS; j += blockDim.x) {
+        const int k0 = j * 64, ke = min(k0 + 64, K);
+        float run = 0.0f;
+        for (int k = k0; k < ke; ++k) {
+            run = run + alpha * phi[k];
+            qs[k] = run;
+        }
+        qt[j] = run;
     }
     __syncthreads();
     if (tid == 0) {
         float acc = 0.0f;
         for (int j = 0; j < NS; ++j) {
-            acc = acc + qtot[j];
+            acc = acc + qt[j];
             qacc[j] = acc;
         }
     }
@@ -379,148 +356,106 @@ __global__ void __launch_bounds__(1024) k_sample(
     const float Q = qacc[NS - 1];
     const float f1 = f123[0], f2 = f123[1], f3 = f123[2];
     const uint32_t k1 = tp.x;
+    uint32_t n_full = 0, n_scanned = 0;
 
     for (uint32_t base = sg.y + (uint32_t)wv * 64u; base < sg.z; base += (uint32_t)nwv * 64u) {
-        const uint32_t t = base + lane;
-        const bool valid = t < sg.z;
-        uint2 di = make_uint2(0u, 0u);
-        float u = 0.0f, P1 = 0.0f;
-        bool need = false;
-        if (valid) {
-            const uint32_t d = tok_doc[t];
-            const uint32_t a = aux[t];
-            di = doc_info[d];
-            u = ez_u24(ez_philox_x0(seed, t, sweep, EZ_TAG_SAMPLE));
-            const uint32_t n1 = a & 0xFFFFu, n2 = a >> 16, nd = di.y >> 16;
-            P1 = ((float)n1 + alpha) * f1;
-            need = true;
-            if (SHORTCUT) {
-                const float maxS = (float)n2 * f2 + (float)(nd - n1 - n2) * f3;
-                const float B = (P1 + maxS) + Q;
-                if (u * B < P1 * kMargin) need = false;
-            }
+        const uint32_t t = base + (uint32_t)lane;
+        if (t >= sg.z) break;
+        const uint32_t d = tok_doc[t];
+        const uint32_t a = aux[t];
+        const uint2 di = doc_info[d];
+        const float u = ez_u24(ez_philox_x0(seed, t, sweep, EZ_TAG_SAMPLE));
+        const uint32_t n1 = a & 0xFFFFu, n2 = a >> 16, nd = di.y >> 16;
+        const float P1 = ((float)n1 + alpha) * f1;
+        bool need = true;
+        if (SHORTCUT) {
+            const float maxS = (float)n2 * f2 + (float)(nd - n1 - n2) * f3;
+            const float B = (P1 + maxS) + Q;
+            if (u * B < P1 * kMargin) need = false;
         }
         uint32_t newz = k1;
-        uint64_t todo = __ballot(need);
-        // Full draws one token at a time; the first 256 DT entries of the
-        // next token's row are loaded while the current token is drawn.
-        const uint32_t lane_bytes = (uint32_t)lane * 4u;
-        int L = todo ? first_lane(todo) : -1;
-        uint32_t off = 0, nnz = 0;
-        Row4 cur{0u, 0u, 0u, 0u};
-        if (L >= 0) {
-            off = readlane_u(di.x, L);
-            nnz = readlane_u(di.y, L) & 0xFFFFu;
-            cur = load_row4(row_rsrc(dt + off, nnz), lane_bytes);
-        }
-        while (L >= 0) {
-            todo &= todo - 1ull;
-            const int L2 = todo ? first_lane(todo) : -1;
-            uint32_t off2 = 0, nnz2 = 0;  // no next token: an empty range reads zeros
-            if (L2 >= 0) {
-                off2 = readlane_u(di.x, L2);
-                nnz2 = readlane_u(di.y, L2) & 0xFFFFu;
-            }
-            const Row4 nxt = load_row4(row_rsrc(dt + off2, nnz2), lane_bytes);
-            const float uL = readlane_f(u, L), P1L = readlane_f(P1, L);
+        if (need) {
+            const uint4 *rp = reinterpret_cast<const uint4 *>(dt + di.x);
+            const uint32_t nnz = di.y & 0xFFFFu;
+            const uint32_t ng = (nnz + 3u) >> 2;
             n_full += 1u;
             n_scanned += nnz;
-            // S bucket, lane-major: lane l sums entries l, l+64, ... in order
-            // (a zero entry beyond the row adds +0 and changes nothing)
-            off = (uint32_t)__builtin_amdgcn_readfirstlane((int)off);
-            nnz = (uint32_t)__builtin_amdgcn_readfirstlane((int)nnz);
-            const __amdgpu_buffer_rsrc_t rr = row_rsrc(dt + off, nnz);
-            float acc = __builtin_fmaf(dt_count(cur.e0), phi_of(phi, cur.e0), 0.0f);
-            acc = __builtin_fmaf(dt_count(cur.e1), phi_of(phi, cur.e1), acc);
-            acc = __builtin_fmaf(dt_count(cur.e2), phi_of(phi, cur.e2), acc);
-            acc = __builtin_fmaf(dt_count(cur.e3), phi_of(phi, cur.e3), acc);
-            for (uint32_t b = 1024u; b < nnz * 4u; b += 256u) {
-                const uint32_t e = row_ld(rr, lane_bytes + b);
-                acc = __builtin_fmaf(dt_count(e), phi_of(phi, e), acc);
+            float S = 0.0f;
+            for (uint32_t g = 0; g < ng; ++g) {
+                const uint4 e = rp[g];
+                S = __builtin_fmaf(dt_count(e.x), phi_of(phi, e.x), S);
+                S = __builtin_fmaf(dt_count(e.y), phi_of(phi, e.y), S);
+                S = __builtin_fmaf(dt_count(e.z), phi_of(phi, e.z), S);
+                S = __builtin_fmaf(dt_count(e.w), phi_of(phi, e.w), S);
             }
-            const float sc = wave_scan(acc);
-            const float S = readlane_f(sc, 63);
-            const float A = P1L + S;
+            const float A = P1 + S;
             const float total = A + Q;
-            const float target = uL * total;
-            uint32_t zz = k1;
-            if (target < P1L) {
-                zz = k1;
+            const float target = u * total;
+            if (target < P1) {
+                newz = k1;
             } else if (target < A) {
-                const float t2 = target - P1L;
-                const uint64_t m = __ballot(sc > t2 && acc > 0.0f);
-                int ls = m ? first_lane(m) : -1;
-                if (ls < 0) {
-                    const uint64_t m2 = __ballot(acc > 0.0f);
-                    ls = m2 ? last_lane(m2) : -1;
-                }
-                if (ls >= 0) {
-                    const float t3 = t2 - (ls > 0 ? readlane_f(sc, ls - 1) : 0.0f);
-                    // walk lane ls's entries ls, ls+64, ... left to right; lane j
-                    // of each pass holds entry ls + 64*(pass*64 + j)
-                    const int cnt = (int)((nnz - (uint32_t)ls + 63u) >> 6);
-                    float run = 0.0f;
-                    int sel = -1;
-                    uint32_t sel_e = 0, last_e = 0;
-                    bool have_last = false;
-                    for (int j0 = 0; j0 < cnt && sel < 0; j0 += 64) {
-                        const uint32_t e = row_ld(rr, ((uint32_t)ls + 64u * (uint32_t)(j0 + lane)) * 4u);
-                        const float c = dt_count(e), f = phi_of(phi, e);
-                        const int n = min(64, cnt - j0);
-                        for (int j = 0; j < n; ++j) {
-                            const float cj = readlane_f(c, j), fj = readlane_f(f, j);
-                            run = __builtin_fmaf(cj, fj, run);
-                            if (cj > 0.0f && fj > 0.0f) {
-                                if (run > t3) {
-                                    sel = j;
-                                    sel_e = readlane_u(e, j);
-                                    break;
-                                }
-                                last_e = readlane_u(e, j);
-                                have_last = true;
-                            }
-                        }
+                const float t2 = target - P1;
+                float run = 0.0f;
+                bool found = false;
+                for (uint32_t g = 0; g < ng; ++g) {
+                    const uint4 e = rp[g];
+                    const float r0 = __builtin_fmaf(dt_count(e.x), phi_of(phi, e.x), run);
+                    const float r1 = __builtin_fmaf(dt_count(e.y), phi_of(phi, e.y), r0);
+                    const float r2 = __builtin_fmaf(dt_count(e.z), phi_of(phi, e.z), r1);
+                    const float r3 = __builtin_fmaf(dt_count(e.w), phi_of(phi, e.w), r2);
+                    if (r3 > t2) {
+                        newz = dt_topic(r0 > t2 ? e.x : r1 > t2 ? e.y : r2 > t2 ? e.z : e.w);
+                        found = true;
+                        break;
                     }
-                    if (sel >= 0)
-                        zz = dt_topic(sel_e);
-                    else if (have_last)
-                        zz = dt_topic(last_e);
+                    run = r3;
+                }
+                if (!found) {  // rounding left every running sum at or below t2
+                    const uint32_t *r = dt + di.x;
+                    for (uint32_t i = 0; i < nnz; ++i)
+                        if (dt_count(r[i]) > 0.0f && phi_of(phi, r[i]) > 0.0f) newz = dt_topic(r[i]);
                 }
             } else {
                 const float t2 = target - A;
-                int js = -1;
-                for (int j0 = 0; j0 < NS && js < 0; j0 += 64) {
-                    const int j = j0 + lane;
-                    const uint64_t m = __ballot(j < NS && qacc[j] > t2);
-                    if (m) js = j0 + first_lane(m);
+                int lo = 0, hi = NS;  // first slab whose running sum exceeds t2
+                while (lo < hi) {
+                    const int mid = (lo + hi) >> 1;
+                    if (qacc[mid] > t2) hi = mid; else lo = mid + 1;
                 }
+                int js = lo < NS ? lo : -1;
                 if (js < 0)
-                    for (int j0 = ((NS - 1) >> 6) << 6; j0 >= 0 && js < 0; j0 -= 64) {
-                        const int j = j0 + lane;
-                        const uint64_t m = __ballot(j < NS && qtot[j] > 0.0f);
-                        if (m) js = j0 + last_lane(m);
-                    }
+                    for (int j = NS - 1; j >= 0; --j)
+                        if (qt[j] > 0.0f) {
+                            js = j;
+                            break;
+                        }
                 if (js >= 0) {
                     const float t3 = t2 - (js > 0 ? qacc[js - 1] : 0.0f);
-                    const int k = js * 64 + lane;
-                    const float p = alpha * phi[k];
-                    const float sc = wave_scan(p);
-                    uint64_t m = __ballot(k < K && sc > t3 && p > 0.0f);
-                    int ls = m ? first_lane(m) : -1;
-                    if (ls < 0) {
-                        const uint64_t m2 = __ballot(p > 0.0f);
-                        ls = m2 ? last_lane(m2) : -1;
+                    const int k0 = js * 64, ke = min(k0 + 64, K);
+                    lo = k0;
+                    hi = ke;
+                    while (lo < hi) {
+                        const int mid = (lo + hi) >> 1;
+                        if (qs[mid] > t3) hi = mid; else lo = mid + 1;
                     }
-                    if (ls >= 0) zz = (uint32_t)(js * 64 + ls);
+                    if (lo < ke) {
+                        newz = (uint32_t)lo;
+                    } else {
+                        for (int k = ke - 1; k >= k0; --k)
+                            if (alpha * phi[k] > 0.0f) {
+                                newz = (uint32_t)k;
+                                break;
+                            }
+                    }
                 }
             }
-            if (lane == L) newz = zz;
-            L = L2;
-            off = off2;
-            nnz = nnz2;
-            cur = nxt;
         }
-        if (valid) z[t] = (uint16_t)newz;
+        z[t] = (uint16_t)newz;
+    }
+    // per-wavefront totals of the counters
+    for (int off = 32; off >= 1; off >>= 1) {
+        n_full += __shfl_xor(n_full, off);
+        n_scanned += __shfl_xor(n_scanned, off);
     }
     if (lane == 0 && n_full) {
         atomicAdd(&stats[0], (unsigned long long)n_full);
@@ -610,13 +545,6 @@ __global__ void k_ll_final(const double *__restrict__ part, int n, double *__res
 }
 
 // ------------------------------------------------------------ self-tests
-__global__ void k_selftest_scan(const float *__restrict__ in, float *__restrict__ out, int n) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    const float v = i < n ? in[i] : 0.0f;
-    const float s = wave_scan(v);
-    if (i < n) out[i] = s;
-}
-
 __global__ void k_selftest_philox(const uint4 *__restrict__ ctr, const uint2 *__restrict__ key,
                                   uint4 *__restrict__ out, int n) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -637,11 +565,6 @@ hipError_t launch_copy(const void *src, void *dst, size_t bytes, hipStream_t st)
     const size_t n = bytes / 16;
     hipLaunchKernelGGL(k_copy_u4, dim3(grid_for(n, 256, 8192)), dim3(256), 0, st,
                        reinterpret_cast<const uint4 *>(src), reinterpret_cast<uint4 *>(dst), n);
-    return hipGetLastError();
-}
-
-hipError_t launch_selftest_scan(const float *in, float *out, int nwaves, hipStream_t st) {
-    hipLaunchKernelGGL(k_selftest_scan, dim3(nwaves), dim3(64), 0, st, in, out, nwaves * 64);
     return hipGetLastError();
 }
 
@@ -703,7 +626,7 @@ hipError_t launch_build_dt(const DevState &s, hipStream_t st) {
 size_t sample_lds_bytes(int K, int bd) {
     const int NS = (K + 63) >> 6;
     (void)bd;
-    return ((size_t)NS * 64 + 2 * (size_t)NS) * sizeof(float);
+    return (2 * (size_t)NS * 64 + 2 * (size_t)NS) * sizeof(float);
 }
 
 hipError_t launch_sample(const DevState &s, bool shortcut, hipStream_t st) {

@@ -57,7 +57,7 @@ struct ezlda_ctx {
     bool counts_fresh = false;
     uint32_t sweep = 0;
     int64_t dev_bytes = 0;
-    int64_t dt_cap = 0, tail_cap = 0;
+    int64_t dt_cap = 0, tail_cap = 0, dt_cap_canon = 0;
     std::vector<void *> allocs;
     // host mirrors for result mapping
     std::vector<uint32_t> doc_tok;   // doc-major position -> token id
@@ -122,8 +122,8 @@ int need_loaded(ezlda_ctx *c) {
     return EZLDA_OK;
 }
 
-// Sampler work items (word, first token, end token).  Head words: docs are
-// cut into T tiles of equal head-token work, T = n_xcd * 2^a chosen so one
+// Sampler work items (word, first token, end token).  Head words: the doc
+// order (by length) is cut into T tiles of equal head-token work, T = n_xcd * 2^a chosen so one
 // tile's DT rows fit in an XCD's L2; XCD x owns tiles [x*T/n_xcd, ...).  A
 // head word gets one item per group of tiles (finer for frequent words, so
 // an item keeps >= kMinItemTokens tokens); each XCD's items are sorted by
@@ -133,9 +133,9 @@ int need_loaded(ezlda_ctx *c) {
 // item per word.  Items longer than token_segment are split.  The order of
 // items changes which workgroup draws a token, never the draw.
 std::vector<uint4> build_sample_items(const ezlda_args &a, const std::vector<uint32_t> &word_off,
-                                      const std::vector<uint32_t> &tok_doc, const std::vector<uint32_t> &doc_wrd,
-                                      const std::vector<uint32_t> &doc_off, int VD, int K, int n_xcd,
-                                      int *tiles_used) {
+                                      const std::vector<uint32_t> &tok_rank, const std::vector<uint32_t> &doc_wrd,
+                                      const std::vector<uint32_t> &doc_off, const std::vector<int32_t> &dord, int VD,
+                                      int K, int n_xcd, int *tiles_used) {
     const uint32_t SEG = (uint32_t)a.token_segment;
     const int V = (int)word_off.size() - 1;
     const int D = (int)doc_off.size() - 1;
@@ -159,11 +159,12 @@ std::vector<uint4> build_sample_items(const ezlda_args &a, const std::vector<uin
     }
     if (T > 0) {
         // tile boundaries: equal head-token work per tile
-        std::vector<uint64_t> hw(D + 1, 0);
-        for (int d = 0; d < D; ++d) {
+        std::vector<uint64_t> hw(D + 1, 0);  // head tokens of the first q docs in doc order
+        for (int q = 0; q < D; ++q) {
+            const int d = dord[q];
             uint64_t h = 0;
             for (uint32_t i = doc_off[d]; i < doc_off[d + 1]; ++i) h += doc_wrd[i] < (uint32_t)VD;
-            hw[d + 1] = hw[d] + h;
+            hw[q + 1] = hw[q] + h;
         }
         std::vector<uint32_t> tile_doc(T + 1, 0);
         for (int t = 0; t <= T; ++t) {
@@ -184,8 +185,8 @@ std::vector<uint4> build_sample_items(const ezlda_args &a, const std::vector<uin
             while (g > 1 && (uint64_t)n * (uint64_t)g / (uint64_t)T >= 2ull * kMinItemTokens) g /= 2;
             for (int t = 0; t < T; t += g) {
                 const uint32_t d0 = tile_doc[t], d1 = tile_doc[std::min(T, t + g)];
-                const uint32_t lo = (uint32_t)(std::lower_bound(tok_doc.begin() + b, tok_doc.begin() + e, d0) - tok_doc.begin());
-                const uint32_t hi = (uint32_t)(std::lower_bound(tok_doc.begin() + b, tok_doc.begin() + e, d1) - tok_doc.begin());
+                const uint32_t lo = (uint32_t)(std::lower_bound(tok_rank.begin() + b, tok_rank.begin() + e, d0) - tok_rank.begin());
+                const uint32_t hi = (uint32_t)(std::lower_bound(tok_rank.begin() + b, tok_rank.begin() + e, d1) - tok_rank.begin());
                 if (hi > lo) lists[t / per_xcd].push_back({(uint32_t)t, (uint32_t)w, lo, hi});
             }
         }
@@ -324,22 +325,35 @@ int ezlda_load_docs(ezlda_ctx *c, int32_t V, int32_t D, const int64_t *doc_off, 
     c->doc_off.assign(D + 1, 0);
     for (int d = 0; d <= D; ++d) c->doc_off[d] = (uint32_t)doc_off[d];
 
-    // placement: contiguous doc ranges per thread keep (doc, position) order
+    // docs by length desc, id asc: the order tokens of a word are numbered in
+    // (neighbouring tokens then have similar DT row lengths)
+    std::vector<int32_t> dord(D);
+    for (int d = 0; d < D; ++d) dord[d] = d;
+    std::sort(dord.begin(), dord.end(), [&](int32_t x, int32_t y) {
+        const int64_t lx = doc_off[x + 1] - doc_off[x], ly = doc_off[y + 1] - doc_off[y];
+        return lx > ly || (lx == ly && x < y);
+    });
+    std::vector<uint32_t> drank(D);
+    for (int q = 0; q < D; ++q) drank[dord[q]] = (uint32_t)q;
+    // placement: contiguous ranges of the doc order per thread keep the
+    // (doc order, position) order of tokens inside each word
     const int T = host_threads();
     std::vector<int> dsplit(T + 1, 0);
-    for (int t = 0; t <= T; ++t) {
-        const int64_t target = N64 * t / T;
-        dsplit[t] = (int)(std::lower_bound(doc_off, doc_off + D + 1, target) - doc_off);
-        if (dsplit[t] > D) dsplit[t] = D;
+    {
+        std::vector<int64_t> pre(D + 1, 0);
+        for (int q = 0; q < D; ++q) pre[q + 1] = pre[q] + (doc_off[dord[q] + 1] - doc_off[dord[q]]);
+        for (int t = 0; t <= T; ++t)
+            dsplit[t] = (int)(std::lower_bound(pre.begin(), pre.end(), N64 * t / T) - pre.begin());
+        dsplit[0] = 0;
+        dsplit[T] = D;
     }
-    dsplit[0] = 0;
-    dsplit[T] = D;
     std::vector<std::vector<uint32_t>> tcnt(T, std::vector<uint32_t>(V, 0));
     {
         std::vector<std::thread> th;
         for (int t = 0; t < T; ++t)
             th.emplace_back([&, t] {
-                for (int64_t i = doc_off[dsplit[t]]; i < doc_off[dsplit[t + 1]]; ++i) tcnt[t][c->rank_of[words[i]]]++;
+                for (int q = dsplit[t]; q < dsplit[t + 1]; ++q)
+                    for (int64_t i = doc_off[dord[q]]; i < doc_off[dord[q] + 1]; ++i) tcnt[t][c->rank_of[words[i]]]++;
             });
         for (auto &x : th) x.join();
     }
@@ -351,7 +365,7 @@ int ezlda_load_docs(ezlda_ctx *c, int32_t V, int32_t D, const int64_t *doc_off, 
             cur += n;
         }
     }
-    std::vector<uint32_t> tok_doc(N), doc_wrd(N);
+    std::vector<uint32_t> tok_doc(N), tok_rank(N), doc_wrd(N);
     std::vector<uint16_t> z(N);
     c->doc_tok.assign(N, 0);
     {
@@ -359,15 +373,18 @@ int ezlda_load_docs(ezlda_ctx *c, int32_t V, int32_t D, const int64_t *doc_off, 
         for (int t = 0; t < T; ++t)
             th.emplace_back([&, t] {
                 std::vector<uint32_t> &cur = tcnt[t];
-                for (int d = dsplit[t]; d < dsplit[t + 1]; ++d)
+                for (int q = dsplit[t]; q < dsplit[t + 1]; ++q) {
+                    const int d = dord[q];
                     for (int64_t i = doc_off[d]; i < doc_off[d + 1]; ++i) {
                         const int32_t r = c->rank_of[words[i]];
                         const uint32_t tk = cur[r]++;
                         tok_doc[tk] = (uint32_t)d;
+                        tok_rank[tk] = (uint32_t)q;
                         c->doc_tok[i] = tk;
                         doc_wrd[i] = (uint32_t)r;
                         if (z0) z[tk] = z0[i];
                     }
+                }
             });
         for (auto &x : th) x.join();
     }
@@ -394,20 +411,24 @@ int ezlda_load_docs(ezlda_ctx *c, int32_t V, int32_t D, const int64_t *doc_off, 
         const uint32_t n = c->word_off[r + 1] - c->word_off[r];
         tail_off[r - VD + 1] = tail_off[r - VD] + std::min<uint32_t>(n, (uint32_t)K);
     }
+    // DT rows in the doc order above, each 16-byte aligned with capacity
+    // min(n_d, K) rounded up to 4 entries (the build zero-fills the rest)
     std::vector<uint2> doc_info(D);
     uint64_t dtc = 0;
-    for (int d = 0; d < D; ++d) {
+    for (int q = 0; q < D; ++q) {
+        const int d = dord[q];
         const uint32_t n = c->doc_off[d + 1] - c->doc_off[d];
         doc_info[d] = make_uint2((uint32_t)dtc, n << 16);
-        dtc += std::min<uint32_t>(n, (uint32_t)K);
+        dtc += (std::min<uint32_t>(n, (uint32_t)K) + 3u) & ~3u;
     }
+    if (dtc >= 0xFFFFFFF0ull) return fail(c, EZLDA_EINVAL, "DT capacity exceeds 2^32 entries");
     // work items
     const uint32_t WT_SEG = 32768;
     std::vector<uint4> wt_segs, smp_segs;
     for (int r = 0; r < VD; ++r)
         for (uint32_t b = c->word_off[r]; b < c->word_off[r + 1]; b += WT_SEG)
             wt_segs.push_back(make_uint4((uint32_t)r, b, std::min(b + WT_SEG, c->word_off[r + 1]), 0u));
-    smp_segs = build_sample_items(c->a, c->word_off, tok_doc, doc_wrd, c->doc_off, VD, K, c->n_xcd,
+    smp_segs = build_sample_items(c->a, c->word_off, tok_rank, doc_wrd, c->doc_off, dord, VD, K, c->n_xcd,
                                   &c->doc_tiles);
 
     ez::DevState &s = c->s;
@@ -418,6 +439,8 @@ int ezlda_load_docs(ezlda_ctx *c, int32_t V, int32_t D, const int64_t *doc_off, 
     s.block_dim = c->a.block_dim;
     s.grid_dim = c->a.grid_dim;
     c->dt_cap = (int64_t)dtc;
+    c->dt_cap_canon = 0;
+    for (int d = 0; d < D; ++d) c->dt_cap_canon += std::min<uint32_t>(c->doc_off[d + 1] - c->doc_off[d], (uint32_t)K);
     c->tail_cap = tail_off.back();
     int r = 0;
     if ((r = upload(c, &s.word_off, c->word_off))) return r;
@@ -473,7 +496,7 @@ int ezlda_get_info(ezlda_ctx *c, ezlda_info *info) {
     info->num_docs = c->s.D;
     info->num_head_words = c->s.VD;
     info->K = c->s.K;
-    info->dt_capacity = c->dt_cap;
+    info->dt_capacity = c->dt_cap_canon;
     info->wt_tail_capacity = c->tail_cap;
     info->num_sample_items = c->s.n_smp_segs;
     info->sweep = c->sweep;
@@ -739,12 +762,14 @@ int ezlda_get_dt_raw(ezlda_ctx *c, uint32_t *entries, int32_t *nnz) {
     std::vector<uint32_t> dt(c->dt_cap);
     if (s.D) EZ_TRY_HIP(c, hipMemcpy(di.data(), s.doc_info, di.size() * sizeof(uint2), hipMemcpyDeviceToHost));
     if (!dt.empty()) EZ_TRY_HIP(c, hipMemcpy(dt.data(), s.dt, dt.size() * 4, hipMemcpyDeviceToHost));
+    uint64_t canon = 0;  // rows packed in input doc order at sum_{e<d} min(n_e, K)
     for (int d = 0; d < s.D; ++d) {
         nnz[d] = (int32_t)(di[d].y & 0xFFFFu);
         for (int32_t i = 0; i < nnz[d]; ++i) {  // device rows hold topic << 2
             const uint32_t e = dt[di[d].x + i];
-            entries[di[d].x + i] = ((e & 0xFFFFu) >> 2) | (e & 0xFFFF0000u);
+            entries[canon + i] = ((e & 0xFFFFu) >> 2) | (e & 0xFFFF0000u);
         }
+        canon += std::min<uint32_t>(c->doc_off[d + 1] - c->doc_off[d], (uint32_t)s.K);
     }
     return EZLDA_OK;
 }
@@ -767,20 +792,6 @@ int ezlda_get_aux(ezlda_ctx *c, uint32_t *aux) {
     std::vector<uint32_t> a(c->s.N);
     EZ_TRY_HIP(c, hipMemcpy(a.data(), c->s.aux, a.size() * 4, hipMemcpyDeviceToHost));
     for (size_t i = 0; i < a.size(); ++i) aux[i] = a[c->doc_tok[i]];
-    return EZLDA_OK;
-}
-
-int ezlda_selftest_scan64(const float *in, float *out, int32_t nw) {
-    if (nw < 1 || !in || !out) return fail(nullptr, EZLDA_EINVAL, "bad selftest arguments");
-    float *di = nullptr, *dout = nullptr;
-    const size_t bytes = (size_t)nw * 64 * sizeof(float);
-    EZ_TRY_HIP(nullptr, hipMalloc(&di, bytes));
-    EZ_TRY_HIP(nullptr, hipMalloc(&dout, bytes));
-    EZ_TRY_HIP(nullptr, hipMemcpy(di, in, bytes, hipMemcpyHostToDevice));
-    EZ_TRY_HIP(nullptr, ez::launch_selftest_scan(di, dout, nw, nullptr));
-    EZ_TRY_HIP(nullptr, hipMemcpy(out, dout, bytes, hipMemcpyDeviceToHost));
-    hipFree(di);
-    hipFree(dout);
     return EZLDA_OK;
 }
 

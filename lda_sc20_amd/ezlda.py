@@ -28,7 +28,7 @@ EXPORTS = (
     "ezlda_load_checkpoint", "ezlda_gen_corpus", "ezlda_corpus_num_tokens", "ezlda_corpus_num_docs",
     "ezlda_corpus_num_words", "ezlda_corpus_doc_off", "ezlda_corpus_words", "ezlda_corpus_topics",
     "ezlda_corpus_free", "ezlda_corpus_from_arrays", "ezlda_read_docword", "ezlda_write_chunked",
-    "ezlda_read_chunked", "ezlda_selftest_scan64", "ezlda_selftest_philox", "ezlda_selftest_copy", "ezlda_get_stats",
+    "ezlda_read_chunked", "ezlda_selftest_philox", "ezlda_selftest_copy", "ezlda_get_stats",
     "ezlda_reset_stats",
 )
 
@@ -113,7 +113,6 @@ def lib() -> C.CDLL:
         "ezlda_read_docword": (P, [C.c_char_p]),
         "ezlda_write_chunked": (C.c_int, [C.c_char_p, P, P, C.c_int32, C.c_int32, C.c_uint64]),
         "ezlda_read_chunked": (P, [C.c_char_p, C.c_int32]),
-        "ezlda_selftest_scan64": (C.c_int, [P, P, C.c_int32]),
         "ezlda_selftest_philox": (C.c_int, [P, P, P, C.c_int32]),
         "ezlda_selftest_copy": (C.c_int, [C.c_int64, C.c_int32, C.POINTER(C.c_double)]),
         "ezlda_get_stats": (C.c_int, [P, P]),
@@ -354,16 +353,6 @@ class Context:
 
     def load_checkpoint(self, path: str):
         self._chk(self._L.ezlda_load_checkpoint(self._h, path.encode()))
-
-
-def selftest_scan64(x: np.ndarray) -> np.ndarray:
-    """Canonical wave64 scan of each 64-float row of x, computed on the GPU."""
-    a = np.ascontiguousarray(x, dtype=np.float32).reshape(-1, 64)
-    out = np.zeros_like(a)
-    r = lib().ezlda_selftest_scan64(_ptr(a), _ptr(out), a.shape[0])
-    if r:
-        raise EzldaError(r, last_error())
-    return out
 
 
 def selftest_philox(ctr: np.ndarray, key: np.ndarray) -> np.ndarray:

@@ -36,7 +36,6 @@ def lib(omp: bool = False) -> C.CDLL:
     P = C.c_void_p
     sig = {
         "orc_philox": (None, [P, P, P]),
-        "orc_scan64": (None, [P]),
         "orc_create": (P, [C.c_int, C.c_float, C.c_float, C.c_uint64, C.c_int, C.c_int, P, P, P, C.c_int]),
         "orc_destroy": (None, [P]),
         "orc_num_tokens": (C.c_int64, [P]),
@@ -80,13 +79,6 @@ def philox(ctr, key, omp: bool = False) -> np.ndarray:
     o = np.zeros(4, np.uint32)
     lib(omp).orc_philox(_p(c), _p(k), _p(o))
     return o
-
-
-def scan64(v) -> np.ndarray:
-    a = np.ascontiguousarray(v, dtype=np.float32).copy()
-    assert a.shape == (64,)
-    lib().orc_scan64(_p(a))
-    return a
 
 
 class Oracle:

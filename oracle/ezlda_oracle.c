@@ -33,36 +33,36 @@
  *        S  = sum_{k != k1} n_dk phi_wk   over the doc's sorted DT row
  *        Q  = alpha sum_{k != k1} phi_wk  over all topics
  *      u = Philox4x32-10(seed; token, sweep, 0).x >> 8 scaled by 2^-24
- *      (replaces curand_uniform, utility.cu:1353).  target = u * total; the
- *      first element of positive mass whose inclusive prefix exceeds the
- *      target wins (a tree scan is not monotone in f32, so the mass test
- *      keeps zero-mass elements out).  S is summed lane-major: DT entry i
- *      goes to lane i mod 64, each lane accumulates its entries left to
- *      right with fmaf (C99, correctly rounded: the device FMA) and
- *      the 64 lane sums go through the wave64 scan tree of scan64(); the Q
- *      bucket is scanned per 64-topic slab and the slabs summed left to
- *      right.  When rounding leaves no prefix above the target, the last
- *      element of positive mass wins.
+ *      (replaces curand_uniform, utility.cu:1353).  target = u * total.
+ *      S is accumulated left to right over the DT row with fused
+ *      multiply-adds (fmaf: C99, correctly rounded, as the device FMA); Q as
+ *      running sums inside each 64-topic slab plus a running sum of the slab
+ *      totals.  Running sums of non-negative terms never decrease, so the
+ *      draw is the first element whose running sum exceeds the target (it
+ *      always has positive mass); when rounding leaves no running sum above
+ *      the target, the last element of positive mass wins.
  *   6. max-topic shortcut (UpdateProbKernelTrainD1, utility.cu:2680-3312;
  *      inline tail test utility.cu:1341-1373): accept k1 when
- *      u * (P1 + maxS + Q) < P1 * (1 - 2^-10), maxS = n_dk2 phi_k2 +
- *      (n_d - n_dk1 - n_dk2) phi_k3 >= S.  The 2^-10 margin exceeds the worst
- *      f32 rounding of the full path, so the shortcut only ever accepts tokens
- *      the full path also sends to k1: shortcut on and off give identical z.
+ *      u * (P1 + maxS + Q) < P1 * (1 - 2^-9), maxS = n_dk2 phi_k2 +
+ *      (n_d - n_dk1 - n_dk2) phi_k3 >= S.  The full path's target carries at
+ *      most (nnz + 4) roundings of 2^-24 (nnz <= K <= 16383), below the
+ *      2^-9 margin, so the shortcut only ever accepts tokens the full path
+ *      also sends to k1: shortcut on and off give identical z.
  *   7. log-likelihood in FP64: sum_t log(sum_k theta_dk phi_wk), the per-token
  *      term of profilePerp/utility.cu:2398,2810 (log((S+Q)/(n_d+K alpha))),
  *      summed over all tokens (profilePerp/utility.cu:2858-2928).
  *
  * Token ids: tokens are numbered in word-major order -- words by descending
  * count, ties by ascending word id (preprocess/streamTestBalance.cpp:281-301
- * ranks words the same way), and within a word by (doc, position in doc).
+ * ranks words the same way); within a word, docs by descending length (ties
+ * by ascending doc id), then position in the doc.
  */
 #include <math.h>
 #include <stdint.h>
 #include <stdlib.h>
 #include <string.h>
 
-#define ORC_MARGIN 0.9990234375f /* 1 - 2^-10 */
+#define ORC_MARGIN 0.998046875f /* 1 - 2^-9 */
 #define ORC_NONE 0xFFFFu
 
 /* ---------------------------------------------------------------- Philox */
@@ -93,26 +93,6 @@ static uint32_t philox_x0(uint64_t seed, uint64_t tok, uint32_t sweep, uint32_t 
 }
 
 static float u24(uint32_t x) { return (float)(x >> 8) * 5.9604644775390625e-08f; }
-
-/* ------------------------------------------------------------ wave scan */
-/* Inclusive scan of 64 floats in the canonical tree: Hillis-Steele inside
- * each 16-element row (shifts 1,2,4,8), then rows 1 and 3 add the last
- * element of rows 0 and 2, then rows 2 and 3 add element 31.  Element 63 is
- * ((R3 + R2) + (R1 + R0)) of the row totals.  Replaces the reference's
- * 32-wide shfl_up scans (utility.cu:1303-1309, :1491-1496). */
-void orc_scan64(float v[64]) {
-    float o[64];
-    for (int h = 1; h <= 8; h <<= 1) {
-        memcpy(o, v, sizeof o);
-        for (int i = 0; i < 64; ++i)
-            if ((i & 15) >= h) v[i] = o[i] + o[i - h];
-    }
-    memcpy(o, v, sizeof o);
-    for (int i = 16; i < 32; ++i) v[i] = o[i] + o[15];
-    for (int i = 48; i < 64; ++i) v[i] = o[i] + o[47];
-    memcpy(o, v, sizeof o);
-    for (int i = 32; i < 64; ++i) v[i] = o[i] + o[31];
-}
 
 /* ------------------------------------------------------------ the state */
 typedef struct {
@@ -184,10 +164,29 @@ void *orc_create(int K, float alpha, float beta, uint64_t seed, int V, int D,
     o->doc_tok = (int64_t *)xcalloc(N, sizeof(int64_t));
     o->doc_wrd = (int32_t *)xcalloc(N, sizeof(int32_t));
     o->z = (uint16_t *)xcalloc(N, sizeof(uint16_t));
+    /* docs by length desc, id asc: tokens of a word come doc by doc in this
+     * order (so neighbouring tokens of a word have similar DT row lengths) */
+    int32_t *dord = (int32_t *)xcalloc(D, sizeof(int32_t));
+    for (int d = 0; d < D; ++d) dord[d] = d;
+    for (int gap = D / 2; gap > 0; gap /= 2)
+        for (int i = gap; i < D; ++i) {
+            int32_t x = dord[i];
+            int64_t lx = doc_off[x + 1] - doc_off[x];
+            int j = i;
+            while (j >= gap) {
+                int32_t y = dord[j - gap];
+                int64_t ly = doc_off[y + 1] - doc_off[y];
+                if (ly > lx || (ly == lx && y < x)) break;
+                dord[j] = y;
+                j -= gap;
+            }
+            dord[j] = x;
+        }
     int64_t *cur = (int64_t *)xcalloc(V, sizeof(int64_t));
     for (int r = 0; r < V; ++r) cur[r] = o->word_off[r];
-    for (int d = 0; d < D; ++d)
-        for (int64_t i = doc_off[d]; i < doc_off[d + 1]; ++i) {
+    for (int q = 0; q < D; ++q)
+        for (int64_t i = doc_off[dord[q]]; i < doc_off[dord[q] + 1]; ++i) {
+            const int d = dord[q];
             int r = o->rank_of[words[i]];
             int64_t t = cur[r]++;
             o->tok_doc[t] = d;
@@ -198,7 +197,7 @@ void *orc_create(int K, float alpha, float beta, uint64_t seed, int V, int D,
     if (!z0)
         for (int64_t t = 0; t < N; ++t)
             o->z[t] = (uint16_t)(philox_x0(seed, (uint64_t)t, 0, 1u) % (uint32_t)K);
-    free(cur); free(ids); free(cnt);
+    free(cur); free(ids); free(cnt); free(dord);
     o->nwk = (int32_t *)xcalloc((size_t)V * K, sizeof(int32_t));
     o->nk = (int32_t *)xcalloc(K, sizeof(int32_t));
     o->dt_off = (int64_t *)xcalloc(D + 1, sizeof(int64_t));
@@ -325,71 +324,50 @@ void orc_build_dt(void *h) {
 /* ------------------------------------------------------- 5-6. sampler */
 /* Full three-bucket draw for one token (LDAKernelTrain(D), utility.cu:
  * 1217-1611, :1625-2103; bucket search :1878-1992).  phi has phi[k1] = 0.
- * S bucket in lane-major order: lane l (0..63) accumulates the products of
- * DT entries l, l+64, l+128, ... left to right with fused multiply-adds
- * (acc = fma(n_dk, phi_k, acc), one rounding each); the 64 lane sums go through
- * scan64(); the draw picks the lane whose prefix first exceeds the target,
- * then walks that lane's entries in the same left-to-right order. */
-static int draw_full(const orc_t *o, const float *phi, const float *qtot, const float *qacc,
+ * S: the DT row (ascending topic) accumulated left to right with fused
+ * multiply-adds, run = fma(n_dk, phi_k, run), one rounding per entry; the
+ * draw takes the first entry whose running sum exceeds the target.
+ * Q: qs[k] is the running sum of alpha*phi inside each 64-topic slab,
+ * qacc[j] the running sum of the slab totals; the draw takes the first slab,
+ * then the first topic in it, whose running sum exceeds the target.  All
+ * running sums of non-negative terms are non-decreasing, so "first" is
+ * well defined and can be found by bisection. */
+static int draw_full(const orc_t *o, const float *phi, const float *qs, const float *qt, const float *qacc,
                      int nsq, int k1, float P1, float Q, float u, int d) {
     const uint32_t *row = o->dt + o->dt_off[d];
     const int nnz = o->dt_nnz[d];
-    float a[64], sc[64];
-    for (int l = 0; l < 64; ++l) {
-        float acc = 0.0f;
-        for (int i = l; i < nnz; i += 64) acc = fmaf((float)(row[i] >> 16), phi[row[i] & 0xFFFFu], acc);
-        a[l] = acc;
-        sc[l] = acc;
-    }
-    orc_scan64(sc);
-    const float S = sc[63];
+    float S = 0.0f;
+    for (int i = 0; i < nnz; ++i) S = fmaf((float)(row[i] >> 16), phi[row[i] & 0xFFFFu], S);
     const float A = P1 + S;
     const float total = A + Q;
     const float target = u * total;
     if (target < P1) return k1;
     if (target < A) {
         const float t2 = target - P1;
-        int ls = -1;
-        for (int l = 0; l < 64; ++l)
-            if (sc[l] > t2 && a[l] > 0.0f) { ls = l; break; }
-        if (ls < 0)
-            for (int l = 63; l >= 0; --l)
-                if (a[l] > 0.0f) { ls = l; break; }
-        if (ls < 0) return k1;
-        const float t3 = t2 - (ls > 0 ? sc[ls - 1] : 0.0f);
         float run = 0.0f;
         int last = -1;
-        for (int i = ls; i < nnz; i += 64) {
+        for (int i = 0; i < nnz; ++i) {
             const float c = (float)(row[i] >> 16), f = phi[row[i] & 0xFFFFu];
             run = fmaf(c, f, run);
-            if (c > 0.0f && f > 0.0f) {
-                if (run > t3) return (int)(row[i] & 0xFFFFu);
-                last = i;
-            }
+            if (run > t2) return (int)(row[i] & 0xFFFFu);
+            if (c > 0.0f && f > 0.0f) last = i;
         }
         return last >= 0 ? (int)(row[last] & 0xFFFFu) : k1;
     }
-    float v[64];
     const float t2 = target - A;
     int js = -1;
     for (int j = 0; j < nsq; ++j)
         if (qacc[j] > t2) { js = j; break; }
     if (js < 0)
         for (int j = nsq - 1; j >= 0; --j)
-            if (qtot[j] > 0.0f) { js = j; break; }
+            if (qt[j] > 0.0f) { js = j; break; }
     if (js < 0) return k1;
     const float t3 = t2 - (js > 0 ? qacc[js - 1] : 0.0f);
-    float p[64];
-    for (int l = 0; l < 64; ++l) {
-        int k = 64 * js + l;
-        p[l] = k < o->K ? o->alpha * phi[k] : 0.0f;
-        v[l] = p[l];
-    }
-    orc_scan64(v);
-    for (int l = 0; l < 64; ++l)
-        if (64 * js + l < o->K && v[l] > t3 && p[l] > 0.0f) return 64 * js + l;
-    for (int l = 63; l >= 0; --l)
-        if (p[l] > 0.0f) return 64 * js + l;
+    const int k0 = 64 * js, k_end = (k0 + 64 < o->K) ? k0 + 64 : o->K;
+    for (int k = k0; k < k_end; ++k)
+        if (qs[k] > t3) return k;
+    for (int k = k_end - 1; k >= k0; --k)
+        if (o->alpha * phi[k] > 0.0f) return k;
     return k1;
 }
 
@@ -401,9 +379,9 @@ void orc_sample(void *h) {
 #pragma omp parallel
     {
         float *phi = (float *)xcalloc(K, sizeof(float));
-        float *qtot = (float *)xcalloc(nsq, sizeof(float));
+        float *qs = (float *)xcalloc(K, sizeof(float));
+        float *qt = (float *)xcalloc(nsq, sizeof(float));
         float *qacc = (float *)xcalloc(nsq, sizeof(float));
-        float v[64];
 #pragma omp for schedule(dynamic, 16)
         for (int w = 0; w < o->V; ++w) {
             if (o->word_off[w + 1] == o->word_off[w]) continue;
@@ -416,13 +394,13 @@ void orc_sample(void *h) {
             phi[k1] = 0.0f;
             float acc = 0.0f;
             for (int j = 0; j < nsq; ++j) {
-                for (int l = 0; l < 64; ++l) {
-                    int k = 64 * j + l;
-                    v[l] = k < K ? alpha * phi[k] : 0.0f;
+                float run = 0.0f;
+                for (int k = 64 * j; k < 64 * j + 64 && k < K; ++k) {
+                    run = run + alpha * phi[k];
+                    qs[k] = run;
                 }
-                orc_scan64(v);
-                qtot[j] = v[63];
-                acc = acc + v[63];
+                qt[j] = run;
+                acc = acc + run;
                 qacc[j] = acc;
             }
             const float Q = qacc[nsq - 1];
@@ -441,10 +419,10 @@ void orc_sample(void *h) {
                         continue;
                     }
                 }
-                o->z[t] = (uint16_t)draw_full(o, phi, qtot, qacc, nsq, k1, P1, Q, u, d);
+                o->z[t] = (uint16_t)draw_full(o, phi, qs, qt, qacc, nsq, k1, P1, Q, u, d);
             }
         }
-        free(phi); free(qtot); free(qacc);
+        free(phi); free(qs); free(qt); free(qacc);
     }
     o->sweep++;
     o->counts_fresh = 0;

@@ -26,20 +26,6 @@ def _pair(off, words, V, K, seed=21, shortcut=True, z0=None, **kw):
 
 
 # ------------------------------------------------------------ primitives
-def test_device_scan_matches_oracle_tree():
-    rng = np.random.default_rng(0)
-    rows = [(rng.random(64) ** 6).astype(np.float32) * rng.random(64).astype(np.float32) for _ in range(300)]
-    v = np.full(64, 2.0**-24, np.float32)
-    v[0] = 1.0
-    rows.append(v)
-    rows.append(v[::-1].copy())
-    rows.append(np.float32(3.0) ** np.arange(64, dtype=np.float32) % np.float32(1e6))
-    x = np.stack(rows).astype(np.float32)
-    got = ezlda.selftest_scan64(x)
-    exp = np.stack([ob.scan64(r) for r in x])
-    assert np.array_equal(got.view(np.uint32), exp.view(np.uint32))
-
-
 def test_device_philox_matches_oracle_and_kats():
     rng = np.random.default_rng(1)
     ctr = rng.integers(0, 2**32, (500, 4), dtype=np.uint64).astype(np.uint32)
