@@ -53,6 +53,18 @@ def device_sync() -> None:
         pass
 
 
+def measured_traffic(workload: str):
+    """HBM bytes per k_sample launch measured by rocprofv3 on this workload
+    (two --pmc runs, FETCH_SIZE then WRITE_SIZE, scaled by a copy kernel of
+    known size), or None when no such measurement is committed."""
+    path = os.path.join(ROOT, "profiles", f"hbm_traffic_{workload}.json")
+    try:
+        with open(path) as f:
+            return float(json.load(f)["hbm_bytes_per_launch"])
+    except (OSError, KeyError, ValueError):
+        return None
+
+
 def cpu_baseline(corpus, K: int, seed: int, max_docs: int, sweeps: int = 2):
     """The OpenMP build of the CPU oracle (same sweep semantics, bit-exact)
     on the first max_docs docs of the same corpus; tokens/s over `sweeps`
@@ -170,8 +182,9 @@ def main() -> int:
                    "num_head_words": VD, "shortcut": True, "sweeps_before_timing": args.warmup,
                    "doc_tiles": int(info.doc_tiles)},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None, "kernel": "k_sample",
-                     "bytes_per_launch": b_sample, "avg_launch_ms": t_sample * 1e3},
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": measured_traffic(args.workload),
+                     "kernel": "k_sample", "bytes_per_launch": b_sample, "avg_launch_ms": t_sample * 1e3,
+                     "traffic_source": "profiles/hbm_traffic_cfg2.md (FETCH_SIZE, WRITE_SIZE; copy-calibrated)"},
         "stage_ms_per_sweep": {k: v / K_steps for k, v in stage_ms.items()},
         "sweep_algorithmic_GBs": b_sweep / (ms_per_step / 1e3) / 1e9,
         "full_draw_fraction": n_full / (N * K_steps),

@@ -1,0 +1,134 @@
+// C++ host surface over the C ABI (include/ezlda.h): RAII handles plus the
+// reference's launcher names, so a driver written against EZLDA's
+// mainKernel.cu reads the same.  Header-only; link with -lezlda.
+//
+// Launcher mapping (reference -> this build):
+//   UpdateWTDenKernel, UpdateWTKernel, WTAdditionKernel,
+//   UpdateWTDenRowSumKernel (WTDenUpdateKernel.cuh:25-26, WTUpdateKernel.cuh:23,
+//   WTAddKernel.cuh:24)                    -> Context::build_wt (one call)
+//   MaxTopicKernel (SamplingKernel.cuh:12) -> Context::topk
+//   UpdateDTKernel (DTUpdateKernel.cuh:13) -> Context::build_dt
+//   UpdateProbKernelD, SampleKernelD, SampleKernel (SamplingKernel.cuh:10,11,15)
+//                                          -> Context::sample (one call)
+//   PerplexityKernel (SamplingKernel.cuh:13) -> Context::loglik
+// The reference launchers return void and leave errors unchecked; these
+// throw ezlda::Error carrying the library's status and message.
+#pragma once
+
+#include <cstdint>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "ezlda.h"
+
+namespace ezlda {
+
+class Error : public std::runtime_error {
+public:
+    Error(int code, const std::string &msg) : std::runtime_error(msg), code_(code) {}
+    int code() const { return code_; }
+
+private:
+    int code_;
+};
+
+inline void check(int rc, const ezlda_ctx *ctx) {
+    if (rc != EZLDA_OK) throw Error(rc, ezlda_last_error(ctx));
+}
+
+class Corpus {
+public:
+    explicit Corpus(ezlda_corpus *h) : h_(h) {
+        if (!h_) throw Error(EZLDA_EIO, ezlda_last_error(nullptr));
+    }
+    Corpus(const Corpus &) = delete;
+    Corpus &operator=(const Corpus &) = delete;
+    Corpus(Corpus &&o) noexcept : h_(o.h_) { o.h_ = nullptr; }
+    ~Corpus() { ezlda_corpus_free(h_); }
+
+    static Corpus generate(const ezlda_gen_params &p) { return Corpus(ezlda_gen_corpus(&p)); }
+    static Corpus read_docword(const std::string &path) { return Corpus(ezlda_read_docword(path.c_str())); }
+    static Corpus read_chunked(const std::string &prefix, int chunks = 0) {
+        return Corpus(ezlda_read_chunked(prefix.c_str(), chunks));
+    }
+    void write_chunked(const std::string &prefix, int K, int chunks, uint64_t seed,
+                       const uint16_t *z0 = nullptr) const {
+        check(ezlda_write_chunked(prefix.c_str(), h_, z0, K, chunks, seed), nullptr);
+    }
+
+    int64_t tokens() const { return ezlda_corpus_num_tokens(h_); }
+    int32_t docs() const { return ezlda_corpus_num_docs(h_); }
+    int32_t words() const { return ezlda_corpus_num_words(h_); }
+    const int64_t *doc_off() const { return ezlda_corpus_doc_off(h_); }
+    const int32_t *word_ids() const { return ezlda_corpus_words(h_); }
+    const uint16_t *topics() const { return ezlda_corpus_topics(h_); }
+
+private:
+    ezlda_corpus *h_;
+};
+
+inline ezlda_args default_args() {
+    ezlda_args a;
+    ezlda_default_args(&a);
+    return a;
+}
+
+class Context {
+public:
+    explicit Context(const ezlda_args &a) : h_(ezlda_create(&a)) {
+        if (!h_) throw Error(EZLDA_EHIP, ezlda_last_error(nullptr));
+    }
+    Context(const Context &) = delete;
+    Context &operator=(const Context &) = delete;
+    ~Context() { ezlda_destroy(h_); }
+
+    void load(const Corpus &c) {
+        check(ezlda_load_docs(h_, c.words(), c.docs(), c.doc_off(), c.word_ids(), c.topics()), h_);
+    }
+    void load_chunked(const std::string &prefix, int chunks = 0) {
+        check(ezlda_load_chunked(h_, prefix.c_str(), chunks), h_);
+    }
+    ezlda_info info() const {
+        ezlda_info i;
+        check(ezlda_get_info(h_, &i), h_);
+        return i;
+    }
+    void build_wt() { check(ezlda_build_wt(h_), h_); }
+    void topk() { check(ezlda_topk(h_), h_); }
+    void build_dt() { check(ezlda_build_dt(h_), h_); }
+    void sample() { check(ezlda_sample(h_), h_); }
+    void sweep(int n = 1) { check(ezlda_sweep(h_, n), h_); }
+    void sweep_graph(int n = 1) { check(ezlda_sweep_graph(h_, n), h_); }
+    double loglik() {
+        double ll = 0.0;
+        check(ezlda_loglik(h_, &ll), h_);
+        return ll;
+    }
+    // Wall ms of n back-to-back sweeps; stage_ms gets EZLDA_NUM_STAGES values.
+    double time_sweeps(int n, bool graph, double *stage_ms = nullptr) {
+        double ms = 0.0, st[EZLDA_NUM_STAGES];
+        check(ezlda_time_sweeps(h_, n, graph ? 1 : 0, &ms, stage_ms ? stage_ms : st), h_);
+        return ms;
+    }
+    std::vector<uint16_t> topics() {
+        std::vector<uint16_t> z((size_t)info().num_tokens);
+        check(ezlda_get_topics(h_, z.data()), h_);
+        return z;
+    }
+    void save_checkpoint(const std::string &p) { check(ezlda_save_checkpoint(h_, p.c_str()), h_); }
+    void load_checkpoint(const std::string &p) { check(ezlda_load_checkpoint(h_, p.c_str()), h_); }
+    ezlda_ctx *handle() { return h_; }
+
+private:
+    ezlda_ctx *h_;
+};
+
+// The reference's launcher names (see the mapping above).
+inline void UpdateWTKernel(Context &c) { c.build_wt(); }
+inline void MaxTopicKernel(Context &c) { c.topk(); }
+inline void UpdateDTKernel(Context &c) { c.build_dt(); }
+inline void SampleKernel(Context &c) { c.sample(); }
+inline double PerplexityKernel(Context &c) { return c.loglik(); }
+
+}  // namespace ezlda
