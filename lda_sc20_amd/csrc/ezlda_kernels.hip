@@ -237,7 +237,7 @@ __global__ void k_word_top(int V, int VD, int K, float beta, const int32_t *__re
 // topics of its tokens into a 16-bit LDS histogram, writes each token's
 // (n_dk1, n_dk2) at its word's top-2 topics (the re-pack of :125-161), and
 // compacts the row in ascending topic order (:183-238).  A row starts on a
-// 16-byte boundary and is zero-padded to a multiple of 4 entries.
+// 128-byte boundary and is zero-padded to a multiple of 32 entries.
 __global__ void k_dt_build(int D, int K, const uint32_t *__restrict__ doc_off, const uint32_t *__restrict__ doc_tok,
                            const uint32_t *__restrict__ doc_wrd, const uint16_t *__restrict__ z,
                            const ushort4 *__restrict__ wtop, uint32_t *__restrict__ aux,
@@ -272,15 +272,34 @@ __global__ void k_dt_build(int D, int K, const uint32_t *__restrict__ doc_off, c
                 if (k < K && !(k & 1)) h[k >> 1] = 0;
             }
             wave_lds_sync();
-            // zero the row up to the next multiple of 4 entries: the sampler
-            // reads rows 16 bytes at a time and a zero entry adds nothing
-            if ((uint32_t)lane < ((4u - (nz & 3u)) & 3u)) dt[base + nz + lane] = 0u;
+            // zero the row up to the next multiple of 32 entries: the sampler
+            // reads rows in 128-byte blocks and a zero entry adds nothing
+            if ((uint32_t)lane < ((32u - (nz & 31u)) & 31u)) dt[base + nz + lane] = 0u;
         }
         if (lane == 0) doc_info[d].y = nz | ((e - b) << 16);
     }
 }
 
 // ------------------------------------------------------------- sampler
+// Checkpoints of the S running sum kept per lane (one per 32-entry block).
+static constexpr int kCP = 16;
+
+// Running FMA sum over one 32-entry (128-byte) block of a DT row: eight
+// 16-byte loads issued together, then the entries in order.
+__device__ __forceinline__ float block_sum(const uint4 *__restrict__ p, const float *phi, float s) {
+    uint4 e[8];
+#pragma unroll
+    for (int i = 0; i < 8; ++i) e[i] = p[i];
+#pragma unroll
+    for (int i = 0; i < 8; ++i) {
+        s = __builtin_fmaf(dt_count(e[i].x), phi_of(phi, e[i].x), s);
+        s = __builtin_fmaf(dt_count(e[i].y), phi_of(phi, e[i].y), s);
+        s = __builtin_fmaf(dt_count(e[i].z), phi_of(phi, e[i].z), s);
+        s = __builtin_fmaf(dt_count(e[i].w), phi_of(phi, e[i].w), s);
+    }
+    return s;
+}
+
 // One workgroup per sampler item (a word, or a doc tile / token segment of a
 // head word).  The workgroup stages phi_w in LDS with phi[k1] = 0, and the Q
 // bucket's running sums: qs[k] inside each 64-topic slab, qacc[j] over the
@@ -377,17 +396,20 @@ __global__ void __launch_bounds__(1024) k_sample(
         if (need) {
             const uint4 *rp = reinterpret_cast<const uint4 *>(dt + di.x);
             const uint32_t nnz = di.y & 0xFFFFu;
-            const uint32_t ng = (nnz + 3u) >> 2;
+            const uint32_t nb = (nnz + 31u) >> 5;  // 32-entry (128-byte) blocks
             n_full += 1u;
             n_scanned += nnz;
+            // S: one FMA per entry, left to right; cp[b] = running sum at the
+            // end of block b (the first kCP blocks), so the S-bucket search
+            // re-reads one block instead of the row up to the target
+            float cp[kCP];
             float S = 0.0f;
-            for (uint32_t g = 0; g < ng; ++g) {
-                const uint4 e = rp[g];
-                S = __builtin_fmaf(dt_count(e.x), phi_of(phi, e.x), S);
-                S = __builtin_fmaf(dt_count(e.y), phi_of(phi, e.y), S);
-                S = __builtin_fmaf(dt_count(e.z), phi_of(phi, e.z), S);
-                S = __builtin_fmaf(dt_count(e.w), phi_of(phi, e.w), S);
+#pragma unroll
+            for (int b = 0; b < kCP; ++b) {
+                if ((uint32_t)b < nb) S = block_sum(rp + 8 * b, phi, S);
+                cp[b] = S;
             }
+            for (uint32_t b = kCP; b < nb; ++b) S = block_sum(rp + 8 * b, phi, S);
             const float A = P1 + S;
             const float total = A + Q;
             const float target = u * total;
@@ -395,9 +417,26 @@ __global__ void __launch_bounds__(1024) k_sample(
                 newz = k1;
             } else if (target < A) {
                 const float t2 = target - P1;
+                int bs = -1;
+#pragma unroll
+                for (int b = kCP - 1; b >= 0; --b)
+                    if (cp[b] > t2) bs = b;
                 float run = 0.0f;
+                uint32_t b0 = 0, b1 = nb;
+                if (bs >= 0) {
+#pragma unroll
+                    for (int b = 0; b < kCP - 1; ++b)
+                        if (b + 1 == bs) run = cp[b];
+                    b0 = (uint32_t)bs;
+                    b1 = b0 + 1u;
+                } else if (nb > (uint32_t)kCP) {
+                    run = cp[kCP - 1];
+                    b0 = kCP;
+                } else {
+                    b1 = 0;  // no running sum exceeds t2
+                }
                 bool found = false;
-                for (uint32_t g = 0; g < ng; ++g) {
+                for (uint32_t g = b0 * 8u; g < b1 * 8u && !found; ++g) {
                     const uint4 e = rp[g];
                     const float r0 = __builtin_fmaf(dt_count(e.x), phi_of(phi, e.x), run);
                     const float r1 = __builtin_fmaf(dt_count(e.y), phi_of(phi, e.y), r0);
@@ -406,7 +445,6 @@ __global__ void __launch_bounds__(1024) k_sample(
                     if (r3 > t2) {
                         newz = dt_topic(r0 > t2 ? e.x : r1 > t2 ? e.y : r2 > t2 ? e.z : e.w);
                         found = true;
-                        break;
                     }
                     run = r3;
                 }

@@ -411,15 +411,15 @@ int ezlda_load_docs(ezlda_ctx *c, int32_t V, int32_t D, const int64_t *doc_off, 
         const uint32_t n = c->word_off[r + 1] - c->word_off[r];
         tail_off[r - VD + 1] = tail_off[r - VD] + std::min<uint32_t>(n, (uint32_t)K);
     }
-    // DT rows in the doc order above, each 16-byte aligned with capacity
-    // min(n_d, K) rounded up to 4 entries (the build zero-fills the rest)
+    // DT rows in the doc order above, each 128-byte aligned with capacity
+    // min(n_d, K) rounded up to 32 entries (the build zero-fills the rest)
     std::vector<uint2> doc_info(D);
     uint64_t dtc = 0;
     for (int q = 0; q < D; ++q) {
         const int d = dord[q];
         const uint32_t n = c->doc_off[d + 1] - c->doc_off[d];
         doc_info[d] = make_uint2((uint32_t)dtc, n << 16);
-        dtc += (std::min<uint32_t>(n, (uint32_t)K) + 3u) & ~3u;
+        dtc += (std::min<uint32_t>(n, (uint32_t)K) + 31u) & ~31u;
     }
     if (dtc >= 0xFFFFFFF0ull) return fail(c, EZLDA_EINVAL, "DT capacity exceeds 2^32 entries");
     // work items
