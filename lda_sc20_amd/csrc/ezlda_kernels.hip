@@ -13,6 +13,16 @@
 #include "ezlda_device.h"
 #include "ezlda_philox.h"
 
+// Sampler row blocking (16-byte loads per block, checkpoints per lane).
+// DT rows are padded to multiples of 32 entries, so blocks up to 8 loads
+// never read past a row's padding.
+#ifndef EZ_SAMPLE_BLK
+#define EZ_SAMPLE_BLK 4
+#endif
+#ifndef EZ_SAMPLE_CP
+#define EZ_SAMPLE_CP 32
+#endif
+
 namespace ez {
 
 static constexpr float kMargin = 0.998046875f;  // 1 - 2^-9
@@ -281,17 +291,19 @@ __global__ void k_dt_build(int D, int K, const uint32_t *__restrict__ doc_off, c
 }
 
 // ------------------------------------------------------------- sampler
-// Checkpoints of the S running sum kept per lane (one per 32-entry block).
-static constexpr int kCP = 16;
+// Checkpoints of the S running sum kept per lane (one per block of kBlkU4
+// 16-byte loads, i.e. 4*kBlkU4 DT entries).
+static constexpr int kCP = EZ_SAMPLE_CP;
+static constexpr int kBlkU4 = EZ_SAMPLE_BLK;
 
-// Running FMA sum over one 32-entry (128-byte) block of a DT row: eight
-// 16-byte loads issued together, then the entries in order.
+// Running FMA sum over one block of a DT row: kBlkU4 16-byte loads issued
+// together, then the entries in order.
 __device__ __forceinline__ float block_sum(const uint4 *__restrict__ p, const float *phi, float s) {
-    uint4 e[8];
+    uint4 e[kBlkU4];
 #pragma unroll
-    for (int i = 0; i < 8; ++i) e[i] = p[i];
+    for (int i = 0; i < kBlkU4; ++i) e[i] = p[i];
 #pragma unroll
-    for (int i = 0; i < 8; ++i) {
+    for (int i = 0; i < kBlkU4; ++i) {
         s = __builtin_fmaf(dt_count(e[i].x), phi_of(phi, e[i].x), s);
         s = __builtin_fmaf(dt_count(e[i].y), phi_of(phi, e[i].y), s);
         s = __builtin_fmaf(dt_count(e[i].z), phi_of(phi, e[i].z), s);
@@ -396,7 +408,7 @@ __global__ void __launch_bounds__(1024) k_sample(
         if (need) {
             const uint4 *rp = reinterpret_cast<const uint4 *>(dt + di.x);
             const uint32_t nnz = di.y & 0xFFFFu;
-            const uint32_t nb = (nnz + 31u) >> 5;  // 32-entry (128-byte) blocks
+            const uint32_t nb = (nnz + 4u * kBlkU4 - 1u) / (4u * kBlkU4);  // blocks of the row
             n_full += 1u;
             n_scanned += nnz;
             // S: one FMA per entry, left to right; cp[b] = running sum at the
@@ -406,10 +418,10 @@ __global__ void __launch_bounds__(1024) k_sample(
             float S = 0.0f;
 #pragma unroll
             for (int b = 0; b < kCP; ++b) {
-                if ((uint32_t)b < nb) S = block_sum(rp + 8 * b, phi, S);
+                if ((uint32_t)b < nb) S = block_sum(rp + kBlkU4 * b, phi, S);
                 cp[b] = S;
             }
-            for (uint32_t b = kCP; b < nb; ++b) S = block_sum(rp + 8 * b, phi, S);
+            for (uint32_t b = kCP; b < nb; ++b) S = block_sum(rp + kBlkU4 * b, phi, S);
             const float A = P1 + S;
             const float total = A + Q;
             const float target = u * total;
@@ -436,7 +448,7 @@ __global__ void __launch_bounds__(1024) k_sample(
                     b1 = 0;  // no running sum exceeds t2
                 }
                 bool found = false;
-                for (uint32_t g = b0 * 8u; g < b1 * 8u && !found; ++g) {
+                for (uint32_t g = b0 * kBlkU4; g < b1 * kBlkU4 && !found; ++g) {
                     const uint4 e = rp[g];
                     const float r0 = __builtin_fmaf(dt_count(e.x), phi_of(phi, e.x), run);
                     const float r1 = __builtin_fmaf(dt_count(e.y), phi_of(phi, e.y), r0);
