@@ -87,6 +87,129 @@ def cpu_baseline(corpus, K: int, seed: int, max_docs: int, sweeps: int = 2):
                       f"{sweeps} sweeps after 1 warm-up sweep, OpenMP oracle ({cores} threads)"}
 
 
+# fold-in requests: further documents of the training corpus's generative
+# model (same seed and topics, documents numbered after the training ones)
+SERVE_GEN = dict(len_dist=1, len_mean=300.0, len_sigma=0.8, len_min=10, len_max=5000)
+SERVE_SWEEPS = 20
+
+
+def _wait_until(t: float) -> None:
+    while True:
+        d = t - time.perf_counter()
+        if d <= 0:
+            return
+        if d > 2e-3:
+            time.sleep(d - 1e-3)
+
+
+def foldin_perplexity(wt, nk, beta: float, alpha: float, docs, counts) -> float:
+    """FP64 perplexity of fold-in documents: p(w|d) = sum_k theta_dk phi_wk with
+    theta_dk = (n_dk + alpha) / (n_d + K alpha) from the folded-in counts."""
+    V, K = wt.shape
+    inv = 1.0 / (nk.astype(np.float64) + V * beta)
+    ll, n = 0.0, 0
+    for w, c in zip(docs, counts):
+        theta = (c + alpha) / (c.sum() + K * alpha)
+        ll += float(np.log(((wt[w].astype(np.float64) + beta) * inv) @ theta).sum())
+        n += len(w)
+    return math.exp(-ll / n)
+
+
+def serving_bench(ctx, ezlda, wl: dict, args) -> dict:
+    """Fold-in serving against the trained model frozen: an open-loop burst
+    measures batched capacity, then Poisson arrivals at fractions of it give
+    p50/p99 latency; the plain one-document-per-launch path runs beside it."""
+    fr = [float(x) for x in args.serve_levels.split(",") if x]
+    n_burst, secs = args.serve_docs, args.serve_seconds
+    K = ctx.args.K
+    alpha = float(np.float32(50.0 / K))
+    g = dict(wl["gen"])
+    base_docs = g.pop("D")
+    g.update(SERVE_GEN)
+    t = time.perf_counter()
+    # enough documents for the burst plus the paced levels at up to 1.5x the
+    # burst's rate; generated once, reused round-robin if a level needs more
+    req = ezlda.gen_corpus(D=max(4 * n_burst, 1), first_doc=base_docs, **g)
+    docs = [req.words[req.doc_off[i]: req.doc_off[i + 1]] for i in range(req.D)]
+    log(f"serving: {req.D} request docs ({req.N} tokens) in {time.perf_counter() - t:.1f}s")
+    srv = ezlda.Server(ctx, max_slots=args.serve_slots, sweeps=SERVE_SWEEPS,
+                       hbm_budget_bytes=args.serve_budget, seed=wl["seed"] + 7)
+    rid = 0
+
+    def run(n: int, qps):
+        nonlocal rid
+        srv.reset_stats()
+        first = rid
+        rng = np.random.default_rng(rid + 1)
+        gaps = rng.exponential(1.0 / qps, n) if qps else np.zeros(n)
+        t0 = time.perf_counter()
+        at = t0 + np.cumsum(gaps)
+        for j in range(n):
+            if qps:
+                _wait_until(at[j])
+            srv.submit(rid, docs[rid % len(docs)])
+            rid += 1
+        srv.drain()
+        wall = time.perf_counter() - t0
+        st = srv.stats()
+        out = {}
+        for r in range(first, rid):
+            _, c, _ = srv.wait(r, len(docs[r % len(docs)]), timeout_ms=60000)
+            out[r] = c
+        lvl = {"offered_qps": qps, "docs": n, "docs_per_s": st.docs_per_s, "p50_ms": st.p50_ms,
+               "p99_ms": st.p99_ms, "max_ms": st.max_ms, "mean_ms": st.mean_ms, "max_active": st.max_active,
+               "launches": st.steps, "wall_s": wall}
+        return lvl, out
+
+    run(min(256, n_burst), None)  # warm-up
+    burst, burst_counts = run(n_burst, None)
+    cap = burst["docs_per_s"]
+    log(f"serving burst: {n_burst} docs, {cap:.0f} docs/s, p50 {burst['p50_ms']:.1f} ms, "
+        f"p99 {burst['p99_ms']:.1f} ms, max active {burst['max_active']}")
+    levels = []
+    for f in fr:
+        qps = f * cap
+        n = max(16, int(qps * secs))
+        lvl, _ = run(n, qps)
+        lvl["fraction_of_burst_rate"] = f
+        levels.append(lvl)
+        log(f"serving at {qps:.0f} qps: {lvl['docs_per_s']:.0f} docs/s, p50 {lvl['p50_ms']:.2f} ms, "
+            f"p99 {lvl['p99_ms']:.2f} ms")
+    # plain path on the first documents of the burst (same request ids)
+    first_burst = rid - n_burst - sum(l["docs"] for l in levels)
+    n_plain = min(args.serve_plain_docs, n_burst)
+    t = time.perf_counter()
+    same = True
+    plain_counts = []
+    for r in range(first_burst, first_burst + n_plain):
+        _, c = srv.plain(r, docs[r % len(docs)])
+        same &= bool(np.array_equal(c, burst_counts[r]))
+        plain_counts.append(c)
+    plain_rate = n_plain / (time.perf_counter() - t)
+    log(f"plain path: {plain_rate:.1f} docs/s over {n_plain} docs; batched/plain = {cap / plain_rate:.1f}x")
+    srv.close()
+    wt, nk = ctx.get_wt_dense(), ctx.get_nk()
+    sample = list(range(first_burst, first_burst + min(256, n_burst)))
+    ppl_b = foldin_perplexity(wt, nk, 0.01, alpha, [docs[r % len(docs)] for r in sample],
+                              [burst_counts[r] for r in sample])
+    pdocs = list(range(first_burst, first_burst + n_plain))
+    ppl_p = foldin_perplexity(wt, nk, 0.01, alpha, [docs[r % len(docs)] for r in pdocs], plain_counts)
+    ppl_pb = foldin_perplexity(wt, nk, 0.01, alpha, [docs[r % len(docs)] for r in pdocs],
+                               [burst_counts[r] for r in pdocs])
+    return {
+        "workload": f"fold-in against the frozen K={K} model just trained (V={wl['gen']['V']}); requests are "
+                    f"further documents of the same generative model, lognormal lengths mean 300 in [10, 5000]; "
+                    f"{SERVE_SWEEPS} fold-in sweeps, alpha=50/K; Poisson arrivals at fractions of the burst rate",
+        "sweeps": SERVE_SWEEPS, "max_slots": args.serve_slots, "hbm_budget_bytes": args.serve_budget,
+        "burst": burst, "levels": levels,
+        "plain_docs_per_s": plain_rate, "plain_docs": n_plain,
+        "batched_vs_plain": cap / plain_rate,
+        "plain_equals_batched": same,
+        "perplexity_batched_sample": ppl_b,
+        "perplexity_plain_docs": {"plain": ppl_p, "batched": ppl_pb},
+    }
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -98,6 +221,14 @@ def main() -> int:
     ap.add_argument("--no-graph", action="store_true")
     ap.add_argument("--doc-tiles", type=int, default=0,
                     help="doc tiles for head-word items (0 auto, <0 off); topics do not depend on it")
+    ap.add_argument("--no-serve", action="store_true", help="skip the fold-in serving measurement")
+    ap.add_argument("--serve-docs", type=int, default=8192, help="open-loop burst size")
+    ap.add_argument("--serve-levels", default="0.25,0.5,0.8",
+                    help="Poisson arrival rates as fractions of the burst's docs/s")
+    ap.add_argument("--serve-seconds", type=float, default=2.0)
+    ap.add_argument("--serve-slots", type=int, default=2048)
+    ap.add_argument("--serve-budget", type=int, default=1 << 30, help="HBM bytes for resident requests")
+    ap.add_argument("--serve-plain-docs", type=int, default=32)
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
                     help="only run the plain copy kernel over BYTES (to scale HBM counters) and exit")
     args = ap.parse_args()
@@ -193,6 +324,8 @@ def main() -> int:
         "perplexity_after": math.exp(-ll / N),
         "wall_ms_timed_region": wall * 1e3,
     }
+    if not args.no_serve:
+        out["serving"] = serving_bench(ctx, ezlda, wl, args)
     ctx.close()
     if not args.no_cpu_baseline:
         t = time.perf_counter()

@@ -178,6 +178,50 @@ int ezlda_selftest_copy(int64_t bytes, int32_t reps, double *ms);
 int ezlda_save_checkpoint(ezlda_ctx *ctx, const char *path);
 int ezlda_load_checkpoint(ezlda_ctx *ctx, const char *path);
 
+/* ---- fold-in serving (no reference counterpart: EZLDA only trains)
+ * A server freezes phi from a trained context's current counts and infers
+ * the topics of incoming documents: `sweeps` Jacobi fold-in sweeps with
+ * p(k) ~ (n_dk + alpha) phi_wk per document, drawn with Philox keyed by
+ * (seed, request id, token, sweep).  Requests are admitted into device slots
+ * while slots and the HBM token budget allow, and every resident request is
+ * advanced by one sweep per launch (continuous batching), so a result does
+ * not depend on what else is in flight.  Thread-safe: submit and wait may be
+ * called from any thread; one internal worker thread drives the device. */
+typedef struct ezlda_server ezlda_server;
+typedef struct ezlda_server_args {
+    int32_t max_slots;        /* documents resident at once */
+    int32_t sweeps;           /* fold-in sweeps per document (e.g. 20) */
+    int64_t hbm_budget_bytes; /* device bytes for resident request state
+                                 (6 bytes per token); admission waits for room,
+                                 a document larger than all of it is rejected */
+    uint64_t seed;
+} ezlda_server_args;
+
+typedef struct ezlda_server_stats {
+    int64_t submitted, completed, rejected, steps;
+    int32_t active, pending, max_active;
+    double p50_ms, p99_ms, mean_ms, max_ms; /* submit -> result ready */
+    double docs_per_s;                      /* completed / (last result - first submit) */
+} ezlda_server_stats;
+
+ezlda_server *ezlda_server_create(ezlda_ctx *model, const ezlda_server_args *args);
+void ezlda_server_destroy(ezlda_server *srv);
+const char *ezlda_server_last_error(const ezlda_server *srv);
+/* Queues one document (input word ids). */
+int ezlda_server_submit(ezlda_server *srv, uint32_t request_id, const int32_t *words, int32_t n);
+/* Blocks until the request is done (timeout_ms < 0: no limit) and copies its
+ * final topics (n values) and topic counts (K values); either may be NULL. */
+int ezlda_server_wait(ezlda_server *srv, uint32_t request_id, uint16_t *z_out, int32_t *counts_out,
+                      double *latency_ms, int32_t timeout_ms);
+/* Blocks until no request is queued or resident. */
+int ezlda_server_drain(ezlda_server *srv);
+int ezlda_server_get_stats(ezlda_server *srv, ezlda_server_stats *stats);
+int ezlda_server_reset_stats(ezlda_server *srv);
+/* The plain path: one document, one launch per sweep, synchronous, on its
+ * own stream; same results as the server. */
+int ezlda_foldin_plain(ezlda_server *srv, uint32_t request_id, const int32_t *words, int32_t n, uint16_t *z_out,
+                       int32_t *counts_out);
+
 /* ---- synthetic corpora (LDA generative process, Philox-keyed) */
 typedef struct ezlda_gen_params {
     int32_t V, D, K_true;
@@ -190,6 +234,9 @@ typedef struct ezlda_gen_params {
     double theta_conc;    /* theta_d ~ Dir(theta_conc) over K_true topics */
     uint64_t seed;
     int32_t threads;      /* host threads (0 = all) */
+    int32_t first_doc;    /* documents are numbered from here: the same seed
+                             with first_doc = D of a corpus draws further
+                             documents of the same model (0: a fresh corpus) */
 } ezlda_gen_params;
 
 typedef struct ezlda_corpus ezlda_corpus;

@@ -35,7 +35,7 @@ SOURCES = [
     ("ezlda_kernels.hip", True),
     ("ezlda_runtime.cpp", True),
     ("ezlda_corpus.cpp", False),
-    ("ezlda_foldin.cpp", False),
+    ("ezlda_foldin.cpp", True),
 ]
 
 

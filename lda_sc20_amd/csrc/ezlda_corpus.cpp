@@ -159,7 +159,7 @@ bool make_dir(const std::string &p) {
 extern "C" {
 
 ezlda_corpus *ezlda_gen_corpus(const ezlda_gen_params *p) {
-    if (!p || p->V < 1 || p->D < 0 || p->K_true < 1) {
+    if (!p || p->V < 1 || p->D < 0 || p->K_true < 1 || p->first_doc < 0) {
         err("bad generator parameters");
         return nullptr;
     }
@@ -191,7 +191,7 @@ ezlda_corpus *ezlda_gen_corpus(const ezlda_gen_params *p) {
     // doc lengths
     std::vector<int32_t> len(D);
     pfor(D, T, [&](int d) {
-        Rng r(p->seed, (uint32_t)d, 2u);
+        Rng r(p->seed, (uint32_t)(d + p->first_doc), 2u);
         double x;
         if (p->len_dist == 0) {
             x = r.poisson(p->len_mean);
@@ -210,7 +210,7 @@ ezlda_corpus *ezlda_gen_corpus(const ezlda_gen_params *p) {
     c->words.resize(c->doc_off[D]);
     std::vector<double> tconc(KT, p->theta_conc);
     pfor(D, T, [&](int d) {
-        Rng r(p->seed, (uint32_t)d, 3u);
+        Rng r(p->seed, (uint32_t)(d + p->first_doc), 3u);
         std::vector<double> theta;
         dirichlet(r, tconc, theta);
         for (int k = 1; k < KT; ++k) theta[k] += theta[k - 1];

@@ -59,6 +59,14 @@ hipError_t launch_set_u32(uint32_t *p, uint32_t v, hipStream_t st);
 hipError_t launch_selftest_philox(const uint32_t *ctr, const uint32_t *key, uint32_t *out, int n, hipStream_t st);
 hipError_t launch_copy(const void *src, void *dst, size_t bytes, hipStream_t st);
 
+// Fold-in against a frozen model: per-word phi rows, Q running sums inside
+// 64-topic slabs (qs, V x K) and over slab totals (qacc, V x ceil(K/64)).
+hipError_t launch_foldin_tables(const DevState &s, float *phi, float *qs, float *qacc, hipStream_t st);
+// One sweep of every slot (rid, first token, n, sweep) in the pools.
+hipError_t launch_foldin_step(const uint4 *slots, int nslots, int K, int sweeps_total, float alpha, const float *phi,
+                              const float *qs, const float *qacc, const uint32_t *wpool, uint16_t *zpool,
+                              int32_t *counts_out, uint64_t seed, hipStream_t st);
+
 // Bytes of dynamic LDS the sampler needs for K topics and block size bd.
 size_t sample_lds_bytes(int K, int bd);
 

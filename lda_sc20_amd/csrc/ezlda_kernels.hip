@@ -594,6 +594,206 @@ __global__ void k_ll_final(const double *__restrict__ part, int n, double *__res
     if (threadIdx.x == 0) *out = red[0];
 }
 
+// --------------------------------------------------------------- fold-in
+// Frozen-model tables for fold-in (no reference counterpart): per word the
+// phi row and the Q bucket's running sums (inside 64-topic slabs, and over the
+// slab totals), from the counts of the model at freezing time.
+__global__ void __launch_bounds__(256) k_foldin_tables(int V, int VD, int K, float alpha, float beta,
+                                                       const int32_t *__restrict__ wt_head,
+                                                       const uint32_t *__restrict__ tail_off,
+                                                       const uint32_t *__restrict__ wt_tail,
+                                                       const uint32_t *__restrict__ tail_nnz,
+                                                       const float *__restrict__ inv, float *__restrict__ phi_out,
+                                                       float *__restrict__ qs_out, float *__restrict__ qacc_out) {
+    extern __shared__ float ph[];
+    const int NS = (K + 63) >> 6;
+    const int tid = threadIdx.x;
+    for (int w = blockIdx.x; w < V; w += gridDim.x) {
+        if (w < VD) {
+            const int32_t *row = wt_head + (size_t)w * K;
+            for (int k = tid; k < K; k += blockDim.x) ph[k] = ((float)row[k] + beta) * inv[k];
+        } else {
+            for (int k = tid; k < K; k += blockDim.x) ph[k] = beta * inv[k];
+            __syncthreads();
+            const int tw = w - VD;
+            const uint32_t b = tail_off[tw], n = tail_nnz[tw];
+            for (uint32_t i = tid; i < n; i += blockDim.x) {
+                const uint32_t e = wt_tail[b + i];
+                const uint32_t k = e & 0xFFFFu;
+                ph[k] = ((float)(e >> 16) + beta) * inv[k];
+            }
+        }
+        __syncthreads();
+        float *po = phi_out + (size_t)w * K;
+        for (int k = tid; k < K; k += blockDim.x) po[k] = ph[k];
+        float *qo = qs_out + (size_t)w * K;
+        __shared__ float qt[1024];  // slab totals (NS <= 256 for K <= 16383)
+        for (int j = tid; j < NS; j += blockDim.x) {
+            const int k0 = j * 64, ke = min(k0 + 64, K);
+            float run = 0.0f;
+            for (int k = k0; k < ke; ++k) {
+                run = run + alpha * ph[k];
+                qo[k] = run;
+            }
+            qt[j] = run;
+        }
+        __syncthreads();
+        if (tid == 0) {
+            float acc = 0.0f;
+            for (int j = 0; j < NS; ++j) {
+                acc = acc + qt[j];
+                qacc_out[(size_t)w * NS + j] = acc;
+            }
+        }
+        __syncthreads();
+    }
+}
+
+// One fold-in sweep for every active request slot (rid, first token, n,
+// sweep): one wavefront per slot.  The doc's topic counts are histogrammed
+// in LDS from its current z and compacted into an ascending row; then one
+// lane per token draws from S (the row, FMA running sum, phi gathered from
+// the frozen table) and Q (the word's precomputed running sums).  The first
+// sweep draws the initial topics; the last one writes the doc's counts.
+__global__ void __launch_bounds__(256) k_foldin_step(
+    const uint4 *__restrict__ slots, int nslots, int K, int sweeps_total, float alpha,
+    const float *__restrict__ phi, const float *__restrict__ qs, const float *__restrict__ qacc,
+    const uint32_t *__restrict__ wpool, uint16_t *__restrict__ zpool, int32_t *__restrict__ counts_out,
+    uint64_t seed) {
+    extern __shared__ uint32_t shf[];
+    const int NS = (K + 63) >> 6;
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6, nwv = blockDim.x >> 6;
+    const int KH = (K + 1) >> 1;
+    uint32_t *h = shf + wv * (KH + K);
+    uint32_t *row = h + KH;
+    for (int k = lane; k < KH; k += 64) h[k] = 0;
+    wave_lds_sync();
+    for (int si = blockIdx.x * nwv + wv; si < nslots; si += gridDim.x * nwv) {
+        const uint4 sl = slots[si];
+        const uint32_t rid = sl.x, off = sl.y, n = sl.z, sweep = sl.w;
+        const uint64_t key = (uint64_t)rid << 32;
+        uint16_t *z = zpool + off;
+        const uint32_t *wd = wpool + off;
+        if (sweep == 0)
+            for (uint32_t i = lane; i < n; i += 64)
+                z[i] = (uint16_t)(ez_philox_x0(seed, key | i, 0u, EZ_TAG_FOLDIN_INIT) % (uint32_t)K);
+        for (uint32_t i = lane; i < n; i += 64) h16_add(h, z[i]);
+        wave_lds_sync();
+        uint32_t nz = 0;
+        for (int j = 0; j < K; j += 64) {
+            const int k = j + lane;
+            const uint32_t c = k < K ? h16_get(h, (uint32_t)k) : 0u;
+            const uint64_t m = __ballot(c != 0);
+            if (c) row[nz + __popcll(m & lanemask_lt(lane))] = (uint32_t)k | (c << 16);
+            nz += (uint32_t)__popcll(m);
+            wave_lds_sync();
+            if (k < K && !(k & 1)) h[k >> 1] = 0;
+        }
+        wave_lds_sync();
+        for (uint32_t base = 0; base < n; base += 64) {
+            const uint32_t i = base + (uint32_t)lane;
+            if (i < n) {
+                const uint32_t w = wd[i];
+                const float *ph = phi + (size_t)w * K;
+                const float *qa = qacc + (size_t)w * NS;
+                const float u = ez_u24(ez_philox_x0(seed, key | i, sweep, EZ_TAG_FOLDIN));
+                float S = 0.0f;
+                for (uint32_t j = 0; j < nz; ++j) {
+                    const uint32_t e = row[j];
+                    S = __builtin_fmaf((float)(e >> 16), ph[e & 0xFFFFu], S);
+                }
+                const float target = u * (S + qa[NS - 1]);
+                int zz = -1;
+                if (target < S) {
+                    float run = 0.0f;
+                    int last = -1;
+                    for (uint32_t j = 0; j < nz; ++j) {
+                        const uint32_t e = row[j];
+                        const float c = (float)(e >> 16), f = ph[e & 0xFFFFu];
+                        run = __builtin_fmaf(c, f, run);
+                        if (run > target) {
+                            zz = (int)(e & 0xFFFFu);
+                            break;
+                        }
+                        if (c > 0.0f && f > 0.0f) last = (int)(e & 0xFFFFu);
+                    }
+                    if (zz < 0) zz = last;
+                } else {
+                    const float t2 = target - S;
+                    int lo = 0, hi = NS;
+                    while (lo < hi) {
+                        const int mid = (lo + hi) >> 1;
+                        if (qa[mid] > t2) hi = mid; else lo = mid + 1;
+                    }
+                    int js = lo < NS ? lo : -1;
+                    if (js < 0)
+                        for (int j = NS - 1; j >= 0; --j)
+                            if (qa[j] > (j > 0 ? qa[j - 1] : 0.0f)) {
+                                js = j;
+                                break;
+                            }
+                    if (js >= 0) {
+                        const float t3 = t2 - (js > 0 ? qa[js - 1] : 0.0f);
+                        const float *qw = qs + (size_t)w * K;
+                        const int k0 = js * 64, ke = min(k0 + 64, K);
+                        lo = k0;
+                        hi = ke;
+                        while (lo < hi) {
+                            const int mid = (lo + hi) >> 1;
+                            if (qw[mid] > t3) hi = mid; else lo = mid + 1;
+                        }
+                        if (lo < ke) {
+                            zz = lo;
+                        } else {
+                            for (int k = ke - 1; k >= k0; --k)
+                                if (alpha * ph[k] > 0.0f) {
+                                    zz = k;
+                                    break;
+                                }
+                        }
+                    }
+                }
+                z[i] = (uint16_t)(zz < 0 ? 0 : zz);
+            }
+        }
+        if (sweep + 1 == (uint32_t)sweeps_total) {  // final counts of the doc
+            wave_lds_sync();
+            for (uint32_t i = lane; i < n; i += 64) h16_add(h, z[i]);
+            wave_lds_sync();
+            int32_t *co = counts_out + (size_t)si * K;
+            for (int k = lane; k < K; k += 64) co[k] = (int32_t)h16_get(h, (uint32_t)k);
+            wave_lds_sync();
+            for (int k = lane; k < KH; k += 64) h[k] = 0;
+        }
+        wave_lds_sync();
+    }
+}
+
+hipError_t launch_foldin_tables(const DevState &s, float *phi, float *qs, float *qacc, hipStream_t st) {
+    hipLaunchKernelGGL(k_foldin_tables, dim3(grid_for(s.V, 1, 1 << 16)), dim3(256), (size_t)s.K * sizeof(float), st,
+                       s.V, s.VD, s.K, s.alpha, s.beta, s.wt_head, s.tail_off, s.wt_tail, s.tail_nnz, s.inv, phi, qs,
+                       qacc);
+    return hipGetLastError();
+}
+
+int foldin_waves_per_block(int K) {
+    const size_t per_wave = (size_t)(((K + 1) >> 1) + K) * sizeof(uint32_t);
+    int w = 4;
+    while (w > 1 && per_wave * w > 64 * 1024) --w;
+    return w;
+}
+
+hipError_t launch_foldin_step(const uint4 *slots, int nslots, int K, int sweeps_total, float alpha, const float *phi,
+                              const float *qs, const float *qacc, const uint32_t *wpool, uint16_t *zpool,
+                              int32_t *counts_out, uint64_t seed, hipStream_t st) {
+    if (nslots <= 0) return hipSuccess;
+    const int nwv = foldin_waves_per_block(K);
+    const size_t lds = (size_t)nwv * (((K + 1) >> 1) + K) * sizeof(uint32_t);
+    hipLaunchKernelGGL(k_foldin_step, dim3(grid_for(nslots, nwv, 1 << 16)), dim3(64 * nwv), lds, st, slots, nslots, K,
+                       sweeps_total, alpha, phi, qs, qacc, wpool, zpool, counts_out, seed);
+    return hipGetLastError();
+}
+
 // ------------------------------------------------------------ self-tests
 __global__ void k_selftest_philox(const uint4 *__restrict__ ctr, const uint2 *__restrict__ key,
                                   uint4 *__restrict__ out, int n) {

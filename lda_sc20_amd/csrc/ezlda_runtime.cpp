@@ -13,6 +13,7 @@
 
 #include "../../include/ezlda.h"
 #include "ezlda_device.h"
+#include "ezlda_internal.h"
 #include "ezlda_philox.h"
 
 namespace {
@@ -47,28 +48,6 @@ void parallel_for(int64_t n, F f) {
 namespace ezc {
 void set_error(const std::string &m) { g_err = m; }
 }  // namespace ezc
-
-struct ezlda_ctx {
-    ezlda_args a{};
-    ez::DevState s{};
-    hipStream_t st = nullptr;
-    std::string err;
-    bool loaded = false;
-    bool counts_fresh = false;
-    uint32_t sweep = 0;
-    int64_t dev_bytes = 0;
-    int64_t dt_cap = 0, tail_cap = 0, dt_cap_canon = 0;
-    std::vector<void *> allocs;
-    // host mirrors for result mapping
-    std::vector<uint32_t> doc_tok;   // doc-major position -> token id
-    std::vector<int32_t> rank_of;    // input word -> rank
-    std::vector<uint32_t> word_off;  // V+1 by rank
-    std::vector<uint32_t> doc_off;   // D+1
-    hipGraph_t graph = nullptr;
-    hipGraphExec_t graph_exec = nullptr;
-    int n_xcd = 1;      // XCDs of the device (round-robin workgroup dispatch)
-    int doc_tiles = 0;  // doc tiles used for head-word items (0 = none)
-};
 
 namespace {
 
@@ -209,6 +188,15 @@ std::vector<uint4> build_sample_items(const ezlda_args &a, const std::vector<uin
 }
 
 }  // namespace
+
+int ezc::refresh_counts(ezlda_ctx *c) {
+    if (int r = need_loaded(c)) return r;
+    if (!c->counts_fresh) EZ_TRY_HIP(c, ez::launch_build_wt(c->s, c->st));
+    EZ_TRY_HIP(c, ez::launch_topk(c->s, c->st));
+    EZ_TRY_HIP(c, hipStreamSynchronize(c->st));
+    c->counts_fresh = true;
+    return EZLDA_OK;
+}
 
 extern "C" {
 

@@ -29,7 +29,9 @@ EXPORTS = (
     "ezlda_corpus_num_words", "ezlda_corpus_doc_off", "ezlda_corpus_words", "ezlda_corpus_topics",
     "ezlda_corpus_free", "ezlda_corpus_from_arrays", "ezlda_read_docword", "ezlda_write_chunked",
     "ezlda_read_chunked", "ezlda_selftest_philox", "ezlda_selftest_copy", "ezlda_get_stats",
-    "ezlda_reset_stats",
+    "ezlda_reset_stats", "ezlda_server_create", "ezlda_server_destroy", "ezlda_server_last_error",
+    "ezlda_server_submit", "ezlda_server_wait", "ezlda_server_drain", "ezlda_server_get_stats",
+    "ezlda_server_reset_stats", "ezlda_foldin_plain",
 )
 
 
@@ -54,12 +56,24 @@ class Info(C.Structure):
                 ("device_bytes", C.c_int64), ("doc_tiles", C.c_int32), ("num_xcd", C.c_int32)]
 
 
+class ServerArgs(C.Structure):
+    _fields_ = [("max_slots", C.c_int32), ("sweeps", C.c_int32), ("hbm_budget_bytes", C.c_int64),
+                ("seed", C.c_uint64)]
+
+
+class ServerStats(C.Structure):
+    _fields_ = [("submitted", C.c_int64), ("completed", C.c_int64), ("rejected", C.c_int64), ("steps", C.c_int64),
+                ("active", C.c_int32), ("pending", C.c_int32), ("max_active", C.c_int32),
+                ("p50_ms", C.c_double), ("p99_ms", C.c_double), ("mean_ms", C.c_double), ("max_ms", C.c_double),
+                ("docs_per_s", C.c_double)]
+
+
 class GenParams(C.Structure):
     _fields_ = [("V", C.c_int32), ("D", C.c_int32), ("K_true", C.c_int32), ("len_dist", C.c_int32),
                 ("len_mean", C.c_double), ("len_sigma", C.c_double), ("pareto_shape", C.c_double),
                 ("pareto_xm", C.c_double), ("len_min", C.c_int32), ("len_max", C.c_int32),
                 ("zipf_s", C.c_double), ("phi_conc", C.c_double), ("theta_conc", C.c_double),
-                ("seed", C.c_uint64), ("threads", C.c_int32)]
+                ("seed", C.c_uint64), ("threads", C.c_int32), ("first_doc", C.c_int32)]
 
 
 _lib = None
@@ -117,6 +131,15 @@ def lib() -> C.CDLL:
         "ezlda_selftest_copy": (C.c_int, [C.c_int64, C.c_int32, C.POINTER(C.c_double)]),
         "ezlda_get_stats": (C.c_int, [P, P]),
         "ezlda_reset_stats": (C.c_int, [P]),
+        "ezlda_server_create": (P, [P, C.POINTER(ServerArgs)]),
+        "ezlda_server_destroy": (None, [P]),
+        "ezlda_server_last_error": (C.c_char_p, [P]),
+        "ezlda_server_submit": (C.c_int, [P, C.c_uint32, P, C.c_int32]),
+        "ezlda_server_wait": (C.c_int, [P, C.c_uint32, P, P, C.POINTER(C.c_double), C.c_int32]),
+        "ezlda_server_drain": (C.c_int, [P]),
+        "ezlda_server_get_stats": (C.c_int, [P, C.POINTER(ServerStats)]),
+        "ezlda_server_reset_stats": (C.c_int, [P]),
+        "ezlda_foldin_plain": (C.c_int, [P, C.c_uint32, P, C.c_int32, P, P]),
     }
     for name, (res, argt) in sig.items():
         f = getattr(L, name)
@@ -198,10 +221,10 @@ class Corpus:
 def gen_corpus(V: int, D: int, K_true: int, len_dist: int = 0, len_mean: float = 100.0, len_sigma: float = 0.8,
                pareto_shape: float = 1.2, pareto_xm: float = 1.0, len_min: int = 1, len_max: int = 65535,
                zipf_s: float = 0.0, phi_conc: float = 0.01, theta_conc: float = 0.1, seed: int = 1001,
-               threads: int = 0) -> Corpus:
+               threads: int = 0, first_doc: int = 0) -> Corpus:
     """Synthetic corpus from the LDA generative process (C++ generator)."""
     p = GenParams(V, D, K_true, len_dist, len_mean, len_sigma, pareto_shape, pareto_xm, len_min, len_max, zipf_s,
-                  phi_conc, theta_conc, seed, threads)
+                  phi_conc, theta_conc, seed, threads, first_doc)
     return Corpus._from_handle(lib().ezlda_gen_corpus(C.byref(p)))
 
 
@@ -353,6 +376,64 @@ class Context:
 
     def load_checkpoint(self, path: str):
         self._chk(self._L.ezlda_load_checkpoint(self._h, path.encode()))
+
+
+class Server:
+    """Fold-in server over a trained Context (its model is frozen at creation)."""
+
+    def __init__(self, ctx: "Context", max_slots: int = 1024, sweeps: int = 20, hbm_budget_bytes: int = 1 << 28,
+                 seed: int = 7):
+        self._L = lib()
+        self.K = ctx.args.K
+        self.sweeps = sweeps
+        a = ServerArgs(max_slots, sweeps, hbm_budget_bytes, seed)
+        self._h = self._L.ezlda_server_create(ctx._h, C.byref(a))
+        if not self._h:
+            raise EzldaError(EHIP, last_error())
+
+    def _chk(self, r):
+        if r:
+            m = self._L.ezlda_server_last_error(self._h)
+            raise EzldaError(r, m.decode() if m else "")
+
+    def close(self):
+        if getattr(self, "_h", None):
+            self._L.ezlda_server_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        self.close()
+
+    def submit(self, rid: int, words: np.ndarray):
+        w = np.ascontiguousarray(words, dtype=np.int32)
+        self._chk(self._L.ezlda_server_submit(self._h, rid, _ptr(w), len(w)))
+
+    def wait(self, rid: int, n: int, timeout_ms: int = -1):
+        """(z, topic counts, latency ms) of a finished request."""
+        z = np.zeros(max(n, 1), np.uint16)
+        c = np.zeros(self.K, np.int32)
+        lat = C.c_double()
+        self._chk(self._L.ezlda_server_wait(self._h, rid, _ptr(z), _ptr(c), C.byref(lat), timeout_ms))
+        return z[:n], c, lat.value
+
+    def drain(self):
+        self._chk(self._L.ezlda_server_drain(self._h))
+
+    def stats(self) -> ServerStats:
+        st = ServerStats()
+        self._chk(self._L.ezlda_server_get_stats(self._h, C.byref(st)))
+        return st
+
+    def reset_stats(self):
+        self._chk(self._L.ezlda_server_reset_stats(self._h))
+
+    def plain(self, rid: int, words: np.ndarray):
+        """One document, one launch per sweep: (z, topic counts)."""
+        w = np.ascontiguousarray(words, dtype=np.int32)
+        z = np.zeros(max(len(w), 1), np.uint16)
+        c = np.zeros(self.K, np.int32)
+        self._chk(self._L.ezlda_foldin_plain(self._h, rid, _ptr(w), len(w), _ptr(z), _ptr(c)))
+        return z[: len(w)], c
 
 
 def selftest_philox(ctr: np.ndarray, key: np.ndarray) -> np.ndarray:

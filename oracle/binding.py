@@ -60,6 +60,7 @@ def lib(omp: bool = False) -> C.CDLL:
         "orc_get_layout": (None, [P, P, P, P, P]),
         "orc_uniform": (C.c_float, [C.c_uint64, C.c_uint64, C.c_uint32]),
         "orc_num_threads": (C.c_int, []),
+        "orc_foldin": (None, [P, C.c_int, P, P, P, C.c_int, C.c_uint64, P, P]),
     }
     for n, (res, argt) in sig.items():
         f = getattr(L, n)
@@ -180,6 +181,17 @@ class Oracle:
         o = np.zeros(self.N, np.uint32)
         self.L.orc_get_aux(self.h, _p(o))
         return o
+
+    def foldin(self, doc_off, words, rids, sweeps: int, seed: int):
+        """Fold-in of new docs against the frozen current model: (z, counts[D, K])."""
+        doc_off = np.ascontiguousarray(doc_off, dtype=np.int64)
+        words = np.ascontiguousarray(words, dtype=np.int32)
+        rids = np.ascontiguousarray(rids, dtype=np.uint32)
+        nd = len(doc_off) - 1
+        z = np.zeros(max(int(doc_off[-1]), 1), np.uint16)
+        cnt = np.zeros((max(nd, 1), self.K), np.int32)
+        self.L.orc_foldin(self.h, nd, _p(doc_off), _p(words), _p(rids), sweeps, seed, _p(z), _p(cnt))
+        return z[: int(doc_off[-1])], cnt[:nd]
 
     def get_layout(self):
         wo = np.zeros(self.V + 1, np.int64)

@@ -482,6 +482,108 @@ double orc_loglik(void *h) {
     return ll;
 }
 
+/* ------------------------------------------------------------ fold-in */
+/* Fold-in of new documents against the frozen model of the current counts
+ * (the reference has no counterpart; the conditional is its sampler's with
+ * WT frozen: p(k) ~ (n_dk + alpha) phi_wk).  Per request r (id rid) with
+ * tokens w_0..w_{n-1} (input word ids):
+ *   z_i = Philox(seed; rid<<32 | i, 0, 4).x mod K;
+ *   each of `sweeps` Jacobi sweeps rebuilds the doc's DT row from z and
+ *   draws every token from two buckets in this order:
+ *     S = doc row (ascending topic) accumulated with fmaf(n_dk, phi_wk, S),
+ *     Q = alpha * phi_w. running sums inside 64-topic slabs + running sum
+ *         of the slab totals (no topic excluded),
+ *   u = Philox(seed; rid<<32 | i, sweep, 3).x >> 8 scaled by 2^-24,
+ *   target = u * (S + Q); S bucket if target < S (first entry whose running
+ *   sum exceeds target), else Q bucket with target - S; the same fallbacks
+ *   as training.
+ * Outputs the final z and the doc's topic counts (K per doc). */
+void orc_foldin(void *h, int nd, const int64_t *doc_off, const int32_t *words, const uint32_t *rids,
+                int sweeps, uint64_t seed, uint16_t *z_out, int32_t *counts_out) {
+    orc_t *o = (orc_t *)h;
+    const int K = o->K, V = o->V;
+    const int nsq = (K + 63) / 64;
+    const float alpha = o->alpha;
+    if (!o->counts_fresh) orc_build_wt(o);
+    for (int k = 0; k < K; ++k) o->inv[k] = 1.0f / ((float)o->nk[k] + o->vbeta);
+    float *phi = (float *)xcalloc((size_t)V * K, sizeof(float));
+    float *qs = (float *)xcalloc((size_t)V * K, sizeof(float));
+    float *qacc = (float *)xcalloc((size_t)V * nsq, sizeof(float));
+    for (int w = 0; w < V; ++w) {
+        float *ph = phi + (size_t)w * K;
+        phi_row(o, w, ph);
+        float acc = 0.0f;
+        for (int j = 0; j < nsq; ++j) {
+            float run = 0.0f;
+            for (int k = 64 * j; k < 64 * j + 64 && k < K; ++k) {
+                run = run + alpha * ph[k];
+                qs[(size_t)w * K + k] = run;
+            }
+            acc = acc + run;
+            qacc[(size_t)w * nsq + j] = acc;
+        }
+    }
+    int32_t *hist = (int32_t *)xcalloc(K, sizeof(int32_t));
+    uint32_t *row = (uint32_t *)xcalloc(K, sizeof(uint32_t));
+    for (int r = 0; r < nd; ++r) {
+        const int64_t b = doc_off[r], n = doc_off[r + 1] - doc_off[r];
+        const uint64_t key = (uint64_t)rids[r] << 32;
+        uint16_t *z = z_out + b;
+        for (int64_t i = 0; i < n; ++i) z[i] = (uint16_t)(philox_x0(seed, key | (uint64_t)i, 0u, 4u) % (uint32_t)K);
+        for (int s = 0; s < sweeps; ++s) {
+            memset(hist, 0, sizeof(int32_t) * K);
+            for (int64_t i = 0; i < n; ++i) hist[z[i]]++;
+            int nnz = 0;
+            for (int k = 0; k < K; ++k)
+                if (hist[k]) row[nnz++] = (uint32_t)k | ((uint32_t)hist[k] << 16);
+            for (int64_t i = 0; i < n; ++i) {
+                const int w = o->rank_of[words[b + i]];
+                const float *ph = phi + (size_t)w * K;
+                const float *qw = qs + (size_t)w * K;
+                const float *qa = qacc + (size_t)w * nsq;
+                const float u = u24(philox_x0(seed, key | (uint64_t)i, (uint32_t)s, 3u));
+                float S = 0.0f;
+                for (int j = 0; j < nnz; ++j) S = fmaf((float)(row[j] >> 16), ph[row[j] & 0xFFFFu], S);
+                const float target = u * (S + qa[nsq - 1]);
+                int zz = -1;
+                if (target < S) {
+                    float run = 0.0f;
+                    int last = -1;
+                    for (int j = 0; j < nnz && zz < 0; ++j) {
+                        const float c = (float)(row[j] >> 16), f = ph[row[j] & 0xFFFFu];
+                        run = fmaf(c, f, run);
+                        if (run > target) zz = (int)(row[j] & 0xFFFFu);
+                        else if (c > 0.0f && f > 0.0f) last = (int)(row[j] & 0xFFFFu);
+                    }
+                    if (zz < 0) zz = last;
+                } else {
+                    const float t2 = target - S;
+                    int js = -1;
+                    for (int j = 0; j < nsq; ++j)
+                        if (qa[j] > t2) { js = j; break; }
+                    if (js < 0)
+                        for (int j = nsq - 1; j >= 0; --j)
+                            if (qa[j] > (j > 0 ? qa[j - 1] : 0.0f)) { js = j; break; }
+                    if (js >= 0) {
+                        const float t3 = t2 - (js > 0 ? qa[js - 1] : 0.0f);
+                        const int k0 = 64 * js, ke = (k0 + 64 < K) ? k0 + 64 : K;
+                        for (int k = k0; k < ke && zz < 0; ++k)
+                            if (qw[k] > t3) zz = k;
+                        for (int k = ke - 1; k >= k0 && zz < 0; --k)
+                            if (alpha * ph[k] > 0.0f) zz = k;
+                    }
+                }
+                /* Jacobi: the row was built before any token of this sweep moved */
+                z_out[b + i] = (uint16_t)(zz < 0 ? 0 : zz);
+            }
+        }
+        int32_t *cnt = counts_out + (size_t)r * K;
+        memset(cnt, 0, sizeof(int32_t) * K);
+        for (int64_t i = 0; i < n; ++i) cnt[z[i]]++;
+    }
+    free(hist); free(row); free(phi); free(qs); free(qacc);
+}
+
 /* ------------------------------------------------------------ getters */
 void orc_get_topics(void *h, uint16_t *z_docmajor) {
     orc_t *o = (orc_t *)h;

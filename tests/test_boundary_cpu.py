@@ -63,6 +63,13 @@ def test_generator_is_deterministic_and_thread_independent():
     assert not np.array_equal(a.words, c.words)
 
 
+def test_generator_first_doc_continues_the_same_corpus():
+    a = ezlda.gen_corpus(V=500, D=200, K_true=10, len_mean=40, seed=5)
+    b = ezlda.gen_corpus(V=500, D=50, K_true=10, len_mean=40, seed=5, first_doc=150)
+    assert np.array_equal(np.diff(a.doc_off)[150:], np.diff(b.doc_off))
+    assert np.array_equal(a.words[a.doc_off[150]:], b.words)
+
+
 def test_generator_poisson_lengths_and_ranges():
     c = ezlda.gen_corpus(V=1000, D=4000, K_true=16, len_dist=0, len_mean=100, seed=1)
     n = np.diff(c.doc_off)
