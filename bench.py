@@ -219,6 +219,9 @@ def main() -> int:
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-docs", type=int, default=30000)
     ap.add_argument("--no-graph", action="store_true")
+    ap.add_argument("--run-cap", type=int, default=0, help="tokens per shared DT scan (0 = library default)")
+    ap.add_argument("--grid-dim", type=int, default=0, help="persistent sampler workgroups (0 = one per item)")
+    ap.add_argument("--block-dim", type=int, default=256)
     ap.add_argument("--doc-tiles", type=int, default=0,
                     help="doc tiles for head-word items (0 auto, <0 off); topics do not depend on it")
     ap.add_argument("--no-serve", action="store_true", help="skip the fold-in serving measurement")
@@ -252,7 +255,8 @@ def main() -> int:
     K = wl["K"]
     dense = -1 if wl["head_frac"] is None else int(math.ceil(wl["head_frac"] * corpus.V))
     ctx = ezlda.Context(K=K, alpha=50.0 / K, beta=0.01, seed=wl["seed"], dense_words=dense, shortcut=1,
-                        doc_tiles=args.doc_tiles)
+                        doc_tiles=args.doc_tiles, run_cap=args.run_cap, grid_dim=args.grid_dim,
+                        block_dim=args.block_dim)
     t = time.perf_counter()
     ctx.load(corpus)
     info = ctx.info()
@@ -277,15 +281,18 @@ def main() -> int:
     value = N * K_steps / (total_ms / 1e3)
     log(f"timed {K_steps} sweeps: {total_ms:.2f} ms (wall {wall * 1e3:.1f} ms), stages {stage_ms}")
 
-    # algorithmic bytes of one sampler launch (per-token streams + gathers,
-    # phi staging once per word, one read of every scanned DT entry)
+    # algorithmic bytes of one sampler launch: the z write per token; per run
+    # (a word's tokens in one doc) its entry, token->doc id, max-topic counts
+    # and doc info (8 + 4 + 4 + 8); phi staging once per word; one read of
+    # every scanned DT entry (one scan per run with a full draw)
     VD = int(info.num_head_words)
+    R = int(info.num_runs)
     s_scanned = n_scanned / K_steps
-    b_sample = 18 * N + 4 * K * VD + 4 * tail_nnz + 4 * K + 4 * s_scanned
+    b_sample = 2 * N + 24 * R + 4 * K * VD + 4 * tail_nnz + 4 * K + 4 * s_scanned
     t_sample = stage_ms["sample"] / K_steps / 1e3
     achieved = b_sample / t_sample / 1e9
     # whole-sweep algorithmic bytes (same accounting for every stage)
-    b_sweep = (18 * N + 4 * K * VD + 4 * tail_nnz + 4 * K + 4 * s_scanned  # sample
+    b_sweep = (b_sample                                                      # sample
                + 2 * N + 2 * N + 4 * K * VD + 4 * tail_nnz                  # n_k + WT rebuild
                + 4 * K * VD + 4 * tail_nnz + 8 * info.num_words             # top-3
                + 8 * N + 2 * N + 4 * N + 4 * dt_nnz + 8 * info.num_docs)    # DT + aux
@@ -311,7 +318,8 @@ def main() -> int:
         "data": "synthetic (LDA generative process, Philox-seeded)",
         "config": {"workload": f"{args.workload}: {wl['desc']}", "num_tokens": N, "K": K,
                    "num_head_words": VD, "shortcut": True, "sweeps_before_timing": args.warmup,
-                   "doc_tiles": int(info.doc_tiles)},
+                   "doc_tiles": int(info.doc_tiles), "run_cap": int(info.run_cap), "num_runs": int(info.num_runs),
+                   "grid_dim": args.grid_dim, "block_dim": args.block_dim},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": measured_traffic(args.workload),
                      "kernel": "k_sample", "bytes_per_launch": b_sample, "avg_launch_ms": t_sample * 1e3,

@@ -44,8 +44,10 @@ typedef struct ezlda_args {
     int32_t block_dim;       /* Argument.cuh:4 BlockDim: threads per workgroup
                                 of the sweep kernels; multiple of 64, <= 1024 */
     int32_t grid_dim;        /* Argument.cuh:5 GridDim: 0 = one workgroup per
-                                work item; > 0 = that many persistent
-                                workgroups pulling work items from a queue */
+                                sampler item; > 0 = that many persistent
+                                workgroups pulling items from per-XCD queues
+                                (then from the others'); topics do not
+                                depend on it */
     int32_t num_chunks;      /* mainKernel.cu:50; recorded, corpus is resident */
     int32_t num_streams;     /* Argument.cuh:7; recorded */
     int32_t token_segment;   /* Doc.cuh:30: max tokens of one word per sampler
@@ -63,6 +65,10 @@ typedef struct ezlda_args {
                                 L2 locality, dealt to XCDs: 0 = auto,
                                 > 0 = that many tiles, < 0 = off; topics do
                                 not depend on it */
+    int32_t run_cap;         /* the sampler shares one DT row scan among the
+                                repeats of a word in a document, at most
+                                run_cap tokens (1..32, 0 = 8); 1 = one scan
+                                per token.  Topics do not depend on it */
 } ezlda_args;
 
 typedef struct ezlda_info {
@@ -78,6 +84,8 @@ typedef struct ezlda_info {
     int64_t device_bytes;    /* device memory held by the context */
     int32_t doc_tiles;       /* doc tiles used for head-word items */
     int32_t num_xcd;         /* XCDs of the device */
+    int32_t run_cap;         /* tokens per run in effect */
+    int64_t num_runs;        /* sampler runs (a word's tokens in one doc) */
 } ezlda_info;
 
 void ezlda_default_args(ezlda_args *args);

@@ -37,9 +37,12 @@ struct DevState {
     // work items
     uint4 *wt_segs = nullptr;       // head-word histogram segments (w, beg, end)
     int32_t n_wt_segs = 0;
-    uint4 *smp_segs = nullptr;      // sampler segments (w, beg, end)
+    uint4 *smp_segs = nullptr;      // sampler items (w, first run, end run, 0)
     int32_t n_smp_segs = 0;
-    uint32_t *counters = nullptr;   // [0] sweep, [1] work queue, [2..] spare
+    uint2 *runs = nullptr;          // (first token, tokens): a word's tokens in one doc
+    int64_t n_runs = 0;
+    int32_t n_queues = 1;           // persistent sampler: item queues (<= 8)
+    uint32_t *counters = nullptr;   // [0] sweep, [8 .. 8 + n_queues) item queues
     unsigned long long *stats = nullptr;  // [0] full draws, [1] DT entries scanned
     double *ll_part = nullptr;      // n_smp_segs
     double *ll_out = nullptr;       // 1
@@ -69,5 +72,7 @@ hipError_t launch_foldin_step(const uint4 *slots, int nslots, int K, int sweeps_
 
 // Bytes of dynamic LDS the sampler needs for K topics and block size bd.
 size_t sample_lds_bytes(int K, int bd);
+// Largest LDS any launch of the library asks for at this K and block size.
+size_t max_lds_bytes(int K, int bd);
 
 }  // namespace ez

@@ -312,50 +312,68 @@ __device__ __forceinline__ float block_sum(const uint4 *__restrict__ p, const fl
     return s;
 }
 
-// One workgroup per sampler item (a word, or a doc tile / token segment of a
-// head word).  The workgroup stages phi_w in LDS with phi[k1] = 0, and the Q
-// bucket's running sums: qs[k] inside each 64-topic slab, qacc[j] over the
-// slab totals.  Then one lane per token: the max-topic shortcut test, and for
-// the other tokens one pass over the doc's DT row (16 bytes per load) for S,
-// the bucket choice, and either a second pass to the first running sum above
-// the target (S) or a bisection of the monotone Q running sums (Q).
+// Sampler work item: a word, or a doc tile / token segment of a head word,
+// given as a range of runs.  A run is up to run_cap consecutive tokens of the
+// word in one document (tokens of a word are numbered by document, so the
+// repeats of a word in a document are adjacent); its tokens share n_d, the
+// max-topic counts and the DT row, hence the S sum and its checkpoints.  One
+// lane per run: the shortcut test of each token, then, if any token needs
+// the full draw, one pass over the doc's DT row (16 bytes per load) for S and
+// the checkpoints, and per such token the bucket choice and either a re-read
+// of one row block to the first running sum above the target (S) or a
+// bisection of the monotone Q running sums (Q).  Every token's draw is the
+// one it would get on its own; only the row scan is shared.
 // Replaces UpdateProbKernelTrainD1 (utility.cu:2680-3312), LDAKernelTrainD
 // (:1625-2103) and LDAKernelTrain (:1217-1611); one kernel serves head and
 // tail words, which differ only in how phi_w is staged.
+struct SampleArgs {
+    const uint4 *segs;
+    const uint2 *runs;
+    uint32_t n_items;
+    int VD, K;
+    float alpha, beta;
+    const int32_t *wt_head;
+    const uint32_t *tail_off, *wt_tail, *tail_nnz;
+    const float *inv;
+    const ushort4 *wtop;
+    const uint32_t *tok_doc, *aux;
+    const uint2 *doc_info;
+    const uint32_t *dt;
+    uint16_t *z;
+    const uint32_t *counters;
+    uint32_t *queue;  // persistent mode: one item counter per queue
+    uint32_t n_queues;
+    unsigned long long *stats;
+    uint64_t seed;
+};
+
 template <bool SHORTCUT>
-__global__ void __launch_bounds__(1024) k_sample(
-    const uint4 *__restrict__ segs, int VD, int K, float alpha, float beta, const int32_t *__restrict__ wt_head,
-    const uint32_t *__restrict__ tail_off, const uint32_t *__restrict__ wt_tail,
-    const uint32_t *__restrict__ tail_nnz, const float *__restrict__ inv, const ushort4 *__restrict__ wtop,
-    const uint32_t *__restrict__ tok_doc, const uint32_t *__restrict__ aux, const uint2 *__restrict__ doc_info,
-    const uint32_t *__restrict__ dt, uint16_t *__restrict__ z, const uint32_t *__restrict__ counters,
-    unsigned long long *__restrict__ stats, uint64_t seed) {
-    extern __shared__ float smf[];
-    __shared__ float f123[3];
-    const int NS = (K + 63) >> 6;
+__device__ __forceinline__ void sample_item(const SampleArgs &A, uint32_t item, float *smf, float *f123,
+                                            uint32_t sweep, uint32_t &n_full, uint32_t &n_scanned) {
+    const int K = A.K, NS = (K + 63) >> 6;
+    const float alpha = A.alpha, beta = A.beta;
     const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6, nwv = blockDim.x >> 6;
     float *phi = smf;              // NS*64
     float *qs = phi + NS * 64;     // NS*64: running sums inside each slab
     float *qt = qs + NS * 64;      // NS: slab totals
     float *qacc = qt + NS;         // NS: running sum of the slab totals
-    const uint4 sg = segs[blockIdx.x];
-    if (sg.x == 0xFFFFFFFFu) return;  // padding item
+    const uint4 sg = A.segs[item];
+    if (sg.x == 0xFFFFFFFFu) return;  // padding item (uniform over the block)
     const int w = (int)sg.x;
-    const ushort4 tp = wtop[w];
-    const uint32_t sweep = counters[0];
+    const ushort4 tp = A.wtop[w];
 
-    if (w < VD) {
-        const int32_t *row = wt_head + (size_t)w * K;
-        for (int k = tid; k < NS * 64; k += blockDim.x) phi[k] = k < K ? ((float)row[k] + beta) * inv[k] : 0.0f;
+    if (w < A.VD) {
+        const int32_t *row = A.wt_head + (size_t)w * K;
+        for (int k = tid; k < NS * 64; k += blockDim.x) phi[k] = k < K ? ((float)row[k] + beta) * A.inv[k] : 0.0f;
     } else {
-        for (int k = tid; k < NS * 64; k += blockDim.x) phi[k] = k < K ? beta * inv[k] : 0.0f;
+        for (int k = tid; k < NS * 64; k += blockDim.x) phi[k] = k < K ? beta * A.inv[k] : 0.0f;
         __syncthreads();
-        const int tw = w - VD;
-        const uint32_t b = tail_off[tw], n = tail_nnz[tw];
+        const int tw = w - A.VD;
+        const uint32_t b = A.tail_off[tw], n = A.tail_nnz[tw];
         for (uint32_t i = tid; i < n; i += blockDim.x) {
-            const uint32_t e = wt_tail[b + i];
+            const uint32_t e = A.wt_tail[b + i];
             const uint32_t k = e & 0xFFFFu;
-            phi[k] = ((float)(e >> 16) + beta) * inv[k];
+            phi[k] = ((float)(e >> 16) + beta) * A.inv[k];
         }
     }
     __syncthreads();
@@ -387,47 +405,59 @@ __global__ void __launch_bounds__(1024) k_sample(
     const float Q = qacc[NS - 1];
     const float f1 = f123[0], f2 = f123[1], f3 = f123[2];
     const uint32_t k1 = tp.x;
-    uint32_t n_full = 0, n_scanned = 0;
 
     for (uint32_t base = sg.y + (uint32_t)wv * 64u; base < sg.z; base += (uint32_t)nwv * 64u) {
-        const uint32_t t = base + (uint32_t)lane;
-        if (t >= sg.z) break;
-        const uint32_t d = tok_doc[t];
-        const uint32_t a = aux[t];
-        const uint2 di = doc_info[d];
-        const float u = ez_u24(ez_philox_x0(seed, t, sweep, EZ_TAG_SAMPLE));
+        const uint32_t r = base + (uint32_t)lane;
+        if (r >= sg.z) break;
+        const uint2 rn = A.runs[r];
+        const uint32_t t0 = rn.x, len = rn.y;
+        const uint32_t d = A.tok_doc[t0];
+        const uint32_t a = A.aux[t0];
+        const uint2 di = A.doc_info[d];
         const uint32_t n1 = a & 0xFFFFu, n2 = a >> 16, nd = di.y >> 16;
         const float P1 = ((float)n1 + alpha) * f1;
-        bool need = true;
-        if (SHORTCUT) {
-            const float maxS = (float)n2 * f2 + (float)(nd - n1 - n2) * f3;
-            const float B = (P1 + maxS) + Q;
-            if (u * B < P1 * kMargin) need = false;
-        }
-        uint32_t newz = k1;
-        if (need) {
-            const uint4 *rp = reinterpret_cast<const uint4 *>(dt + di.x);
-            const uint32_t nnz = di.y & 0xFFFFu;
-            const uint32_t nb = (nnz + 4u * kBlkU4 - 1u) / (4u * kBlkU4);  // blocks of the row
-            n_full += 1u;
-            n_scanned += nnz;
-            // S: one FMA per entry, left to right; cp[b] = running sum at the
-            // end of block b (the first kCP blocks), so the S-bucket search
-            // re-reads one block instead of the row up to the target
-            float cp[kCP];
-            float S = 0.0f;
-#pragma unroll
-            for (int b = 0; b < kCP; ++b) {
-                if ((uint32_t)b < nb) S = block_sum(rp + kBlkU4 * b, phi, S);
-                cp[b] = S;
+        // tokens of the run that need the full draw; shortcut tokens take k1
+        uint32_t need = 0;
+        for (uint32_t j = 0; j < len; ++j) {
+            bool full = true;
+            if (SHORTCUT) {
+                const float u = ez_u24(ez_philox_x0(A.seed, t0 + j, sweep, EZ_TAG_SAMPLE));
+                const float maxS = (float)n2 * f2 + (float)(nd - n1 - n2) * f3;
+                const float B = (P1 + maxS) + Q;
+                full = !(u * B < P1 * kMargin);
             }
-            for (uint32_t b = kCP; b < nb; ++b) S = block_sum(rp + kBlkU4 * b, phi, S);
-            const float A = P1 + S;
-            const float total = A + Q;
+            if (full) need |= 1u << j;
+            else A.z[t0 + j] = (uint16_t)k1;
+        }
+        if (!need) continue;
+        const uint4 *rp = reinterpret_cast<const uint4 *>(A.dt + di.x);
+        const uint32_t nnz = di.y & 0xFFFFu;
+        const uint32_t nb = (nnz + 4u * kBlkU4 - 1u) / (4u * kBlkU4);  // blocks of the row
+        n_full += (uint32_t)__popc(need);
+        n_scanned += nnz;
+        // S: one FMA per entry, left to right; cp[b] = running sum at the
+        // end of block b (the first kCP blocks), so the S-bucket search
+        // re-reads one block instead of the row up to the target
+        float cp[kCP];
+        float S = 0.0f;
+#pragma unroll
+        for (int b = 0; b < kCP; ++b) {
+            if ((uint32_t)b < nb) S = block_sum(rp + kBlkU4 * b, phi, S);
+            cp[b] = S;
+        }
+        for (uint32_t b = kCP; b < nb; ++b) S = block_sum(rp + kBlkU4 * b, phi, S);
+        const float Aa = P1 + S;
+        const float total = Aa + Q;
+        while (need) {
+            const uint32_t j = (uint32_t)__ffs(need) - 1u;
+            need &= need - 1u;
+            const uint32_t t = t0 + j;
+            const float u = ez_u24(ez_philox_x0(A.seed, t, sweep, EZ_TAG_SAMPLE));
             const float target = u * total;
+            uint32_t newz = k1;
             if (target < P1) {
                 newz = k1;
-            } else if (target < A) {
+            } else if (target < Aa) {
                 const float t2 = target - P1;
                 int bs = -1;
 #pragma unroll
@@ -461,12 +491,12 @@ __global__ void __launch_bounds__(1024) k_sample(
                     run = r3;
                 }
                 if (!found) {  // rounding left every running sum at or below t2
-                    const uint32_t *r = dt + di.x;
+                    const uint32_t *rr = A.dt + di.x;
                     for (uint32_t i = 0; i < nnz; ++i)
-                        if (dt_count(r[i]) > 0.0f && phi_of(phi, r[i]) > 0.0f) newz = dt_topic(r[i]);
+                        if (dt_count(rr[i]) > 0.0f && phi_of(phi, rr[i]) > 0.0f) newz = dt_topic(rr[i]);
                 }
             } else {
-                const float t2 = target - A;
+                const float t2 = target - Aa;
                 int lo = 0, hi = NS;  // first slab whose running sum exceeds t2
                 while (lo < hi) {
                     const int mid = (lo + hi) >> 1;
@@ -474,9 +504,9 @@ __global__ void __launch_bounds__(1024) k_sample(
                 }
                 int js = lo < NS ? lo : -1;
                 if (js < 0)
-                    for (int j = NS - 1; j >= 0; --j)
-                        if (qt[j] > 0.0f) {
-                            js = j;
+                    for (int jj = NS - 1; jj >= 0; --jj)
+                        if (qt[jj] > 0.0f) {
+                            js = jj;
                             break;
                         }
                 if (js >= 0) {
@@ -499,17 +529,48 @@ __global__ void __launch_bounds__(1024) k_sample(
                     }
                 }
             }
+            A.z[t] = (uint16_t)newz;
         }
-        z[t] = (uint16_t)newz;
+    }
+}
+
+// One workgroup per item (grid = items), or persistent workgroups pulling
+// items from n_queues counters: queue q holds the items with index = q mod
+// n_queues (the items of one XCD when items are dealt round-robin to XCDs),
+// a workgroup drains its own queue (blockIdx mod n_queues) and then the
+// others.  Every workgroup leaves once every queue is past its last item.
+template <bool SHORTCUT>
+__global__ void __launch_bounds__(1024) k_sample(const SampleArgs A) {
+    extern __shared__ float smf[];
+    __shared__ float f123[3];
+    __shared__ uint32_t s_item;
+    const uint32_t sweep = A.counters[0];
+    uint32_t n_full = 0, n_scanned = 0;
+    if (A.n_queues == 0) {
+        sample_item<SHORTCUT>(A, blockIdx.x, smf, f123, sweep, n_full, n_scanned);
+    } else {
+        const uint32_t nq = A.n_queues, x0 = blockIdx.x % nq;
+        for (uint32_t h = 0; h < nq; ++h) {
+            const uint32_t q = (x0 + h) % nq;
+            while (true) {
+                __syncthreads();  // the previous item is done with s_item and LDS
+                if (threadIdx.x == 0) s_item = atomicAdd(&A.queue[q], 1u);
+                __syncthreads();
+                const uint64_t item = (uint64_t)q + (uint64_t)s_item * nq;
+                if (item >= A.n_items) break;
+                sample_item<SHORTCUT>(A, (uint32_t)item, smf, f123, sweep, n_full, n_scanned);
+            }
+        }
     }
     // per-wavefront totals of the counters
+    const int lane = threadIdx.x & 63;
     for (int off = 32; off >= 1; off >>= 1) {
         n_full += __shfl_xor(n_full, off);
         n_scanned += __shfl_xor(n_scanned, off);
     }
     if (lane == 0 && n_full) {
-        atomicAdd(&stats[0], (unsigned long long)n_full);
-        atomicAdd(&stats[1], (unsigned long long)n_scanned);
+        atomicAdd(&A.stats[0], (unsigned long long)n_full);
+        atomicAdd(&A.stats[1], (unsigned long long)n_scanned);
     }
 }
 
@@ -518,7 +579,8 @@ __global__ void __launch_bounds__(1024) k_sample(
 // per token, the term of profilePerp/utility.cu:2398,2810 reduced in FP64
 // (profilePerp/utility.cu:2858-2928).  One workgroup per sampler item; the
 // per-item partials are summed in fixed order by k_ll_final.
-__global__ void __launch_bounds__(1024) k_ll(const uint4 *__restrict__ segs, int VD, int K, double alpha, double beta,
+__global__ void __launch_bounds__(1024) k_ll(const uint4 *__restrict__ segs, const uint2 *__restrict__ runs, int VD,
+                                             int K, double alpha, double beta,
                                              double vbeta, const int32_t *__restrict__ wt_head,
                                              const uint32_t *__restrict__ tail_off,
                                              const uint32_t *__restrict__ wt_tail,
@@ -561,8 +623,9 @@ __global__ void __launch_bounds__(1024) k_ll(const uint4 *__restrict__ segs, int
     q *= alpha;
     __syncthreads();
     double acc = 0.0;
-    for (uint32_t t = sg.y + (uint32_t)wv; t < sg.z; t += (uint32_t)nwv) {
-        const uint2 di = doc_info[tok_doc[t]];
+    for (uint32_t r = sg.y + (uint32_t)wv; r < sg.z; r += (uint32_t)nwv) {
+        const uint2 rn = runs[r];  // rn.y tokens of this word in one doc share the term
+        const uint2 di = doc_info[tok_doc[rn.x]];
         const uint32_t nnz = di.y & 0xFFFFu;
         double s = 0.0;
         for (uint32_t i = lane; i < nnz; i += 64) {
@@ -570,7 +633,7 @@ __global__ void __launch_bounds__(1024) k_ll(const uint4 *__restrict__ segs, int
             s += (double)(e >> 16) * phi[dt_topic(e)];
         }
         for (int off = 32; off >= 1; off >>= 1) s += __shfl_xor(s, off);
-        acc += log((s + q) / ((double)(di.y >> 16) + (double)K * alpha));
+        acc += (double)rn.y * log((s + q) / ((double)(di.y >> 16) + (double)K * alpha));
     }
     if (lane == 0) red[wv] = acc;
     __syncthreads();
@@ -879,18 +942,54 @@ size_t sample_lds_bytes(int K, int bd) {
     return (2 * (size_t)NS * 64 + 2 * (size_t)NS) * sizeof(float);
 }
 
+size_t max_lds_bytes(int K, int bd) {
+    size_t m = sample_lds_bytes(K, bd);
+    m = std::max(m, 4 * wave_hist_bytes(K));                         // WT tail, top-3, DT build
+    m = std::max(m, (size_t)K * sizeof(double));                     // log-likelihood
+    m = std::max(m, (size_t)K * sizeof(float) + 1024 * sizeof(float));  // fold-in tables
+    m = std::max(m, (size_t)foldin_waves_per_block(K) * (((K + 1) >> 1) + K) * sizeof(uint32_t));
+    return m;
+}
+
 hipError_t launch_sample(const DevState &s, bool shortcut, hipStream_t st) {
     if (s.n_smp_segs == 0) return hipSuccess;
     const int bd = s.block_dim;
     const size_t lds = sample_lds_bytes(s.K, bd);
+    SampleArgs A;
+    A.segs = s.smp_segs;
+    A.runs = s.runs;
+    A.n_items = (uint32_t)s.n_smp_segs;
+    A.VD = s.VD;
+    A.K = s.K;
+    A.alpha = s.alpha;
+    A.beta = s.beta;
+    A.wt_head = s.wt_head;
+    A.tail_off = s.tail_off;
+    A.wt_tail = s.wt_tail;
+    A.tail_nnz = s.tail_nnz;
+    A.inv = s.inv;
+    A.wtop = s.wtop;
+    A.tok_doc = s.tok_doc;
+    A.aux = s.aux;
+    A.doc_info = s.doc_info;
+    A.dt = s.dt;
+    A.z = s.z;
+    A.counters = s.counters;
+    A.queue = s.counters + 8;  // counters[8 .. 8 + n_queues)
+    A.n_queues = s.grid_dim > 0 ? (uint32_t)s.n_queues : 0u;
+    A.stats = s.stats;
+    A.seed = s.seed;
+    unsigned grid = (unsigned)s.n_smp_segs;
+    if (A.n_queues) {
+        // queue heads start at zero (a fill kernel, so a captured sweep
+        // replays it)
+        hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(64), 0, st, s.counters + 8, (size_t)A.n_queues, 0u);
+        grid = (unsigned)s.grid_dim;
+    }
     if (shortcut)
-        hipLaunchKernelGGL(k_sample<true>, dim3(s.n_smp_segs), dim3(bd), lds, st, s.smp_segs, s.VD, s.K, s.alpha,
-                           s.beta, s.wt_head, s.tail_off, s.wt_tail, s.tail_nnz, s.inv, s.wtop, s.tok_doc, s.aux,
-                           s.doc_info, s.dt, s.z, s.counters, s.stats, s.seed);
+        hipLaunchKernelGGL(k_sample<true>, dim3(grid), dim3(bd), lds, st, A);
     else
-        hipLaunchKernelGGL(k_sample<false>, dim3(s.n_smp_segs), dim3(bd), lds, st, s.smp_segs, s.VD, s.K, s.alpha,
-                           s.beta, s.wt_head, s.tail_off, s.wt_tail, s.tail_nnz, s.inv, s.wtop, s.tok_doc, s.aux,
-                           s.doc_info, s.dt, s.z, s.counters, s.stats, s.seed);
+        hipLaunchKernelGGL(k_sample<false>, dim3(grid), dim3(bd), lds, st, A);
     return hipGetLastError();
 }
 
@@ -905,7 +1004,8 @@ hipError_t launch_loglik(const DevState &s, hipStream_t st) {
         return hipGetLastError();
     }
     const int bd = 256;
-    hipLaunchKernelGGL(k_ll, dim3(s.n_smp_segs), dim3(bd), (size_t)s.K * sizeof(double), st, s.smp_segs, s.VD, s.K,
+    hipLaunchKernelGGL(k_ll, dim3(s.n_smp_segs), dim3(bd), (size_t)s.K * sizeof(double), st, s.smp_segs, s.runs, s.VD,
+                       s.K,
                        (double)s.alpha, (double)s.beta, (double)s.V * (double)s.beta, s.wt_head, s.tail_off,
                        s.wt_tail, s.tail_nnz, s.nk, s.tok_doc, s.doc_info, s.dt, s.ll_part);
     hipLaunchKernelGGL(k_ll_final, dim3(1), dim3(256), 0, st, s.ll_part, s.n_smp_segs, s.ll_out);

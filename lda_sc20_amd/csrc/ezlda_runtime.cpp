@@ -187,6 +187,46 @@ std::vector<uint4> build_sample_items(const ezlda_args &a, const std::vector<uin
     return out;
 }
 
+// Runs: a word's consecutive tokens in one document (tokens of a word are
+// numbered in doc order), at most `cap` each.  Inside an item the runs are
+// ordered longest first (then by token), so the lanes of a wavefront draw
+// similar numbers of tokens; the order does not affect any draw.  Items are
+// rewritten as run ranges (w, first run, end run, 0).
+std::vector<uint2> build_runs(std::vector<uint4> &items, const std::vector<uint32_t> &tok_doc, uint32_t cap) {
+    const int64_t n = (int64_t)items.size();
+    std::vector<uint64_t> cnt(n + 1, 0);
+    auto for_runs = [&](const uint4 &it, auto &&emit) {
+        uint32_t t = it.y;
+        while (t < it.z) {
+            uint32_t e = t + 1;
+            while (e < it.z && e - t < cap && tok_doc[e] == tok_doc[t]) ++e;
+            emit(t, e - t);
+            t = e;
+        }
+    };
+    parallel_for(n, [&](int64_t b, int64_t e, int) {
+        for (int64_t i = b; i < e; ++i) {
+            uint64_t k = 0;
+            if (items[i].x != 0xFFFFFFFFu) for_runs(items[i], [&](uint32_t, uint32_t) { ++k; });
+            cnt[i + 1] = k;
+        }
+    });
+    for (int64_t i = 0; i < n; ++i) cnt[i + 1] += cnt[i];
+    std::vector<uint2> runs(cnt[n]);
+    parallel_for(n, [&](int64_t b, int64_t e, int) {
+        for (int64_t i = b; i < e; ++i) {
+            uint4 &it = items[i];
+            if (it.x == 0xFFFFFFFFu) continue;
+            uint64_t k = cnt[i];
+            for_runs(it, [&](uint32_t t, uint32_t l) { runs[k++] = make_uint2(t, l); });
+            std::stable_sort(runs.begin() + cnt[i], runs.begin() + cnt[i + 1],
+                             [](const uint2 &x, const uint2 &y) { return x.y > y.y; });
+            it = make_uint4(it.x, (uint32_t)cnt[i], (uint32_t)cnt[i + 1], 0u);
+        }
+    });
+    return runs;
+}
+
 }  // namespace
 
 int ezc::refresh_counts(ezlda_ctx *c) {
@@ -215,6 +255,7 @@ void ezlda_default_args(ezlda_args *a) {
     a->dense_threshold = 0;  // K, as preprocess/streamTestBalance.cpp:291
     a->shortcut = 1;
     a->topic_bits = 16;
+    a->run_cap = 8;
 }
 
 const char *ezlda_last_error(const ezlda_ctx *ctx) { return ctx ? ctx->err.c_str() : g_err.c_str(); }
@@ -243,15 +284,30 @@ ezlda_ctx *ezlda_create(const ezlda_args *args) {
         return nullptr;
     }
     if (a.grid_dim < 0) a.grid_dim = 0;
+    if (a.run_cap == 0) a.run_cap = 8;
+    if (a.run_cap < 1 || a.run_cap > 32) {
+        fail(nullptr, EZLDA_EINVAL, "run_cap must be in [1, 32]");
+        return nullptr;
+    }
     int ndev = 0;
     if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) {
         fail(nullptr, EZLDA_EHIP, "no HIP device");
         return nullptr;
     }
-    const size_t lds = ez::sample_lds_bytes(a.K, a.block_dim);
-    if (lds > 160 * 1024) {
-        fail(nullptr, EZLDA_EINVAL, "K too large for the sampler's LDS at this block_dim");
-        return nullptr;
+    {
+        int dev = 0, lmax = 0;
+        const size_t lds = ez::max_lds_bytes(a.K, a.block_dim);
+        if (hipGetDevice(&dev) != hipSuccess ||
+            hipDeviceGetAttribute(&lmax, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) != hipSuccess) {
+            fail(nullptr, EZLDA_EHIP, "cannot query the device's LDS size");
+            return nullptr;
+        }
+        if (lds > (size_t)lmax) {
+            fail(nullptr, EZLDA_EINVAL,
+                 "K=" + std::to_string(a.K) + " needs " + std::to_string(lds) + " bytes of LDS per block; the device "
+                 "offers " + std::to_string(lmax));
+            return nullptr;
+        }
     }
     ezlda_ctx *c = new ezlda_ctx();
     c->a = a;
@@ -418,6 +474,7 @@ int ezlda_load_docs(ezlda_ctx *c, int32_t V, int32_t D, const int64_t *doc_off, 
             wt_segs.push_back(make_uint4((uint32_t)r, b, std::min(b + WT_SEG, c->word_off[r + 1]), 0u));
     smp_segs = build_sample_items(c->a, c->word_off, tok_rank, doc_wrd, c->doc_off, dord, VD, K, c->n_xcd,
                                   &c->doc_tiles);
+    const std::vector<uint2> runs = build_runs(smp_segs, tok_doc, (uint32_t)c->a.run_cap);
 
     ez::DevState &s = c->s;
     s.K = K; s.V = V; s.VD = VD; s.D = D; s.N = N;
@@ -453,6 +510,9 @@ int ezlda_load_docs(ezlda_ctx *c, int32_t V, int32_t D, const int64_t *doc_off, 
     s.n_wt_segs = (int32_t)wt_segs.size();
     if ((r = upload(c, &s.smp_segs, smp_segs))) return r;
     s.n_smp_segs = (int32_t)smp_segs.size();
+    if ((r = upload(c, &s.runs, runs))) return r;
+    s.n_runs = (int64_t)runs.size();
+    s.n_queues = c->doc_tiles > 0 ? std::min(c->n_xcd, 8) : 1;
     if ((r = dalloc(c, &s.counters, 16))) return r;
     if ((r = dalloc(c, &s.stats, 2))) return r;
     if ((r = dalloc(c, &s.ll_part, smp_segs.size()))) return r;
@@ -491,6 +551,8 @@ int ezlda_get_info(ezlda_ctx *c, ezlda_info *info) {
     info->device_bytes = c->dev_bytes;
     info->doc_tiles = c->doc_tiles;
     info->num_xcd = c->n_xcd;
+    info->run_cap = c->a.run_cap;
+    info->num_runs = c->s.n_runs;
     return EZLDA_OK;
 }
 

@@ -46,14 +46,15 @@ class Args(C.Structure):
                 ("grid_dim", C.c_int32), ("num_chunks", C.c_int32), ("num_streams", C.c_int32),
                 ("token_segment", C.c_int32), ("seed", C.c_uint64), ("dense_words", C.c_int32),
                 ("dense_threshold", C.c_int32), ("shortcut", C.c_int32), ("topic_bits", C.c_int32),
-                ("doc_tiles", C.c_int32)]
+                ("doc_tiles", C.c_int32), ("run_cap", C.c_int32)]
 
 
 class Info(C.Structure):
     _fields_ = [("num_tokens", C.c_int64), ("num_words", C.c_int32), ("num_docs", C.c_int32),
                 ("num_head_words", C.c_int32), ("K", C.c_int32), ("dt_capacity", C.c_int64),
                 ("wt_tail_capacity", C.c_int64), ("num_sample_items", C.c_int32), ("sweep", C.c_uint32),
-                ("device_bytes", C.c_int64), ("doc_tiles", C.c_int32), ("num_xcd", C.c_int32)]
+                ("device_bytes", C.c_int64), ("doc_tiles", C.c_int32), ("num_xcd", C.c_int32),
+                ("run_cap", C.c_int32), ("num_runs", C.c_int64)]
 
 
 class ServerArgs(C.Structure):

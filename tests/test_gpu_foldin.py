@@ -40,7 +40,8 @@ def _docs(V, lens, seed=5):
 LENS = [0, 1, 2, 63, 64, 65, 129, 500, 1200, 37, 5, 300]
 
 
-@pytest.mark.parametrize("K,kw", [(64, {}), (100, {"dense_threshold": 30}), (7, {}), (1024, {})])
+@pytest.mark.parametrize("K,kw", [(64, {}), (100, {"dense_threshold": 30}), (7, {}), (1024, {}),
+                                  (10000, {"dense_threshold": 100})])
 def test_server_matches_oracle(K, kw):
     ctx, orc = _model(K, **kw)
     off, words = _docs(300, LENS)

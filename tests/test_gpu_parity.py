@@ -83,6 +83,16 @@ CASES = [
     (300, 700, 200, 120, {"token_segment": 64, "dense_words": 25}),
     (64, 400, 300, 60, {"block_dim": 64}),
     (64, 400, 300, 60, {"block_dim": 512}),
+    # one DT scan per token; long runs; persistent workgroups on item queues
+    (64, 400, 300, 60, {"run_cap": 1}),
+    (64, 400, 300, 60, {"run_cap": 32, "grid_dim": 37}),
+    (1024, 600, 150, 300, {"dense_threshold": 200, "run_cap": 3, "grid_dim": 300, "doc_tiles": 16}),
+    (100, 500, 200, 80, {"grid_dim": 1, "doc_tiles": -1}),
+    # large K: the per-block LDS of the sampler, the histograms and the
+    # log-likelihood pass 64 KiB (the topic count of the large-K corpus, and
+    # the largest K the 16-bit DT entries allow)
+    (10000, 300, 120, 200, {"dense_threshold": 150}),
+    (16383, 150, 40, 120, {"dense_threshold": 100}),
 ]
 
 
@@ -117,10 +127,11 @@ def test_head_tail_split_does_not_change_topics():
     assert np.array_equal(zs[0], zs[1]) and np.array_equal(zs[0], zs[2])
 
 
-def test_graph_replay_matches_eager():
+@pytest.mark.parametrize("kw", [{}, {"grid_dim": 40, "doc_tiles": 8}])
+def test_graph_replay_matches_eager(kw):
     off, words = make_corpus(5, 300, 400, 70)
-    a, orc = _pair(off, words, 400, 64)
-    b, _ = _pair(off, words, 400, 64)
+    a, orc = _pair(off, words, 400, 64, **kw)
+    b, _ = _pair(off, words, 400, 64, **kw)
     for _ in range(3):
         a.sweep_graph(1)
         b.sweep(1)
