@@ -161,6 +161,26 @@ int ezlda_reset_stats(ezlda_ctx *ctx);
 int ezlda_get_topics(ezlda_ctx *ctx, uint16_t *z);
 int ezlda_set_topics(ezlda_ctx *ctx, const uint16_t *z);
 int ezlda_set_sweep(ezlda_ctx *ctx, uint32_t sweep);
+
+/* ---- layout tuning (no reference counterpart: the reference's BlockDim,
+ * GridDim, tokenSegment and head threshold are compile-time constants,
+ * Argument.cuh:3-9, preprocess/streamTestBalance.cpp:291).  Topics never
+ * depend on these arguments, so they can change between sweeps. */
+/* Rebuilds the layout with args' block_dim, grid_dim, token_segment,
+ * dense_words, dense_threshold, doc_tiles and run_cap (others ignored). */
+int ezlda_set_layout(ezlda_ctx *ctx, const ezlda_args *args);
+
+typedef struct ezlda_tune_trial {
+    ezlda_args args;
+    double ms_per_sweep;
+} ezlda_tune_trial;
+/* Coordinate descent over the layout arguments (head size, run cap, doc
+ * tiles, block size, token segment, persistent grid), each trial one untimed
+ * and sweeps_per_trial timed sweeps; the sweeps are ordinary sweeps of the
+ * run.  Leaves the context on the fastest layout and reports it; trials may
+ * be NULL. */
+int ezlda_autotune(ezlda_ctx *ctx, int32_t sweeps_per_trial, ezlda_args *chosen, double *ms_per_sweep,
+                   ezlda_tune_trial *trials, int32_t max_trials, int32_t *num_trials);
 int ezlda_get_nk(ezlda_ctx *ctx, int32_t *nk);
 /* Dense V x K word-topic counts, rows by input word id. */
 int ezlda_get_wt_dense(ezlda_ctx *ctx, int32_t *wt);

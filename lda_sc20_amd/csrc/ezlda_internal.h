@@ -18,7 +18,9 @@ struct ezlda_ctx {
     uint32_t sweep = 0;
     int64_t dev_bytes = 0;
     int64_t dt_cap = 0, tail_cap = 0, dt_cap_canon = 0;
-    std::vector<void *> allocs;
+    std::vector<void *> allocs;         // corpus-level device arrays
+    std::vector<void *> layout_allocs;  // arrays the tuning arguments decide
+    int64_t layout_bytes = 0;
     // host mirrors for result mapping
     std::vector<uint32_t> doc_tok;   // doc-major position -> token id
     std::vector<int32_t> rank_of;    // input word -> rank

@@ -6,6 +6,7 @@
 #include <algorithm>
 #include <array>
 #include <cstdio>
+#include <cmath>
 #include <cstring>
 #include <string>
 #include <thread>
@@ -90,9 +91,31 @@ void free_device(ezlda_ctx *c) {
     c->graph = nullptr;
     for (void *p : c->allocs) hipFree(p);
     c->allocs.clear();
+    for (void *p : c->layout_allocs) hipFree(p);
+    c->layout_allocs.clear();
     c->dev_bytes = 0;
+    c->layout_bytes = 0;
     c->s = ez::DevState{};
     c->loaded = false;
+}
+
+template <class T>
+int ldalloc(ezlda_ctx *c, T **p, size_t n) {
+    *p = nullptr;
+    const size_t bytes = std::max<size_t>(n, 1) * sizeof(T);
+    hipError_t e = hipMalloc((void **)p, bytes);
+    if (e != hipSuccess) return fail(c, EZLDA_EHIP, std::string("hipMalloc: ") + hipGetErrorString(e));
+    c->layout_allocs.push_back((void *)*p);
+    c->layout_bytes += (int64_t)bytes;
+    return EZLDA_OK;
+}
+
+template <class T>
+int lupload(ezlda_ctx *c, T **p, const std::vector<T> &v) {
+    int r = ldalloc(c, p, v.size());
+    if (r) return r;
+    if (!v.empty()) EZ_TRY_HIP(c, hipMemcpy(*p, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice));
+    return EZLDA_OK;
 }
 
 int need_loaded(ezlda_ctx *c) {
@@ -225,6 +248,75 @@ std::vector<uint2> build_runs(std::vector<uint4> &items, const std::vector<uint3
         }
     });
     return runs;
+}
+
+// The part of the device layout that the tuning arguments decide (block_dim,
+// grid_dim, token_segment, dense_words / dense_threshold, doc_tiles,
+// run_cap): the head/tail split of WT, the WT head segments, the sampler
+// items and runs.  Token order, DT rows and z do not depend on it, so it can
+// be rebuilt on a loaded corpus; counts are stale afterwards.
+int build_layout(ezlda_ctx *c, const std::vector<uint32_t> &tok_doc, const std::vector<uint32_t> &doc_wrd,
+                 const std::vector<int32_t> &dord) {
+    ez::DevState &s = c->s;
+    const int V = s.V, K = s.K, D = s.D;
+    if (c->graph_exec) hipGraphExecDestroy(c->graph_exec);
+    if (c->graph) hipGraphDestroy(c->graph);
+    c->graph_exec = nullptr;
+    c->graph = nullptr;
+    EZ_TRY_HIP(c, hipStreamSynchronize(c->st));
+    for (void *p : c->layout_allocs) hipFree(p);
+    c->layout_allocs.clear();
+    c->layout_bytes = 0;
+    // head / tail split (words by rank: count desc)
+    auto count = [&](int r) { return (int64_t)(c->word_off[r + 1] - c->word_off[r]); };
+    int VD;
+    if (c->a.dense_words >= 0) {
+        VD = std::min(c->a.dense_words, V);
+    } else {
+        const int64_t thr = c->a.dense_threshold > 0 ? c->a.dense_threshold : K;
+        VD = 0;
+        while (VD < V && count(VD) >= thr) ++VD;
+    }
+    while (VD < V && count(VD) >= 65536) ++VD;  // tail counts are 16-bit
+    std::vector<uint32_t> tail_off(V - VD + 1, 0);
+    for (int r = VD; r < V; ++r)
+        tail_off[r - VD + 1] = tail_off[r - VD] + std::min<uint32_t>((uint32_t)count(r), (uint32_t)K);
+    // work items
+    const uint32_t WT_SEG = 32768;
+    std::vector<uint4> wt_segs;
+    for (int r = 0; r < VD; ++r)
+        for (uint32_t b = c->word_off[r]; b < c->word_off[r + 1]; b += WT_SEG)
+            wt_segs.push_back(make_uint4((uint32_t)r, b, std::min(b + WT_SEG, c->word_off[r + 1]), 0u));
+    std::vector<uint32_t> drank(D);
+    for (int q = 0; q < D; ++q) drank[dord[q]] = (uint32_t)q;
+    std::vector<uint32_t> tok_rank(tok_doc.size());
+    parallel_for((int64_t)tok_doc.size(), [&](int64_t b, int64_t e, int) {
+        for (int64_t t = b; t < e; ++t) tok_rank[t] = drank[tok_doc[t]];
+    });
+    std::vector<uint4> smp_segs = build_sample_items(c->a, c->word_off, tok_rank, doc_wrd, c->doc_off, dord, VD, K,
+                                                     c->n_xcd, &c->doc_tiles);
+    tok_rank.clear();
+    tok_rank.shrink_to_fit();
+    const std::vector<uint2> runs = build_runs(smp_segs, tok_doc, (uint32_t)c->a.run_cap);
+    int r = 0;
+    if ((r = ldalloc(c, &s.wt_head, (size_t)VD * K))) return r;
+    if ((r = lupload(c, &s.tail_off, tail_off))) return r;
+    if ((r = ldalloc(c, &s.wt_tail, tail_off.back()))) return r;
+    if ((r = ldalloc(c, &s.tail_nnz, V - VD))) return r;
+    if ((r = lupload(c, &s.wt_segs, wt_segs))) return r;
+    if ((r = lupload(c, &s.smp_segs, smp_segs))) return r;
+    if ((r = lupload(c, &s.runs, runs))) return r;
+    if ((r = ldalloc(c, &s.ll_part, smp_segs.size()))) return r;
+    s.VD = VD;
+    s.n_wt_segs = (int32_t)wt_segs.size();
+    s.n_smp_segs = (int32_t)smp_segs.size();
+    s.n_runs = (int64_t)runs.size();
+    s.n_queues = c->doc_tiles > 0 ? std::min(c->n_xcd, 8) : 1;
+    s.block_dim = c->a.block_dim;
+    s.grid_dim = c->a.grid_dim;
+    c->tail_cap = tail_off.back();
+    c->counts_fresh = false;
+    return EZLDA_OK;
 }
 
 }  // namespace
@@ -409,7 +501,7 @@ int ezlda_load_docs(ezlda_ctx *c, int32_t V, int32_t D, const int64_t *doc_off, 
             cur += n;
         }
     }
-    std::vector<uint32_t> tok_doc(N), tok_rank(N), doc_wrd(N);
+    std::vector<uint32_t> tok_doc(N), doc_wrd(N);
     std::vector<uint16_t> z(N);
     c->doc_tok.assign(N, 0);
     {
@@ -423,7 +515,6 @@ int ezlda_load_docs(ezlda_ctx *c, int32_t V, int32_t D, const int64_t *doc_off, 
                         const int32_t r = c->rank_of[words[i]];
                         const uint32_t tk = cur[r]++;
                         tok_doc[tk] = (uint32_t)d;
-                        tok_rank[tk] = (uint32_t)q;
                         c->doc_tok[i] = tk;
                         doc_wrd[i] = (uint32_t)r;
                         if (z0) z[tk] = z0[i];
@@ -440,21 +531,6 @@ int ezlda_load_docs(ezlda_ctx *c, int32_t V, int32_t D, const int64_t *doc_off, 
                 z[t] = (uint16_t)(ez_philox_x0(seed, (uint64_t)t, 0u, EZ_TAG_INIT) % (uint32_t)K);
         });
     }
-    // head / tail split
-    int VD;
-    if (c->a.dense_words >= 0) {
-        VD = std::min(c->a.dense_words, V);
-    } else {
-        const int64_t thr = c->a.dense_threshold > 0 ? c->a.dense_threshold : K;
-        VD = 0;
-        while (VD < V && cnt[ids[VD]] >= thr) ++VD;
-    }
-    while (VD < V && cnt[ids[VD]] >= 65536) ++VD;  // tail counts are 16-bit
-    std::vector<uint32_t> tail_off(V - VD + 1, 0);
-    for (int r = VD; r < V; ++r) {
-        const uint32_t n = c->word_off[r + 1] - c->word_off[r];
-        tail_off[r - VD + 1] = tail_off[r - VD] + std::min<uint32_t>(n, (uint32_t)K);
-    }
     // DT rows in the doc order above, each 128-byte aligned with capacity
     // min(n_d, K) rounded up to 32 entries (the build zero-fills the rest)
     std::vector<uint2> doc_info(D);
@@ -466,27 +542,14 @@ int ezlda_load_docs(ezlda_ctx *c, int32_t V, int32_t D, const int64_t *doc_off, 
         dtc += (std::min<uint32_t>(n, (uint32_t)K) + 31u) & ~31u;
     }
     if (dtc >= 0xFFFFFFF0ull) return fail(c, EZLDA_EINVAL, "DT capacity exceeds 2^32 entries");
-    // work items
-    const uint32_t WT_SEG = 32768;
-    std::vector<uint4> wt_segs, smp_segs;
-    for (int r = 0; r < VD; ++r)
-        for (uint32_t b = c->word_off[r]; b < c->word_off[r + 1]; b += WT_SEG)
-            wt_segs.push_back(make_uint4((uint32_t)r, b, std::min(b + WT_SEG, c->word_off[r + 1]), 0u));
-    smp_segs = build_sample_items(c->a, c->word_off, tok_rank, doc_wrd, c->doc_off, dord, VD, K, c->n_xcd,
-                                  &c->doc_tiles);
-    const std::vector<uint2> runs = build_runs(smp_segs, tok_doc, (uint32_t)c->a.run_cap);
-
     ez::DevState &s = c->s;
-    s.K = K; s.V = V; s.VD = VD; s.D = D; s.N = N;
+    s.K = K; s.V = V; s.D = D; s.N = N;
     s.alpha = c->a.alpha; s.beta = c->a.beta;
     s.vbeta = (float)((double)V * (double)c->a.beta);
     s.seed = c->a.seed;
-    s.block_dim = c->a.block_dim;
-    s.grid_dim = c->a.grid_dim;
     c->dt_cap = (int64_t)dtc;
     c->dt_cap_canon = 0;
     for (int d = 0; d < D; ++d) c->dt_cap_canon += std::min<uint32_t>(c->doc_off[d + 1] - c->doc_off[d], (uint32_t)K);
-    c->tail_cap = tail_off.back();
     int r = 0;
     if ((r = upload(c, &s.word_off, c->word_off))) return r;
     if ((r = upload(c, &s.tok_doc, tok_doc))) return r;
@@ -499,24 +562,13 @@ int ezlda_load_docs(ezlda_ctx *c, int32_t V, int32_t D, const int64_t *doc_off, 
     // 256 entries of zero padding past the last row
     if ((r = dalloc(c, &s.dt, dtc + 256))) return r;
     EZ_TRY_HIP(c, hipMemset(s.dt, 0, (dtc + 256) * sizeof(uint32_t)));
-    if ((r = dalloc(c, &s.wt_head, (size_t)VD * K))) return r;
-    if ((r = upload(c, &s.tail_off, tail_off))) return r;
-    if ((r = dalloc(c, &s.wt_tail, tail_off.back()))) return r;
-    if ((r = dalloc(c, &s.tail_nnz, V - VD))) return r;
     if ((r = dalloc(c, &s.nk, K))) return r;
     if ((r = dalloc(c, &s.inv, K))) return r;
     if ((r = dalloc(c, &s.wtop, V))) return r;
-    if ((r = upload(c, &s.wt_segs, wt_segs))) return r;
-    s.n_wt_segs = (int32_t)wt_segs.size();
-    if ((r = upload(c, &s.smp_segs, smp_segs))) return r;
-    s.n_smp_segs = (int32_t)smp_segs.size();
-    if ((r = upload(c, &s.runs, runs))) return r;
-    s.n_runs = (int64_t)runs.size();
-    s.n_queues = c->doc_tiles > 0 ? std::min(c->n_xcd, 8) : 1;
     if ((r = dalloc(c, &s.counters, 16))) return r;
     if ((r = dalloc(c, &s.stats, 2))) return r;
-    if ((r = dalloc(c, &s.ll_part, smp_segs.size()))) return r;
     if ((r = dalloc(c, &s.ll_out, 1))) return r;
+    if ((r = build_layout(c, tok_doc, doc_wrd, dord))) return r;
     EZ_TRY_HIP(c, hipMemsetAsync(s.counters, 0, 16 * sizeof(uint32_t), c->st));
     EZ_TRY_HIP(c, hipMemsetAsync(s.stats, 0, 2 * sizeof(unsigned long long), c->st));
     EZ_TRY_HIP(c, hipStreamSynchronize(c->st));
@@ -548,7 +600,7 @@ int ezlda_get_info(ezlda_ctx *c, ezlda_info *info) {
     info->wt_tail_capacity = c->tail_cap;
     info->num_sample_items = c->s.n_smp_segs;
     info->sweep = c->sweep;
-    info->device_bytes = c->dev_bytes;
+    info->device_bytes = c->dev_bytes + c->layout_bytes;
     info->doc_tiles = c->doc_tiles;
     info->num_xcd = c->n_xcd;
     info->run_cap = c->a.run_cap;
@@ -646,6 +698,142 @@ int ezlda_sweep_graph(ezlda_ctx *c, int32_t n) {
 // Back-to-back sweeps with no host synchronisation inside the timed region.
 // Eager: every stage is bracketed by events, all read after the last sweep.
 // Graph: one event pair around n replays (stage times are then -1).
+namespace {
+
+// Validates the layout arguments of `a` (those build_layout reads) and
+// fills their defaults; returns an error message or "".
+std::string check_layout_args(ezlda_args &a) {
+    if (a.block_dim <= 0) a.block_dim = 256;
+    if (a.block_dim % 64 != 0 || a.block_dim > 1024) return "block_dim must be a multiple of 64 and <= 1024";
+    if (a.token_segment <= 0) a.token_segment = 4096;
+    if (a.grid_dim < 0) a.grid_dim = 0;
+    if (a.run_cap == 0) a.run_cap = 8;
+    if (a.run_cap < 1 || a.run_cap > 32) return "run_cap must be in [1, 32]";
+    return "";
+}
+
+// Host copies of the corpus arrays the layout is built from.
+int layout_inputs(ezlda_ctx *c, std::vector<uint32_t> &tok_doc, std::vector<uint32_t> &doc_wrd,
+                  std::vector<int32_t> &dord) {
+    const ez::DevState &s = c->s;
+    tok_doc.resize(s.N);
+    doc_wrd.resize(s.N);
+    if (s.N) {
+        EZ_TRY_HIP(c, hipMemcpy(tok_doc.data(), s.tok_doc, (size_t)s.N * 4, hipMemcpyDeviceToHost));
+        EZ_TRY_HIP(c, hipMemcpy(doc_wrd.data(), s.doc_wrd, (size_t)s.N * 4, hipMemcpyDeviceToHost));
+    }
+    dord.resize(s.D);
+    for (int d = 0; d < s.D; ++d) dord[d] = d;
+    const std::vector<uint32_t> &off = c->doc_off;
+    std::sort(dord.begin(), dord.end(), [&](int32_t x, int32_t y) {
+        const uint32_t lx = off[x + 1] - off[x], ly = off[y + 1] - off[y];
+        return lx > ly || (lx == ly && x < y);
+    });
+    return EZLDA_OK;
+}
+
+}  // namespace
+
+int ezlda_set_layout(ezlda_ctx *c, const ezlda_args *args) {
+    if (int r = need_loaded(c)) return r;
+    if (!args) return fail(c, EZLDA_EINVAL, "null args");
+    ezlda_args a = c->a;
+    a.block_dim = args->block_dim;
+    a.grid_dim = args->grid_dim;
+    a.token_segment = args->token_segment;
+    a.dense_words = args->dense_words;
+    a.dense_threshold = args->dense_threshold;
+    a.doc_tiles = args->doc_tiles;
+    a.run_cap = args->run_cap;
+    const std::string m = check_layout_args(a);
+    if (!m.empty()) return fail(c, EZLDA_EINVAL, m);
+    std::vector<uint32_t> tok_doc, doc_wrd;
+    std::vector<int32_t> dord;
+    if (int r = layout_inputs(c, tok_doc, doc_wrd, dord)) return r;
+    c->a = a;
+    return build_layout(c, tok_doc, doc_wrd, dord);
+}
+
+int ezlda_autotune(ezlda_ctx *c, int32_t sweeps_per_trial, ezlda_args *chosen, double *ms_per_sweep,
+                   ezlda_tune_trial *trials, int32_t max_trials, int32_t *num_trials) {
+    if (int r = need_loaded(c)) return r;
+    if (sweeps_per_trial < 1) return fail(c, EZLDA_EINVAL, "sweeps_per_trial must be >= 1");
+    std::vector<uint32_t> tok_doc, doc_wrd;
+    std::vector<int32_t> dord;
+    if (int r = layout_inputs(c, tok_doc, doc_wrd, dord)) return r;
+    int dev = 0, ncu = 256;
+    if (hipGetDevice(&dev) != hipSuccess ||
+        hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || ncu < 1)
+        ncu = 256;
+    const int V = c->s.V;
+    int32_t nt = 0;
+    auto trial = [&](const ezlda_args &a, double *ms) -> int {
+        c->a = a;
+        if (int r = build_layout(c, tok_doc, doc_wrd, dord)) return r;
+        double t = 0.0, st[EZLDA_NUM_STAGES];
+        if (int r = ezlda_time_sweeps(c, 1, 0, &t, st)) return r;  // first sweep of a layout: not timed
+        if (int r = ezlda_time_sweeps(c, sweeps_per_trial, 0, &t, st)) return r;
+        *ms = t / sweeps_per_trial;
+        if (trials && nt < max_trials) {
+            trials[nt].args = a;
+            trials[nt].ms_per_sweep = *ms;
+        }
+        ++nt;
+        return EZLDA_OK;
+    };
+    ezlda_args best = c->a;
+    {
+        const std::string m = check_layout_args(best);
+        if (!m.empty()) return fail(c, EZLDA_EINVAL, m);
+    }
+    double best_ms = 0.0;
+    if (int r = trial(best, &best_ms)) return r;
+    // coordinate descent, one argument at a time, in order of expected effect
+    enum { kHead, kRun, kTiles, kBlock, kSeg, kGrid };
+    for (int which : {kHead, kRun, kTiles, kBlock, kSeg, kGrid}) {
+        std::vector<ezlda_args> cand;
+        ezlda_args a = best;
+        switch (which) {
+        case kHead:  // dense head: a fraction of the vocabulary by rank
+            for (double f : {0.0025, 0.005, 0.01, 0.02, 0.04}) {
+                a.dense_words = std::max(1, (int)std::ceil(f * V));
+                cand.push_back(a);
+            }
+            break;
+        case kRun:
+            for (int rc : {1, 2, 4, 8, 16, 32}) { a.run_cap = rc; cand.push_back(a); }
+            break;
+        case kTiles:
+            for (int t : {0, -1}) { a.doc_tiles = t; cand.push_back(a); }
+            break;
+        case kBlock:
+            for (int b : {128, 256, 512, 1024}) { a.block_dim = b; cand.push_back(a); }
+            break;
+        case kSeg:
+            for (int g : {1024, 4096, 16384}) { a.token_segment = g; cand.push_back(a); }
+            break;
+        case kGrid:
+            for (int g : {0, 2 * ncu, 4 * ncu, 8 * ncu}) { a.grid_dim = g; cand.push_back(a); }
+            break;
+        }
+        for (const ezlda_args &x : cand) {
+            if (std::memcmp(&x, &best, sizeof(x)) == 0) continue;
+            double ms = 0.0;
+            if (int r = trial(x, &ms)) return r;
+            if (ms < best_ms) {
+                best_ms = ms;
+                best = x;
+            }
+        }
+    }
+    c->a = best;
+    if (int r = build_layout(c, tok_doc, doc_wrd, dord)) return r;
+    if (chosen) *chosen = best;
+    if (ms_per_sweep) *ms_per_sweep = best_ms;
+    if (num_trials) *num_trials = nt;
+    return EZLDA_OK;
+}
+
 int ezlda_time_sweeps(ezlda_ctx *c, int32_t n, int32_t use_graph, double *total_ms, double *stage_ms) {
     if (int r = need_loaded(c)) return r;
     if (n < 1) return fail(c, EZLDA_EINVAL, "num_sweeps must be >= 1");

@@ -31,7 +31,7 @@ EXPORTS = (
     "ezlda_read_chunked", "ezlda_selftest_philox", "ezlda_selftest_copy", "ezlda_get_stats",
     "ezlda_reset_stats", "ezlda_server_create", "ezlda_server_destroy", "ezlda_server_last_error",
     "ezlda_server_submit", "ezlda_server_wait", "ezlda_server_drain", "ezlda_server_get_stats",
-    "ezlda_server_reset_stats", "ezlda_foldin_plain",
+    "ezlda_server_reset_stats", "ezlda_foldin_plain", "ezlda_set_layout", "ezlda_autotune",
 )
 
 
@@ -55,6 +55,10 @@ class Info(C.Structure):
                 ("wt_tail_capacity", C.c_int64), ("num_sample_items", C.c_int32), ("sweep", C.c_uint32),
                 ("device_bytes", C.c_int64), ("doc_tiles", C.c_int32), ("num_xcd", C.c_int32),
                 ("run_cap", C.c_int32), ("num_runs", C.c_int64)]
+
+
+class TuneTrial(C.Structure):
+    _fields_ = [("args", Args), ("ms_per_sweep", C.c_double)]
 
 
 class ServerArgs(C.Structure):
@@ -132,6 +136,9 @@ def lib() -> C.CDLL:
         "ezlda_selftest_copy": (C.c_int, [C.c_int64, C.c_int32, C.POINTER(C.c_double)]),
         "ezlda_get_stats": (C.c_int, [P, P]),
         "ezlda_reset_stats": (C.c_int, [P]),
+        "ezlda_set_layout": (C.c_int, [P, C.POINTER(Args)]),
+        "ezlda_autotune": (C.c_int, [P, C.c_int32, C.POINTER(Args), C.POINTER(C.c_double), P, C.c_int32,
+                                     C.POINTER(C.c_int32)]),
         "ezlda_server_create": (P, [P, C.POINTER(ServerArgs)]),
         "ezlda_server_destroy": (None, [P]),
         "ezlda_server_last_error": (C.c_char_p, [P]),
@@ -321,6 +328,32 @@ class Context:
 
     def reset_stats(self):
         self._chk(self._L.ezlda_reset_stats(self._h))
+
+    # layout tuning (topics never depend on it)
+    def set_layout(self, **kw):
+        """Rebuild the layout with the given block_dim, grid_dim, token_segment,
+        dense_words, dense_threshold, doc_tiles or run_cap."""
+        a = Args.from_buffer_copy(self.args)
+        for k, v in kw.items():
+            if not hasattr(a, k):
+                raise TypeError(f"unknown ezlda argument {k}")
+            setattr(a, k, v)
+        self._chk(self._L.ezlda_set_layout(self._h, C.byref(a)))
+        self.args = a
+
+    def autotune(self, sweeps_per_trial: int = 3, max_trials: int = 64):
+        """Coordinate-descent layout tuning: (chosen Args, ms per sweep, trials)."""
+        best = Args()
+        ms = C.c_double()
+        tr = (TuneTrial * max_trials)()
+        nt = C.c_int32()
+        self._chk(self._L.ezlda_autotune(self._h, sweeps_per_trial, C.byref(best), C.byref(ms), tr, max_trials,
+                                         C.byref(nt)))
+        self.args = best
+        keys = ("dense_words", "run_cap", "doc_tiles", "block_dim", "token_segment", "grid_dim")
+        trials = [dict({k: getattr(tr[i].args, k) for k in keys}, ms_per_sweep=tr[i].ms_per_sweep)
+                  for i in range(min(nt.value, max_trials))]
+        return best, ms.value, trials
 
     # state
     def _n(self) -> int:

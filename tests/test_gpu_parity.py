@@ -210,3 +210,36 @@ def test_bad_inputs_rejected():
         ctx.load(ezlda.Corpus(5, np.array([0, 70000], np.int64), np.zeros(70000, np.int32)))
     with pytest.raises(ezlda.EzldaError, match="no corpus"):
         ctx.sweep(1)
+
+
+# ----------------------------------------------------------- layout tuning
+def test_set_layout_between_sweeps_keeps_topics():
+    off, words = make_corpus(6, 300, 400, 70, zipf=1.2)
+    ctx, orc = _pair(off, words, 400, 64)
+    ctx.sweep(2)
+    ctx.set_layout(run_cap=1, dense_words=0, grid_dim=50)
+    ctx.sweep(2)
+    orc.sweep(4)
+    assert np.array_equal(ctx.get_topics(), orc.get_topics())
+    ctx.set_layout(dense_words=10**6, doc_tiles=-1, block_dim=512, run_cap=32, grid_dim=0, token_segment=100)
+    ctx.sweep_graph(2)  # the layout change dropped the captured sweep
+    orc.sweep(2)
+    assert np.array_equal(ctx.get_topics(), orc.get_topics())
+    assert ctx.info().num_head_words == 400
+    with pytest.raises(ezlda.EzldaError):
+        ctx.set_layout(run_cap=33)
+
+
+def test_autotune_sweeps_are_ordinary_sweeps():
+    off, words = make_corpus(7, 400, 500, 80, zipf=1.2)
+    ctx, orc = _pair(off, words, 500, 48)
+    best, ms, trials = ctx.autotune(sweeps_per_trial=1)
+    assert ms > 0 and len(trials) >= 10
+    assert min(t["ms_per_sweep"] for t in trials) == pytest.approx(ms)
+    n = ctx.info().sweep
+    assert n == 2 * len(trials)
+    orc.sweep(n)
+    assert np.array_equal(ctx.get_topics(), orc.get_topics())
+    ctx.sweep(1)
+    orc.sweep(1)
+    assert np.array_equal(ctx.get_topics(), orc.get_topics())
