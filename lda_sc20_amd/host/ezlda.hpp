@@ -15,6 +15,7 @@
 // throw ezlda::Error carrying the library's status and message.
 #pragma once
 
+#include <algorithm>
 #include <cstdint>
 #include <stdexcept>
 #include <string>
@@ -116,12 +117,61 @@ public:
         check(ezlda_get_topics(h_, z.data()), h_);
         return z;
     }
+    // Layout arguments only (block_dim, grid_dim, token_segment, dense_words,
+    // dense_threshold, doc_tiles, run_cap); topics do not depend on them.
+    void set_layout(const ezlda_args &a) { check(ezlda_set_layout(h_, &a), h_); }
+    std::vector<ezlda_tune_trial> autotune(int sweeps_per_trial, ezlda_args *chosen, double *ms_per_sweep) {
+        std::vector<ezlda_tune_trial> t(64);
+        int32_t n = 0;
+        check(ezlda_autotune(h_, sweeps_per_trial, chosen, ms_per_sweep, t.data(), (int32_t)t.size(), &n), h_);
+        t.resize(std::min<size_t>((size_t)n, t.size()));
+        return t;
+    }
     void save_checkpoint(const std::string &p) { check(ezlda_save_checkpoint(h_, p.c_str()), h_); }
     void load_checkpoint(const std::string &p) { check(ezlda_load_checkpoint(h_, p.c_str()), h_); }
     ezlda_ctx *handle() { return h_; }
 
 private:
     ezlda_ctx *h_;
+};
+
+// Fold-in server over a trained context (the model is frozen at creation).
+class Server {
+public:
+    Server(Context &model, const ezlda_server_args &a) : h_(ezlda_server_create(model.handle(), &a)) {
+        if (!h_) throw Error(EZLDA_EHIP, ezlda_last_error(nullptr));
+    }
+    Server(const Server &) = delete;
+    Server &operator=(const Server &) = delete;
+    ~Server() { ezlda_server_destroy(h_); }
+
+    void submit(uint32_t rid, const std::vector<int32_t> &words) {
+        chk(ezlda_server_submit(h_, rid, words.data(), (int32_t)words.size()));
+    }
+    // Topics and topic counts of a finished request; latency in ms.
+    double wait(uint32_t rid, std::vector<uint16_t> *z, std::vector<int32_t> *counts, int32_t timeout_ms = -1) {
+        double lat = 0.0;
+        chk(ezlda_server_wait(h_, rid, z ? z->data() : nullptr, counts ? counts->data() : nullptr, &lat, timeout_ms));
+        return lat;
+    }
+    void drain() { chk(ezlda_server_drain(h_)); }
+    ezlda_server_stats stats() {
+        ezlda_server_stats st;
+        chk(ezlda_server_get_stats(h_, &st));
+        return st;
+    }
+    void reset_stats() { chk(ezlda_server_reset_stats(h_)); }
+    void plain(uint32_t rid, const std::vector<int32_t> &words, std::vector<uint16_t> *z,
+               std::vector<int32_t> *counts) {
+        chk(ezlda_foldin_plain(h_, rid, words.data(), (int32_t)words.size(), z ? z->data() : nullptr,
+                               counts ? counts->data() : nullptr));
+    }
+
+private:
+    void chk(int rc) {
+        if (rc != EZLDA_OK) throw Error(rc, ezlda_server_last_error(h_));
+    }
+    ezlda_server *h_;
 };
 
 // The reference's launcher names (see the mapping above).

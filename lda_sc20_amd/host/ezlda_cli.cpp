@@ -12,6 +12,7 @@
 // line to DIR/SamplingDRecord.txt (mainKernel.cu:59,376) and prints the
 // perplexity.  `preprocess` replaces preprocess/streamTestBalance.cpp, whose
 // input and output paths are hard-coded (:90,:176,:334).
+#include <algorithm>
 #include <chrono>
 #include <cmath>
 #include <cstdio>
@@ -44,11 +45,14 @@ struct Opts {
 };
 
 const char *kUsage =
-    "usage: ezlda <train|gen|preprocess|bench> [options]\n"
+    "usage: ezlda <train|foldin|gen|preprocess|bench> [options]\n"
     "  train      --prefix DIR [--chunks N] [--iters N] [--K K] [--alpha A] [--beta B]\n"
     "             [--block-dim N] [--grid-dim N] [--streams N] [--token-segment N] [--seed S]\n"
-    "             [--dense-threshold N] [--dense-words N] [--doc-tiles N] [--plain] [--graph]\n"
-    "             [--ll-every N] [--checkpoint FILE] [--resume FILE] [--record FILE]\n"
+    "             [--dense-threshold N] [--dense-words N] [--doc-tiles N] [--run-cap N] [--plain]\n"
+    "             [--graph] [--autotune] [--ll-every N] [--checkpoint FILE] [--resume FILE] [--record FILE]\n"
+    "  foldin     train options, plus --requests DOCWORD [--foldin-sweeps 20] [--slots N]\n"
+    "             [--budget-mb N] [--out FILE]: folds the documents of a docword file into the\n"
+    "             trained model and writes each one's topic counts\n"
     "  gen        --out DIR --V N --D N --K-true N [--K K] [--chunks N] [--len poisson|lognormal|pareto]\n"
     "             [--len-mean X] [--len-sigma X] [--pareto-shape X] [--pareto-xm X] [--len-min N]\n"
     "             [--len-max N] [--zipf S] [--phi-conc C] [--theta-conc C] [--seed S]\n"
@@ -81,6 +85,7 @@ ezlda_args args_from(const Opts &o) {
     a.dense_threshold = (int32_t)o.integer("dense-threshold", a.dense_threshold);
     a.dense_words = (int32_t)o.integer("dense-words", a.dense_words);
     a.doc_tiles = (int32_t)o.integer("doc-tiles", a.doc_tiles);
+    a.run_cap = (int32_t)o.integer("run-cap", a.run_cap);
     a.shortcut = o.has("plain") ? 0 : 1;
     return a;
 }
@@ -107,13 +112,14 @@ ezlda_gen_params gen_from(const Opts &o) {
     return p;
 }
 
-int cmd_train(const Opts &o) {
+// Trains on --prefix (the body of main(), mainKernel.cu:30-386); the
+// context is left with the trained model.
+int run_training(const Opts &o, ezlda::Context &ctx) {
     if (!o.has("prefix")) throw ezlda::Error(EZLDA_EINVAL, "train needs --prefix DIR");
     const std::string prefix = o.str("prefix");
     const int iters = (int)o.integer("iters", 100);  // mainKernel.cu:51
     const int ll_every = (int)o.integer("ll-every", 0);
     const bool graph = o.has("graph");
-    ezlda::Context ctx(args_from(o));
     auto t0 = std::chrono::steady_clock::now();
     ctx.load_chunked(prefix, (int)o.integer("chunks", 0));
     const ezlda_info inf = ctx.info();
@@ -124,6 +130,19 @@ int cmd_train(const Opts &o) {
     if (o.has("resume")) {
         ctx.load_checkpoint(o.str("resume"));
         std::printf("resumed at sweep %u\n", ctx.info().sweep);
+    }
+    if (o.has("autotune")) {
+        ezlda_args best;
+        double ms = 0.0;
+        const auto tr = ctx.autotune((int)o.integer("tune-sweeps", 3), &best, &ms);
+        for (const auto &t : tr)
+            std::printf("tune: dense_words %d run_cap %d doc_tiles %d block_dim %d token_segment %d grid_dim %d: "
+                        "%.3f ms/sweep\n", t.args.dense_words, t.args.run_cap, t.args.doc_tiles, t.args.block_dim,
+                        t.args.token_segment, t.args.grid_dim, t.ms_per_sweep);
+        std::printf("tuned layout: dense_words %d run_cap %d doc_tiles %d block_dim %d token_segment %d grid_dim %d "
+                    "(%.3f ms/sweep; the %zu trials ran %u sweeps)\n", best.dense_words, best.run_cap,
+                    best.doc_tiles, best.block_dim, best.token_segment, best.grid_dim, ms, tr.size(),
+                    ctx.info().sweep);
     }
     std::ofstream rec(o.str("record", prefix + "/SamplingDRecord.txt"), std::ios::app);
     double total = 0.0;
@@ -141,6 +160,54 @@ int cmd_train(const Opts &o) {
     std::printf("%d sweeps in %.1f ms (%.3e tokens/s); log-likelihood %.6e; perplexity %.4f\n", iters, total,
                 inf.num_tokens * (double)iters / (total / 1e3), ll, std::exp(-ll / (double)inf.num_tokens));
     if (o.has("checkpoint")) ctx.save_checkpoint(o.str("checkpoint"));
+    return 0;
+}
+
+int cmd_train(const Opts &o) {
+    ezlda::Context ctx(args_from(o));
+    return run_training(o, ctx);
+}
+
+// Fold-in of a docword file's documents (word ids as in the training
+// corpus) against the trained model: one line per document, "doc n_d
+// topic:count ...", topics 0-based, by count.
+int cmd_foldin(const Opts &o) {
+    if (!o.has("requests")) throw ezlda::Error(EZLDA_EINVAL, "foldin needs --requests DOCWORD");
+    ezlda::Context ctx(args_from(o));
+    run_training(o, ctx);
+    ezlda::Corpus req = ezlda::Corpus::read_docword(o.str("requests"));
+    const int K = ctx.info().K;
+    ezlda_server_args sa;
+    std::memset(&sa, 0, sizeof sa);
+    sa.max_slots = (int32_t)o.integer("slots", 1024);
+    sa.sweeps = (int32_t)o.integer("foldin-sweeps", 20);
+    sa.hbm_budget_bytes = (int64_t)o.integer("budget-mb", 1024) << 20;
+    sa.seed = (uint64_t)o.integer("seed", 2002) + 7;
+    ezlda::Server srv(ctx, sa);
+    const int D = req.docs();
+    const int64_t *off = req.doc_off();
+    const int32_t *w = req.word_ids();
+    for (int d = 0; d < D; ++d) srv.submit((uint32_t)d, std::vector<int32_t>(w + off[d], w + off[d + 1]));
+    srv.drain();
+    std::FILE *f = o.has("out") ? std::fopen(o.str("out").c_str(), "w") : stdout;
+    if (!f) throw ezlda::Error(EZLDA_EIO, "cannot open " + o.str("out"));
+    std::vector<int32_t> cnt(K);
+    std::vector<uint16_t> z;
+    for (int d = 0; d < D; ++d) {
+        z.assign((size_t)std::max<int64_t>(1, off[d + 1] - off[d]), 0);
+        srv.wait((uint32_t)d, &z, &cnt);
+        std::vector<std::pair<int32_t, int>> nz;
+        for (int k = 0; k < K; ++k)
+            if (cnt[k]) nz.push_back({-cnt[k], k});
+        std::sort(nz.begin(), nz.end());
+        std::fprintf(f, "%d %lld", d, (long long)(off[d + 1] - off[d]));
+        for (auto &x : nz) std::fprintf(f, " %d:%d", x.second, -x.first);
+        std::fputc('\n', f);
+    }
+    if (f != stdout) std::fclose(f);
+    const ezlda_server_stats st = srv.stats();
+    std::fprintf(stderr, "folded in %lld docs: %.1f docs/s, latency p50 %.2f ms p99 %.2f ms, %lld launches\n",
+                 (long long)st.completed, st.docs_per_s, st.p50_ms, st.p99_ms, (long long)st.steps);
     return 0;
 }
 
@@ -203,6 +270,7 @@ int main(int argc, char **argv) {
     try {
         const Opts o = parse(argc, argv, 2);
         if (cmd == "train") return cmd_train(o);
+        if (cmd == "foldin") return cmd_foldin(o);
         if (cmd == "gen") return cmd_gen(o);
         if (cmd == "preprocess") return cmd_preprocess(o);
         if (cmd == "bench") return cmd_bench(o);
