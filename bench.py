@@ -34,6 +34,18 @@ WORKLOADS = {
                  K=1024, seed=2002, head_frac=0.01,
                  desc="V=102660, D=300000, lognormal(mean 330, sigma 0.8), Zipf(1.07) base, ~99M tokens, "
                       "K=1024, alpha=50/K, beta=0.01, dense head = top 1% of words"),
+    # PubMed-sized corpus, fully dense WT (K in {1024, 4096, 10000} via --K)
+    "cfg3": dict(gen=dict(V=141043, D=8200000, K_true=1024, len_dist=1, len_mean=90.0, len_sigma=0.8,
+                          zipf_s=1.07, phi_conc=0.01, theta_conc=0.1, seed=1003),
+                 K=1024, seed=2003, head_frac=1.0,
+                 desc="V=141043, D=8.2M, lognormal(mean 90, sigma 0.8), Zipf(1.07) base, ~738M tokens, "
+                      "alpha=50/K, beta=0.01, dense WT for every word"),
+    # heavy-tailed document lengths: Pareto(1.2) with x_m = 48, clamped to [1, 20000]
+    "cfg4": dict(gen=dict(V=100000, D=2000000, K_true=1024, len_dist=2, pareto_shape=1.2, pareto_xm=48.0,
+                          len_min=1, len_max=20000, zipf_s=1.1, phi_conc=0.01, theta_conc=0.1, seed=1004),
+                 K=2048, seed=2004, head_frac=0.01,
+                 desc="V=100000, D=2M, Pareto(shape 1.2, x_m 48) lengths in [1, 20000], Zipf(1.1) base, "
+                      "~400M tokens, K=2048, alpha=50/K, beta=0.01, dense head = top 1% of words"),
     # the reference's CPU-sized case
     "cfg1": dict(gen=dict(V=10000, D=10000, K_true=64, len_dist=0, len_mean=100.0, len_sigma=0.0,
                           zipf_s=0.0, phi_conc=0.01, theta_conc=0.1, seed=1001),
@@ -219,9 +231,14 @@ def main() -> int:
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-docs", type=int, default=30000)
     ap.add_argument("--no-graph", action="store_true")
+    ap.add_argument("--K", type=int, default=0, help="topics (0 = the workload's)")
+    ap.add_argument("--autotune", action="store_true",
+                    help="time the default layout, then tune the layout (its sweeps count as warm-up) and time "
+                         "the tuned one")
     ap.add_argument("--run-cap", type=int, default=0, help="tokens per shared DT scan (0 = library default)")
-    ap.add_argument("--grid-dim", type=int, default=0, help="persistent sampler workgroups (0 = one per item)")
-    ap.add_argument("--block-dim", type=int, default=256)
+    ap.add_argument("--grid-dim", type=int, default=-1,
+                    help="persistent sampler workgroups (-1 = library default, 0 = one per item)")
+    ap.add_argument("--block-dim", type=int, default=0, help="sampler workgroup size (0 = library default)")
     ap.add_argument("--doc-tiles", type=int, default=0,
                     help="doc tiles for head-word items (0 auto, <0 off); topics do not depend on it")
     ap.add_argument("--no-serve", action="store_true", help="skip the fold-in serving measurement")
@@ -252,7 +269,7 @@ def main() -> int:
     t = time.perf_counter()
     corpus = ezlda.gen_corpus(**wl["gen"])
     log(f"generated {corpus.N} tokens in {time.perf_counter() - t:.1f}s")
-    K = wl["K"]
+    K = args.K or wl["K"]
     dense = -1 if wl["head_frac"] is None else int(math.ceil(wl["head_frac"] * corpus.V))
     ctx = ezlda.Context(K=K, alpha=50.0 / K, beta=0.01, seed=wl["seed"], dense_words=dense, shortcut=1,
                         doc_tiles=args.doc_tiles, run_cap=args.run_cap, grid_dim=args.grid_dim,
@@ -267,8 +284,22 @@ def main() -> int:
     t = time.perf_counter()
     ctx.sweep(args.warmup)
     log(f"{args.warmup} warm-up sweeps in {time.perf_counter() - t:.2f}s")
+    tuning = None
+    if args.autotune:
+        n_cmp = max(5, min(20, args.steps // 4))
+        plain_ms, _ = ctx.time_sweeps(n_cmp, use_graph=False)
+        t = time.perf_counter()
+        best, tuned_ms, trials = ctx.autotune(sweeps_per_trial=3)
+        tuning = {"default_layout_ms_per_sweep": plain_ms / n_cmp, "default_layout_sweeps": n_cmp,
+                  "tuned_ms_per_sweep_in_tuner": tuned_ms, "trials": trials, "tuning_s": time.perf_counter() - t,
+                  "chosen": {k: getattr(best, k) for k in ("dense_words", "run_cap", "doc_tiles", "block_dim",
+                                                           "token_segment", "grid_dim")}}
+        log(f"autotune: default layout {plain_ms / n_cmp:.3f} ms/sweep -> {tuned_ms:.3f} ms/sweep with "
+            f"{tuning['chosen']} ({len(trials)} trials, {tuning['tuning_s']:.1f}s)")
+        info = ctx.info()
 
     device_sync()
+    sweeps_before = int(ctx.info().sweep)
     ctx.reset_stats()
     t0 = time.perf_counter()
     total_ms, stage_ms = ctx.time_sweeps(args.steps, use_graph=False)
@@ -317,9 +348,10 @@ def main() -> int:
         "dtype": "f32",
         "data": "synthetic (LDA generative process, Philox-seeded)",
         "config": {"workload": f"{args.workload}: {wl['desc']}", "num_tokens": N, "K": K,
-                   "num_head_words": VD, "shortcut": True, "sweeps_before_timing": args.warmup,
+                   "num_head_words": VD, "shortcut": True, "sweeps_before_timing": sweeps_before,
                    "doc_tiles": int(info.doc_tiles), "run_cap": int(info.run_cap), "num_runs": int(info.num_runs),
-                   "grid_dim": args.grid_dim, "block_dim": args.block_dim},
+                   "grid_dim": int(info.grid_dim), "block_dim": int(info.block_dim),
+                   "token_segment": int(info.token_segment)},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": measured_traffic(args.workload),
                      "kernel": "k_sample", "bytes_per_launch": b_sample, "avg_launch_ms": t_sample * 1e3,
@@ -332,6 +364,9 @@ def main() -> int:
         "perplexity_after": math.exp(-ll / N),
         "wall_ms_timed_region": wall * 1e3,
     }
+    if tuning is not None:
+        tuning["graph_vs_default_layout"] = (tuning["default_layout_ms_per_sweep"] / graph_ms) if graph_ms else None
+        out["autotune"] = tuning
     if not args.no_serve:
         out["serving"] = serving_bench(ctx, ezlda, wl, args)
     ctx.close()

@@ -42,12 +42,13 @@ typedef struct ezlda_args {
     float alpha;             /* Argument.cuh:9; <= 0 means 50/K */
     float beta;              /* Argument.cuh:8; <= 0 means 0.01 */
     int32_t block_dim;       /* Argument.cuh:4 BlockDim: threads per workgroup
-                                of the sweep kernels; multiple of 64, <= 1024 */
-    int32_t grid_dim;        /* Argument.cuh:5 GridDim: 0 = one workgroup per
-                                sampler item; > 0 = that many persistent
-                                workgroups pulling items from per-XCD queues
-                                (then from the others'); topics do not
-                                depend on it */
+                                of the sampler; multiple of 64, <= 1024
+                                (<= 0 = 128) */
+    int32_t grid_dim;        /* Argument.cuh:5 GridDim: > 0 = that many
+                                persistent sampler workgroups pulling items
+                                from per-XCD queues (then from the others');
+                                0 = one workgroup per item; < 0 = 8 per CU.
+                                Topics do not depend on it */
     int32_t num_chunks;      /* mainKernel.cu:50; recorded, corpus is resident */
     int32_t num_streams;     /* Argument.cuh:7; recorded */
     int32_t token_segment;   /* Doc.cuh:30: max tokens of one word per sampler
@@ -67,7 +68,7 @@ typedef struct ezlda_args {
                                 not depend on it */
     int32_t run_cap;         /* the sampler shares one DT row scan among the
                                 repeats of a word in a document, at most
-                                run_cap tokens (1..32, 0 = 8); 1 = one scan
+                                run_cap tokens (1..32, 0 = 16); 1 = one scan
                                 per token.  Topics do not depend on it */
 } ezlda_args;
 
@@ -85,6 +86,9 @@ typedef struct ezlda_info {
     int32_t doc_tiles;       /* doc tiles used for head-word items */
     int32_t num_xcd;         /* XCDs of the device */
     int32_t run_cap;         /* tokens per run in effect */
+    int32_t grid_dim;        /* sampler workgroups in effect (0 = one per item) */
+    int32_t block_dim;       /* sampler workgroup size in effect */
+    int32_t token_segment;   /* in effect */
     int64_t num_runs;        /* sampler runs (a word's tokens in one doc) */
 } ezlda_info;
 

@@ -44,6 +44,27 @@ void parallel_for(int64_t n, F f) {
     for (auto &x : th) x.join();
 }
 
+// Compute units of the current device (256 on MI355X).
+int device_cus() {
+    int dev = 0, n = 0;
+    if (hipGetDevice(&dev) != hipSuccess || hipDeviceGetAttribute(&n, hipDeviceAttributeMultiprocessorCount, dev) !=
+                                                 hipSuccess || n < 1)
+        n = 256;
+    return n;
+}
+
+// Validates the layout arguments of `a` (those build_layout reads) and
+// fills their defaults; returns an error message or "".
+std::string check_layout_args(ezlda_args &a) {
+    if (a.block_dim <= 0) a.block_dim = 128;
+    if (a.block_dim % 64 != 0 || a.block_dim > 1024) return "block_dim must be a multiple of 64 and <= 1024";
+    if (a.token_segment <= 0) a.token_segment = 4096;
+    if (a.grid_dim < 0) a.grid_dim = 8 * device_cus();  // auto: 8 persistent workgroups per CU
+    if (a.run_cap == 0) a.run_cap = 16;
+    if (a.run_cap < 1 || a.run_cap > 32) return "run_cap must be in [1, 32]";
+    return "";
+}
+
 }  // namespace
 
 namespace ezc {
@@ -337,8 +358,8 @@ void ezlda_default_args(ezlda_args *a) {
     a->K = 1024;             // Argument.cuh:6
     a->alpha = 50.0f / 1024; // Argument.cuh:9
     a->beta = 0.01f;         // Argument.cuh:8
-    a->block_dim = 256;      // Argument.cuh:4
-    a->grid_dim = 0;
+    a->block_dim = 128;      // Argument.cuh:4 has 256; 128 measures faster here
+    a->grid_dim = -1;        // auto: 8 persistent workgroups per CU
     a->num_chunks = 1;
     a->num_streams = 1;
     a->token_segment = 4096;
@@ -347,7 +368,7 @@ void ezlda_default_args(ezlda_args *a) {
     a->dense_threshold = 0;  // K, as preprocess/streamTestBalance.cpp:291
     a->shortcut = 1;
     a->topic_bits = 16;
-    a->run_cap = 8;
+    a->run_cap = 16;
 }
 
 const char *ezlda_last_error(const ezlda_ctx *ctx) { return ctx ? ctx->err.c_str() : g_err.c_str(); }
@@ -364,27 +385,22 @@ ezlda_ctx *ezlda_create(const ezlda_args *args) {
     }
     if (a.alpha <= 0.f) a.alpha = 50.0f / (float)a.K;
     if (a.beta <= 0.f) a.beta = 0.01f;
-    if (a.block_dim <= 0) a.block_dim = 256;
-    if (a.block_dim % 64 != 0 || a.block_dim > 1024) {
-        fail(nullptr, EZLDA_EINVAL, "block_dim must be a multiple of 64 and <= 1024");
-        return nullptr;
-    }
-    if (a.token_segment <= 0) a.token_segment = 4096;
     if (a.topic_bits == 0) a.topic_bits = 16;
     if (a.topic_bits != 16) {
         fail(nullptr, EZLDA_EINVAL, "only 16-bit topics are supported");
-        return nullptr;
-    }
-    if (a.grid_dim < 0) a.grid_dim = 0;
-    if (a.run_cap == 0) a.run_cap = 8;
-    if (a.run_cap < 1 || a.run_cap > 32) {
-        fail(nullptr, EZLDA_EINVAL, "run_cap must be in [1, 32]");
         return nullptr;
     }
     int ndev = 0;
     if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) {
         fail(nullptr, EZLDA_EHIP, "no HIP device");
         return nullptr;
+    }
+    {
+        const std::string m = check_layout_args(a);
+        if (!m.empty()) {
+            fail(nullptr, EZLDA_EINVAL, m);
+            return nullptr;
+        }
     }
     {
         int dev = 0, lmax = 0;
@@ -604,6 +620,9 @@ int ezlda_get_info(ezlda_ctx *c, ezlda_info *info) {
     info->doc_tiles = c->doc_tiles;
     info->num_xcd = c->n_xcd;
     info->run_cap = c->a.run_cap;
+    info->grid_dim = c->a.grid_dim;
+    info->block_dim = c->a.block_dim;
+    info->token_segment = c->a.token_segment;
     info->num_runs = c->s.n_runs;
     return EZLDA_OK;
 }
@@ -700,18 +719,6 @@ int ezlda_sweep_graph(ezlda_ctx *c, int32_t n) {
 // Graph: one event pair around n replays (stage times are then -1).
 namespace {
 
-// Validates the layout arguments of `a` (those build_layout reads) and
-// fills their defaults; returns an error message or "".
-std::string check_layout_args(ezlda_args &a) {
-    if (a.block_dim <= 0) a.block_dim = 256;
-    if (a.block_dim % 64 != 0 || a.block_dim > 1024) return "block_dim must be a multiple of 64 and <= 1024";
-    if (a.token_segment <= 0) a.token_segment = 4096;
-    if (a.grid_dim < 0) a.grid_dim = 0;
-    if (a.run_cap == 0) a.run_cap = 8;
-    if (a.run_cap < 1 || a.run_cap > 32) return "run_cap must be in [1, 32]";
-    return "";
-}
-
 // Host copies of the corpus arrays the layout is built from.
 int layout_inputs(ezlda_ctx *c, std::vector<uint32_t> &tok_doc, std::vector<uint32_t> &doc_wrd,
                   std::vector<int32_t> &dord) {
@@ -807,13 +814,13 @@ int ezlda_autotune(ezlda_ctx *c, int32_t sweeps_per_trial, ezlda_args *chosen, d
             for (int t : {0, -1}) { a.doc_tiles = t; cand.push_back(a); }
             break;
         case kBlock:
-            for (int b : {128, 256, 512, 1024}) { a.block_dim = b; cand.push_back(a); }
+            for (int b : {64, 128, 256, 512}) { a.block_dim = b; cand.push_back(a); }
             break;
         case kSeg:
             for (int g : {1024, 4096, 16384}) { a.token_segment = g; cand.push_back(a); }
             break;
         case kGrid:
-            for (int g : {0, 2 * ncu, 4 * ncu, 8 * ncu}) { a.grid_dim = g; cand.push_back(a); }
+            for (int g : {0, 4 * ncu, 8 * ncu, 16 * ncu}) { a.grid_dim = g; cand.push_back(a); }
             break;
         }
         for (const ezlda_args &x : cand) {

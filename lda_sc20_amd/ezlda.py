@@ -54,7 +54,8 @@ class Info(C.Structure):
                 ("num_head_words", C.c_int32), ("K", C.c_int32), ("dt_capacity", C.c_int64),
                 ("wt_tail_capacity", C.c_int64), ("num_sample_items", C.c_int32), ("sweep", C.c_uint32),
                 ("device_bytes", C.c_int64), ("doc_tiles", C.c_int32), ("num_xcd", C.c_int32),
-                ("run_cap", C.c_int32), ("num_runs", C.c_int64)]
+                ("run_cap", C.c_int32), ("grid_dim", C.c_int32), ("block_dim", C.c_int32),
+                ("token_segment", C.c_int32), ("num_runs", C.c_int64)]
 
 
 class TuneTrial(C.Structure):

@@ -88,6 +88,7 @@ CASES = [
     (64, 400, 300, 60, {"run_cap": 32, "grid_dim": 37}),
     (1024, 600, 150, 300, {"dense_threshold": 200, "run_cap": 3, "grid_dim": 300, "doc_tiles": 16}),
     (100, 500, 200, 80, {"grid_dim": 1, "doc_tiles": -1}),
+    (100, 500, 200, 80, {"grid_dim": 0, "block_dim": 256}),
     # large K: the per-block LDS of the sampler, the histograms and the
     # log-likelihood pass 64 KiB (the topic count of the large-K corpus, and
     # the largest K the 16-bit DT entries allow)
