@@ -14,17 +14,17 @@ struct DevState {
     uint32_t N = 0;
     float alpha = 0.f, beta = 0.f, vbeta = 0.f;
     uint64_t seed = 0;
-    // tokens, word-major
+    // tokens, word-major (token id = word rank, doc order, position)
     uint32_t *word_off = nullptr;   // V+1
-    uint32_t *tok_doc = nullptr;    // N
     uint16_t *z = nullptr;          // N
-    uint32_t *aux = nullptr;        // N: n_dk1 | n_dk2 << 16
-    // tokens, doc-major
-    uint32_t *doc_off = nullptr;    // D+1
-    uint32_t *doc_tok = nullptr;    // N: doc-major position -> token id
-    uint32_t *doc_wrd = nullptr;    // N: doc-major position -> word
+    // tokens by doc slot: doc q (docs by descending length) owns slots
+    // [soff[q], soff[q+1]), its tokens grouped by word rank, then position
+    uint32_t *soff = nullptr;       // D+1
+    uint32_t *slot_wrd = nullptr;   // N: word rank
+    uint16_t *z_doc = nullptr;      // N: z by slot, written with z by the sampler
+    uint32_t *aux_doc = nullptr;    // N: n_dk1 | n_dk2 << 16 of the slot's word
     // doc-topic rows
-    uint2 *doc_info = nullptr;      // D: x = DT row offset, y = nnz | len << 16
+    uint2 *doc_info = nullptr;      // D, by doc q: x = DT row offset, y = nnz | len << 16
     uint32_t *dt = nullptr;         // sum_d min(n_d,K): topic | count << 16
     // word-topic: dense head rows + sorted sparse tail rows
     int32_t *wt_head = nullptr;     // VD*K
@@ -39,14 +39,14 @@ struct DevState {
     int32_t n_wt_segs = 0;
     uint4 *smp_segs = nullptr;      // sampler items (w, first run, end run, 0)
     int32_t n_smp_segs = 0;
-    uint2 *runs = nullptr;          // (first token, tokens): a word's tokens in one doc
+    uint4 *runs = nullptr;          // (first token, first slot, doc q, tokens): a word's tokens in one doc
     int64_t n_runs = 0;
     int32_t n_queues = 1;           // persistent sampler: item queues (<= 8)
     uint32_t *counters = nullptr;   // [0] sweep, [8 .. 8 + n_queues) item queues
     unsigned long long *stats = nullptr;  // [0] full draws, [1] DT entries scanned
     double *ll_part = nullptr;      // n_smp_segs
     double *ll_out = nullptr;       // 1
-    int32_t block_dim = 256;
+    int32_t block_dim = 128;
     int32_t grid_dim = 0;           // 0 = one workgroup per sampler item
 };
 

@@ -26,6 +26,9 @@ struct ezlda_ctx {
     std::vector<int32_t> rank_of;    // input word -> rank
     std::vector<uint32_t> word_off;  // V+1 by rank
     std::vector<uint32_t> doc_off;   // D+1
+    std::vector<int32_t> dord;       // doc order q -> input doc (length desc, id asc)
+    std::vector<uint32_t> soff;      // D+1: slots of doc q
+    std::vector<uint32_t> slot_tok;  // N: slot -> token id
     hipGraph_t graph = nullptr;
     hipGraphExec_t graph_exec = nullptr;
     int n_xcd = 1;      // XCDs of the device (round-robin workgroup dispatch)

@@ -243,33 +243,35 @@ __global__ void k_word_top(int V, int VD, int K, float beta, const int32_t *__re
 }
 
 // --------------------------------------------------- DT rows and aux
-// DT_Update_Kernel (utility.cu:79-284): one wavefront per doc gathers the
-// topics of its tokens into a 16-bit LDS histogram, writes each token's
-// (n_dk1, n_dk2) at its word's top-2 topics (the re-pack of :125-161), and
-// compacts the row in ascending topic order (:183-238).  A row starts on a
-// 128-byte boundary and is zero-padded to a multiple of 32 entries.
-__global__ void k_dt_build(int D, int K, const uint32_t *__restrict__ doc_off, const uint32_t *__restrict__ doc_tok,
-                           const uint32_t *__restrict__ doc_wrd, const uint16_t *__restrict__ z,
-                           const ushort4 *__restrict__ wtop, uint32_t *__restrict__ aux,
-                           uint2 *__restrict__ doc_info, uint32_t *__restrict__ dt) {
+// DT_Update_Kernel (utility.cu:79-284), on the doc-major copy of z: doc q
+// (docs in descending length order) owns slots [soff[q], soff[q+1]), its
+// tokens grouped by word, and the sampler keeps z_doc in step with z, so
+// every read and write here is a coalesced stream.  One wavefront per doc
+// builds a 16-bit LDS histogram, writes each slot's (n_dk1, n_dk2) at its
+// word's top-2 topics (the re-pack of :125-161), and compacts the row in
+// ascending topic order (:183-238).  A row starts on a 128-byte boundary and
+// is zero-padded to a multiple of 32 entries.
+__global__ void k_dt_build(int D, int K, const uint32_t *__restrict__ soff, const uint32_t *__restrict__ slot_wrd,
+                           const uint16_t *__restrict__ z_doc, const ushort4 *__restrict__ wtop,
+                           uint32_t *__restrict__ aux_doc, uint2 *__restrict__ doc_info, uint32_t *__restrict__ dt) {
     extern __shared__ uint32_t sh[];
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6, nwv = blockDim.x >> 6;
     const int KH = (K + 1) >> 1;
     uint32_t *h = sh + wv * KH;
     for (int k = lane; k < KH; k += 64) h[k] = 0;
     wave_lds_sync();
-    for (int d = blockIdx.x * nwv + wv; d < D; d += gridDim.x * nwv) {
-        const uint32_t b = doc_off[d], e = doc_off[d + 1];
-        for (uint32_t i = b + lane; i < e; i += 64) h16_add(h, z[doc_tok[i]]);
+    for (int q = blockIdx.x * nwv + wv; q < D; q += gridDim.x * nwv) {
+        const uint32_t b = soff[q], e = soff[q + 1];
+        for (uint32_t i = b + lane; i < e; i += 64) h16_add(h, z_doc[i]);
         wave_lds_sync();
         for (uint32_t i = b + lane; i < e; i += 64) {
-            const ushort4 tp = wtop[doc_wrd[i]];
+            const ushort4 tp = wtop[slot_wrd[i]];
             const uint32_t n1 = h16_get(h, tp.x);
             const uint32_t n2 = tp.y != kNone ? h16_get(h, tp.y) : 0u;
-            aux[doc_tok[i]] = n1 | (n2 << 16);
+            aux_doc[i] = n1 | (n2 << 16);
         }
         wave_lds_sync();
-        const uint32_t base = doc_info[d].x;
+        const uint32_t base = doc_info[q].x;
         uint32_t nz = 0;
         if (e > b) {
             for (int j = 0; j < K; j += 64) {
@@ -286,7 +288,7 @@ __global__ void k_dt_build(int D, int K, const uint32_t *__restrict__ doc_off, c
             // reads rows in 128-byte blocks and a zero entry adds nothing
             if ((uint32_t)lane < ((32u - (nz & 31u)) & 31u)) dt[base + nz + lane] = 0u;
         }
-        if (lane == 0) doc_info[d].y = nz | ((e - b) << 16);
+        if (lane == 0) doc_info[q].y = nz | ((e - b) << 16);
     }
 }
 
@@ -328,7 +330,7 @@ __device__ __forceinline__ float block_sum(const uint4 *__restrict__ p, const fl
 // tail words, which differ only in how phi_w is staged.
 struct SampleArgs {
     const uint4 *segs;
-    const uint2 *runs;
+    const uint4 *runs;  // (first token, first doc slot, doc, tokens)
     uint32_t n_items;
     int VD, K;
     float alpha, beta;
@@ -336,10 +338,10 @@ struct SampleArgs {
     const uint32_t *tail_off, *wt_tail, *tail_nnz;
     const float *inv;
     const ushort4 *wtop;
-    const uint32_t *tok_doc, *aux;
+    const uint32_t *aux_doc;  // by doc slot: n_dk1 | n_dk2 << 16
     const uint2 *doc_info;
     const uint32_t *dt;
-    uint16_t *z;
+    uint16_t *z, *z_doc;      // by token (word-major) and by doc slot
     const uint32_t *counters;
     uint32_t *queue;  // persistent mode: one item counter per queue
     uint32_t n_queues;
@@ -409,11 +411,10 @@ __device__ __forceinline__ void sample_item(const SampleArgs &A, uint32_t item, 
     for (uint32_t base = sg.y + (uint32_t)wv * 64u; base < sg.z; base += (uint32_t)nwv * 64u) {
         const uint32_t r = base + (uint32_t)lane;
         if (r >= sg.z) break;
-        const uint2 rn = A.runs[r];
-        const uint32_t t0 = rn.x, len = rn.y;
-        const uint32_t d = A.tok_doc[t0];
-        const uint32_t a = A.aux[t0];
-        const uint2 di = A.doc_info[d];
+        const uint4 rn = A.runs[r];
+        const uint32_t t0 = rn.x, p0 = rn.y, len = rn.w;
+        const uint32_t a = A.aux_doc[p0];
+        const uint2 di = A.doc_info[rn.z];
         const uint32_t n1 = a & 0xFFFFu, n2 = a >> 16, nd = di.y >> 16;
         const float P1 = ((float)n1 + alpha) * f1;
         // tokens of the run that need the full draw; shortcut tokens take k1
@@ -426,8 +427,12 @@ __device__ __forceinline__ void sample_item(const SampleArgs &A, uint32_t item, 
                 const float B = (P1 + maxS) + Q;
                 full = !(u * B < P1 * kMargin);
             }
-            if (full) need |= 1u << j;
-            else A.z[t0 + j] = (uint16_t)k1;
+            if (full) {
+                need |= 1u << j;
+            } else {
+                A.z[t0 + j] = (uint16_t)k1;
+                A.z_doc[p0 + j] = (uint16_t)k1;
+            }
         }
         if (!need) continue;
         const uint4 *rp = reinterpret_cast<const uint4 *>(A.dt + di.x);
@@ -530,6 +535,7 @@ __device__ __forceinline__ void sample_item(const SampleArgs &A, uint32_t item, 
                 }
             }
             A.z[t] = (uint16_t)newz;
+            A.z_doc[p0 + j] = (uint16_t)newz;
         }
     }
 }
@@ -579,13 +585,12 @@ __global__ void __launch_bounds__(1024) k_sample(const SampleArgs A) {
 // per token, the term of profilePerp/utility.cu:2398,2810 reduced in FP64
 // (profilePerp/utility.cu:2858-2928).  One workgroup per sampler item; the
 // per-item partials are summed in fixed order by k_ll_final.
-__global__ void __launch_bounds__(1024) k_ll(const uint4 *__restrict__ segs, const uint2 *__restrict__ runs, int VD,
+__global__ void __launch_bounds__(1024) k_ll(const uint4 *__restrict__ segs, const uint4 *__restrict__ runs, int VD,
                                              int K, double alpha, double beta,
                                              double vbeta, const int32_t *__restrict__ wt_head,
                                              const uint32_t *__restrict__ tail_off,
                                              const uint32_t *__restrict__ wt_tail,
                                              const uint32_t *__restrict__ tail_nnz, const int32_t *__restrict__ nk,
-                                             const uint32_t *__restrict__ tok_doc,
                                              const uint2 *__restrict__ doc_info, const uint32_t *__restrict__ dt,
                                              double *__restrict__ part) {
     extern __shared__ double smd[];
@@ -624,8 +629,8 @@ __global__ void __launch_bounds__(1024) k_ll(const uint4 *__restrict__ segs, con
     __syncthreads();
     double acc = 0.0;
     for (uint32_t r = sg.y + (uint32_t)wv; r < sg.z; r += (uint32_t)nwv) {
-        const uint2 rn = runs[r];  // rn.y tokens of this word in one doc share the term
-        const uint2 di = doc_info[tok_doc[rn.x]];
+        const uint4 rn = runs[r];  // rn.w tokens of this word in doc rn.z share the term
+        const uint2 di = doc_info[rn.z];
         const uint32_t nnz = di.y & 0xFFFFu;
         double s = 0.0;
         for (uint32_t i = lane; i < nnz; i += 64) {
@@ -633,7 +638,7 @@ __global__ void __launch_bounds__(1024) k_ll(const uint4 *__restrict__ segs, con
             s += (double)(e >> 16) * phi[dt_topic(e)];
         }
         for (int off = 32; off >= 1; off >>= 1) s += __shfl_xor(s, off);
-        acc += (double)rn.y * log((s + q) / ((double)(di.y >> 16) + (double)K * alpha));
+        acc += (double)rn.w * log((s + q) / ((double)(di.y >> 16) + (double)K * alpha));
     }
     if (lane == 0) red[wv] = acc;
     __syncthreads();
@@ -932,7 +937,7 @@ hipError_t launch_build_dt(const DevState &s, hipStream_t st) {
     if (s.D == 0) return hipSuccess;
     const int nwv = 4;
     hipLaunchKernelGGL(k_dt_build, dim3(grid_for(s.D, nwv, 1 << 18)), dim3(64 * nwv), nwv * wave_hist_bytes(s.K), st,
-                       s.D, s.K, s.doc_off, s.doc_tok, s.doc_wrd, s.z, s.wtop, s.aux, s.doc_info, s.dt);
+                       s.D, s.K, s.soff, s.slot_wrd, s.z_doc, s.wtop, s.aux_doc, s.doc_info, s.dt);
     return hipGetLastError();
 }
 
@@ -969,11 +974,11 @@ hipError_t launch_sample(const DevState &s, bool shortcut, hipStream_t st) {
     A.tail_nnz = s.tail_nnz;
     A.inv = s.inv;
     A.wtop = s.wtop;
-    A.tok_doc = s.tok_doc;
-    A.aux = s.aux;
+    A.aux_doc = s.aux_doc;
     A.doc_info = s.doc_info;
     A.dt = s.dt;
     A.z = s.z;
+    A.z_doc = s.z_doc;
     A.counters = s.counters;
     A.queue = s.counters + 8;  // counters[8 .. 8 + n_queues)
     A.n_queues = s.grid_dim > 0 ? (uint32_t)s.n_queues : 0u;
@@ -1007,7 +1012,7 @@ hipError_t launch_loglik(const DevState &s, hipStream_t st) {
     hipLaunchKernelGGL(k_ll, dim3(s.n_smp_segs), dim3(bd), (size_t)s.K * sizeof(double), st, s.smp_segs, s.runs, s.VD,
                        s.K,
                        (double)s.alpha, (double)s.beta, (double)s.V * (double)s.beta, s.wt_head, s.tail_off,
-                       s.wt_tail, s.tail_nnz, s.nk, s.tok_doc, s.doc_info, s.dt, s.ll_part);
+                       s.wt_tail, s.tail_nnz, s.nk, s.doc_info, s.dt, s.ll_part);
     hipLaunchKernelGGL(k_ll_final, dim3(1), dim3(256), 0, st, s.ll_part, s.n_smp_segs, s.ll_out);
     return hipGetLastError();
 }

@@ -156,12 +156,12 @@ int need_loaded(ezlda_ctx *c) {
 // item per word.  Items longer than token_segment are split.  The order of
 // items changes which workgroup draws a token, never the draw.
 std::vector<uint4> build_sample_items(const ezlda_args &a, const std::vector<uint32_t> &word_off,
-                                      const std::vector<uint32_t> &tok_rank, const std::vector<uint32_t> &doc_wrd,
-                                      const std::vector<uint32_t> &doc_off, const std::vector<int32_t> &dord, int VD,
-                                      int K, int n_xcd, int *tiles_used) {
+                                      const std::vector<uint32_t> &tok_rank, const std::vector<uint32_t> &soff,
+                                      const std::vector<uint32_t> &slot_wrd, int VD, int K, int n_xcd,
+                                      int *tiles_used) {
     const uint32_t SEG = (uint32_t)a.token_segment;
     const int V = (int)word_off.size() - 1;
-    const int D = (int)doc_off.size() - 1;
+    const int D = (int)soff.size() - 1;
     std::vector<uint4> out;
     auto push_split = [&](std::vector<uint4> &v, uint32_t w, uint32_t b, uint32_t e) {
         for (uint32_t x = b; x < e; x += SEG) v.push_back(make_uint4(w, x, std::min(x + SEG, e), 0u));
@@ -171,7 +171,7 @@ std::vector<uint4> build_sample_items(const ezlda_args &a, const std::vector<uin
     if (a.doc_tiles >= 0 && VD > 0 && D > 0 && n_xcd > 0 && head_tokens > 0) {
         // DT bytes of the whole corpus -> tiles of ~2 MiB
         uint64_t dt_bytes = 0;
-        for (int d = 0; d < D; ++d) dt_bytes += 4ull * std::min<uint32_t>(doc_off[d + 1] - doc_off[d], (uint32_t)K);
+        for (int q = 0; q < D; ++q) dt_bytes += 4ull * std::min<uint32_t>(soff[q + 1] - soff[q], (uint32_t)K);
         if (a.doc_tiles > 0) {
             T = a.doc_tiles;
         } else {
@@ -184,9 +184,8 @@ std::vector<uint4> build_sample_items(const ezlda_args &a, const std::vector<uin
         // tile boundaries: equal head-token work per tile
         std::vector<uint64_t> hw(D + 1, 0);  // head tokens of the first q docs in doc order
         for (int q = 0; q < D; ++q) {
-            const int d = dord[q];
             uint64_t h = 0;
-            for (uint32_t i = doc_off[d]; i < doc_off[d + 1]; ++i) h += doc_wrd[i] < (uint32_t)VD;
+            for (uint32_t i = soff[q]; i < soff[q + 1]; ++i) h += slot_wrd[i] < (uint32_t)VD;
             hw[q + 1] = hw[q] + h;
         }
         std::vector<uint32_t> tile_doc(T + 1, 0);
@@ -232,18 +231,22 @@ std::vector<uint4> build_sample_items(const ezlda_args &a, const std::vector<uin
 }
 
 // Runs: a word's consecutive tokens in one document (tokens of a word are
-// numbered in doc order), at most `cap` each.  Inside an item the runs are
-// ordered longest first (then by token), so the lanes of a wavefront draw
-// similar numbers of tokens; the order does not affect any draw.  Items are
-// rewritten as run ranges (w, first run, end run, 0).
-std::vector<uint2> build_runs(std::vector<uint4> &items, const std::vector<uint32_t> &tok_doc, uint32_t cap) {
+// numbered in doc order), at most `cap` each, as (first token, first slot,
+// doc, tokens); a run's slots are consecutive too (a doc's slots are grouped
+// by word, then position).  Inside an item the runs are ordered longest first
+// (then by token), so the lanes of a wavefront draw similar numbers of
+// tokens; the order does not affect any draw.  Items are rewritten as run
+// ranges (w, first run, end run, 0).
+std::vector<uint4> build_runs(std::vector<uint4> &items, const std::vector<uint32_t> &tok_doc,
+                              const std::vector<uint32_t> &tok_slot, uint32_t cap) {
     const int64_t n = (int64_t)items.size();
     std::vector<uint64_t> cnt(n + 1, 0);
     auto for_runs = [&](const uint4 &it, auto &&emit) {
         uint32_t t = it.y;
         while (t < it.z) {
             uint32_t e = t + 1;
-            while (e < it.z && e - t < cap && tok_doc[e] == tok_doc[t]) ++e;
+            // same doc implies the next slot (the sampler writes z_doc[slot + j])
+            while (e < it.z && e - t < cap && tok_doc[e] == tok_doc[t] && tok_slot[e] == tok_slot[t] + (e - t)) ++e;
             emit(t, e - t);
             t = e;
         }
@@ -256,15 +259,15 @@ std::vector<uint2> build_runs(std::vector<uint4> &items, const std::vector<uint3
         }
     });
     for (int64_t i = 0; i < n; ++i) cnt[i + 1] += cnt[i];
-    std::vector<uint2> runs(cnt[n]);
+    std::vector<uint4> runs(cnt[n]);
     parallel_for(n, [&](int64_t b, int64_t e, int) {
         for (int64_t i = b; i < e; ++i) {
             uint4 &it = items[i];
             if (it.x == 0xFFFFFFFFu) continue;
             uint64_t k = cnt[i];
-            for_runs(it, [&](uint32_t t, uint32_t l) { runs[k++] = make_uint2(t, l); });
+            for_runs(it, [&](uint32_t t, uint32_t l) { runs[k++] = make_uint4(t, tok_slot[t], tok_doc[t], l); });
             std::stable_sort(runs.begin() + cnt[i], runs.begin() + cnt[i + 1],
-                             [](const uint2 &x, const uint2 &y) { return x.y > y.y; });
+                             [](const uint4 &x, const uint4 &y) { return x.w > y.w; });
             it = make_uint4(it.x, (uint32_t)cnt[i], (uint32_t)cnt[i + 1], 0u);
         }
     });
@@ -276,8 +279,8 @@ std::vector<uint2> build_runs(std::vector<uint4> &items, const std::vector<uint3
 // run_cap): the head/tail split of WT, the WT head segments, the sampler
 // items and runs.  Token order, DT rows and z do not depend on it, so it can
 // be rebuilt on a loaded corpus; counts are stale afterwards.
-int build_layout(ezlda_ctx *c, const std::vector<uint32_t> &tok_doc, const std::vector<uint32_t> &doc_wrd,
-                 const std::vector<int32_t> &dord) {
+int build_layout(ezlda_ctx *c, const std::vector<uint32_t> &tok_doc, const std::vector<uint32_t> &tok_slot,
+                 const std::vector<uint32_t> &slot_wrd) {
     ez::DevState &s = c->s;
     const int V = s.V, K = s.K, D = s.D;
     if (c->graph_exec) hipGraphExecDestroy(c->graph_exec);
@@ -308,17 +311,10 @@ int build_layout(ezlda_ctx *c, const std::vector<uint32_t> &tok_doc, const std::
     for (int r = 0; r < VD; ++r)
         for (uint32_t b = c->word_off[r]; b < c->word_off[r + 1]; b += WT_SEG)
             wt_segs.push_back(make_uint4((uint32_t)r, b, std::min(b + WT_SEG, c->word_off[r + 1]), 0u));
-    std::vector<uint32_t> drank(D);
-    for (int q = 0; q < D; ++q) drank[dord[q]] = (uint32_t)q;
-    std::vector<uint32_t> tok_rank(tok_doc.size());
-    parallel_for((int64_t)tok_doc.size(), [&](int64_t b, int64_t e, int) {
-        for (int64_t t = b; t < e; ++t) tok_rank[t] = drank[tok_doc[t]];
-    });
-    std::vector<uint4> smp_segs = build_sample_items(c->a, c->word_off, tok_rank, doc_wrd, c->doc_off, dord, VD, K,
-                                                     c->n_xcd, &c->doc_tiles);
-    tok_rank.clear();
-    tok_rank.shrink_to_fit();
-    const std::vector<uint2> runs = build_runs(smp_segs, tok_doc, (uint32_t)c->a.run_cap);
+    (void)D;
+    std::vector<uint4> smp_segs =
+        build_sample_items(c->a, c->word_off, tok_doc, c->soff, slot_wrd, VD, K, c->n_xcd, &c->doc_tiles);
+    const std::vector<uint4> runs = build_runs(smp_segs, tok_doc, tok_slot, (uint32_t)c->a.run_cap);
     int r = 0;
     if ((r = ldalloc(c, &s.wt_head, (size_t)VD * K))) return r;
     if ((r = lupload(c, &s.tail_off, tail_off))) return r;
@@ -485,20 +481,18 @@ int ezlda_load_docs(ezlda_ctx *c, int32_t V, int32_t D, const int64_t *doc_off, 
         const int64_t lx = doc_off[x + 1] - doc_off[x], ly = doc_off[y + 1] - doc_off[y];
         return lx > ly || (lx == ly && x < y);
     });
-    std::vector<uint32_t> drank(D);
-    for (int q = 0; q < D; ++q) drank[dord[q]] = (uint32_t)q;
+    // doc slots: doc q = dord[q] owns [soff[q], soff[q+1])
+    c->dord = dord;
+    c->soff.assign(D + 1, 0);
+    for (int q = 0; q < D; ++q) c->soff[q + 1] = c->soff[q] + (uint32_t)(doc_off[dord[q] + 1] - doc_off[dord[q]]);
     // placement: contiguous ranges of the doc order per thread keep the
     // (doc order, position) order of tokens inside each word
     const int T = host_threads();
     std::vector<int> dsplit(T + 1, 0);
-    {
-        std::vector<int64_t> pre(D + 1, 0);
-        for (int q = 0; q < D; ++q) pre[q + 1] = pre[q] + (doc_off[dord[q] + 1] - doc_off[dord[q]]);
-        for (int t = 0; t <= T; ++t)
-            dsplit[t] = (int)(std::lower_bound(pre.begin(), pre.end(), N64 * t / T) - pre.begin());
-        dsplit[0] = 0;
-        dsplit[T] = D;
-    }
+    for (int t = 0; t <= T; ++t)
+        dsplit[t] = (int)(std::lower_bound(c->soff.begin(), c->soff.end(), (uint32_t)(N64 * t / T)) - c->soff.begin());
+    dsplit[0] = 0;
+    dsplit[T] = D;
     std::vector<std::vector<uint32_t>> tcnt(T, std::vector<uint32_t>(V, 0));
     {
         std::vector<std::thread> th;
@@ -517,22 +511,34 @@ int ezlda_load_docs(ezlda_ctx *c, int32_t V, int32_t D, const int64_t *doc_off, 
             cur += n;
         }
     }
-    std::vector<uint32_t> tok_doc(N), doc_wrd(N);
+    // token ids and slots: inside a doc, slots follow (word rank, position);
+    // the doc's tokens of one word take consecutive ids in position order
+    std::vector<uint32_t> tok_doc(N), tok_slot(N), slot_wrd(N);
     std::vector<uint16_t> z(N);
     c->doc_tok.assign(N, 0);
+    c->slot_tok.assign(N, 0);
     {
         std::vector<std::thread> th;
         for (int t = 0; t < T; ++t)
             th.emplace_back([&, t] {
                 std::vector<uint32_t> &cur = tcnt[t];
+                std::vector<uint64_t> key;
                 for (int q = dsplit[t]; q < dsplit[t + 1]; ++q) {
                     const int d = dord[q];
-                    for (int64_t i = doc_off[d]; i < doc_off[d + 1]; ++i) {
-                        const int32_t r = c->rank_of[words[i]];
+                    const int64_t b = doc_off[d], n = doc_off[d + 1] - b;
+                    key.resize((size_t)n);
+                    for (int64_t i = 0; i < n; ++i) key[i] = ((uint64_t)c->rank_of[words[b + i]] << 16) | (uint64_t)i;
+                    std::sort(key.begin(), key.end());
+                    uint32_t p = c->soff[q];
+                    for (int64_t j = 0; j < n; ++j, ++p) {
+                        const uint32_t r = (uint32_t)(key[j] >> 16);
+                        const int64_t i = b + (int64_t)(key[j] & 0xFFFFu);
                         const uint32_t tk = cur[r]++;
-                        tok_doc[tk] = (uint32_t)d;
+                        tok_doc[tk] = (uint32_t)q;
+                        tok_slot[tk] = p;
+                        c->slot_tok[p] = tk;
+                        slot_wrd[p] = r;
                         c->doc_tok[i] = tk;
-                        doc_wrd[i] = (uint32_t)r;
                         if (z0) z[tk] = z0[i];
                     }
                 }
@@ -547,14 +553,17 @@ int ezlda_load_docs(ezlda_ctx *c, int32_t V, int32_t D, const int64_t *doc_off, 
                 z[t] = (uint16_t)(ez_philox_x0(seed, (uint64_t)t, 0u, EZ_TAG_INIT) % (uint32_t)K);
         });
     }
-    // DT rows in the doc order above, each 128-byte aligned with capacity
-    // min(n_d, K) rounded up to 32 entries (the build zero-fills the rest)
+    std::vector<uint16_t> z_doc(N);
+    parallel_for(N, [&](int64_t b, int64_t e, int) {
+        for (int64_t p = b; p < e; ++p) z_doc[p] = z[c->slot_tok[p]];
+    });
+    // DT rows in doc order q, each 128-byte aligned with capacity min(n_d, K)
+    // rounded up to 32 entries (the build zero-fills the rest)
     std::vector<uint2> doc_info(D);
     uint64_t dtc = 0;
     for (int q = 0; q < D; ++q) {
-        const int d = dord[q];
-        const uint32_t n = c->doc_off[d + 1] - c->doc_off[d];
-        doc_info[d] = make_uint2((uint32_t)dtc, n << 16);
+        const uint32_t n = c->soff[q + 1] - c->soff[q];
+        doc_info[q] = make_uint2((uint32_t)dtc, n << 16);
         dtc += (std::min<uint32_t>(n, (uint32_t)K) + 31u) & ~31u;
     }
     if (dtc >= 0xFFFFFFF0ull) return fail(c, EZLDA_EINVAL, "DT capacity exceeds 2^32 entries");
@@ -568,12 +577,11 @@ int ezlda_load_docs(ezlda_ctx *c, int32_t V, int32_t D, const int64_t *doc_off, 
     for (int d = 0; d < D; ++d) c->dt_cap_canon += std::min<uint32_t>(c->doc_off[d + 1] - c->doc_off[d], (uint32_t)K);
     int r = 0;
     if ((r = upload(c, &s.word_off, c->word_off))) return r;
-    if ((r = upload(c, &s.tok_doc, tok_doc))) return r;
     if ((r = upload(c, &s.z, z))) return r;
-    if ((r = dalloc(c, &s.aux, N))) return r;
-    if ((r = upload(c, &s.doc_off, c->doc_off))) return r;
-    if ((r = upload(c, &s.doc_tok, c->doc_tok))) return r;
-    if ((r = upload(c, &s.doc_wrd, doc_wrd))) return r;
+    if ((r = upload(c, &s.soff, c->soff))) return r;
+    if ((r = upload(c, &s.slot_wrd, slot_wrd))) return r;
+    if ((r = upload(c, &s.z_doc, z_doc))) return r;
+    if ((r = dalloc(c, &s.aux_doc, N))) return r;
     if ((r = upload(c, &s.doc_info, doc_info))) return r;
     // 256 entries of zero padding past the last row
     if ((r = dalloc(c, &s.dt, dtc + 256))) return r;
@@ -584,7 +592,7 @@ int ezlda_load_docs(ezlda_ctx *c, int32_t V, int32_t D, const int64_t *doc_off, 
     if ((r = dalloc(c, &s.counters, 16))) return r;
     if ((r = dalloc(c, &s.stats, 2))) return r;
     if ((r = dalloc(c, &s.ll_out, 1))) return r;
-    if ((r = build_layout(c, tok_doc, doc_wrd, dord))) return r;
+    if ((r = build_layout(c, tok_doc, tok_slot, slot_wrd))) return r;
     EZ_TRY_HIP(c, hipMemsetAsync(s.counters, 0, 16 * sizeof(uint32_t), c->st));
     EZ_TRY_HIP(c, hipMemsetAsync(s.stats, 0, 2 * sizeof(unsigned long long), c->st));
     EZ_TRY_HIP(c, hipStreamSynchronize(c->st));
@@ -719,24 +727,28 @@ int ezlda_sweep_graph(ezlda_ctx *c, int32_t n) {
 // Graph: one event pair around n replays (stage times are then -1).
 namespace {
 
-// Host copies of the corpus arrays the layout is built from.
-int layout_inputs(ezlda_ctx *c, std::vector<uint32_t> &tok_doc, std::vector<uint32_t> &doc_wrd,
-                  std::vector<int32_t> &dord) {
-    const ez::DevState &s = c->s;
-    tok_doc.resize(s.N);
-    doc_wrd.resize(s.N);
-    if (s.N) {
-        EZ_TRY_HIP(c, hipMemcpy(tok_doc.data(), s.tok_doc, (size_t)s.N * 4, hipMemcpyDeviceToHost));
-        EZ_TRY_HIP(c, hipMemcpy(doc_wrd.data(), s.doc_wrd, (size_t)s.N * 4, hipMemcpyDeviceToHost));
-    }
-    dord.resize(s.D);
-    for (int d = 0; d < s.D; ++d) dord[d] = d;
-    const std::vector<uint32_t> &off = c->doc_off;
-    std::sort(dord.begin(), dord.end(), [&](int32_t x, int32_t y) {
-        const uint32_t lx = off[x + 1] - off[x], ly = off[y + 1] - off[y];
-        return lx > ly || (lx == ly && x < y);
+// The per-token inputs of the layout, from the host mirrors: doc q and slot
+// of every token, word rank of every slot.
+void layout_inputs(ezlda_ctx *c, std::vector<uint32_t> &tok_doc, std::vector<uint32_t> &tok_slot,
+                   std::vector<uint32_t> &slot_wrd) {
+    const uint32_t N = c->s.N;
+    const int D = c->s.D, V = c->s.V;
+    tok_doc.resize(N);
+    tok_slot.resize(N);
+    slot_wrd.resize(N);
+    parallel_for(D, [&](int64_t b, int64_t e, int) {
+        for (int64_t q = b; q < e; ++q)
+            for (uint32_t p = c->soff[q]; p < c->soff[q + 1]; ++p) {
+                tok_doc[c->slot_tok[p]] = (uint32_t)q;
+                tok_slot[c->slot_tok[p]] = p;
+            }
     });
-    return EZLDA_OK;
+    std::vector<uint32_t> tok_wrd(N);
+    for (int r = 0; r < V; ++r)
+        for (uint32_t t = c->word_off[r]; t < c->word_off[r + 1]; ++t) tok_wrd[t] = (uint32_t)r;
+    parallel_for(N, [&](int64_t b, int64_t e, int) {
+        for (int64_t p = b; p < e; ++p) slot_wrd[p] = tok_wrd[c->slot_tok[p]];
+    });
 }
 
 }  // namespace
@@ -754,20 +766,18 @@ int ezlda_set_layout(ezlda_ctx *c, const ezlda_args *args) {
     a.run_cap = args->run_cap;
     const std::string m = check_layout_args(a);
     if (!m.empty()) return fail(c, EZLDA_EINVAL, m);
-    std::vector<uint32_t> tok_doc, doc_wrd;
-    std::vector<int32_t> dord;
-    if (int r = layout_inputs(c, tok_doc, doc_wrd, dord)) return r;
+    std::vector<uint32_t> tok_doc, tok_slot, slot_wrd;
+    layout_inputs(c, tok_doc, tok_slot, slot_wrd);
     c->a = a;
-    return build_layout(c, tok_doc, doc_wrd, dord);
+    return build_layout(c, tok_doc, tok_slot, slot_wrd);
 }
 
 int ezlda_autotune(ezlda_ctx *c, int32_t sweeps_per_trial, ezlda_args *chosen, double *ms_per_sweep,
                    ezlda_tune_trial *trials, int32_t max_trials, int32_t *num_trials) {
     if (int r = need_loaded(c)) return r;
     if (sweeps_per_trial < 1) return fail(c, EZLDA_EINVAL, "sweeps_per_trial must be >= 1");
-    std::vector<uint32_t> tok_doc, doc_wrd;
-    std::vector<int32_t> dord;
-    if (int r = layout_inputs(c, tok_doc, doc_wrd, dord)) return r;
+    std::vector<uint32_t> tok_doc, tok_slot, slot_wrd;
+    layout_inputs(c, tok_doc, tok_slot, slot_wrd);
     int dev = 0, ncu = 256;
     if (hipGetDevice(&dev) != hipSuccess ||
         hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || ncu < 1)
@@ -776,7 +786,7 @@ int ezlda_autotune(ezlda_ctx *c, int32_t sweeps_per_trial, ezlda_args *chosen, d
     int32_t nt = 0;
     auto trial = [&](const ezlda_args &a, double *ms) -> int {
         c->a = a;
-        if (int r = build_layout(c, tok_doc, doc_wrd, dord)) return r;
+        if (int r = build_layout(c, tok_doc, tok_slot, slot_wrd)) return r;
         double t = 0.0, st[EZLDA_NUM_STAGES];
         if (int r = ezlda_time_sweeps(c, 1, 0, &t, st)) return r;  // first sweep of a layout: not timed
         if (int r = ezlda_time_sweeps(c, sweeps_per_trial, 0, &t, st)) return r;
@@ -834,7 +844,7 @@ int ezlda_autotune(ezlda_ctx *c, int32_t sweeps_per_trial, ezlda_args *chosen, d
         }
     }
     c->a = best;
-    if (int r = build_layout(c, tok_doc, doc_wrd, dord)) return r;
+    if (int r = build_layout(c, tok_doc, tok_slot, slot_wrd)) return r;
     if (chosen) *chosen = best;
     if (ms_per_sweep) *ms_per_sweep = best_ms;
     if (num_trials) *num_trials = nt;
@@ -944,12 +954,14 @@ int ezlda_get_topics(ezlda_ctx *c, uint16_t *zout) {
 
 int ezlda_set_topics(ezlda_ctx *c, const uint16_t *zin) {
     if (int r = need_loaded(c)) return r;
-    std::vector<uint16_t> z(c->s.N);
+    std::vector<uint16_t> z(c->s.N), z_doc(c->s.N);
     for (size_t i = 0; i < z.size(); ++i) {
         if (zin[i] >= c->s.K) return fail(c, EZLDA_EINVAL, "topic out of range");
         z[c->doc_tok[i]] = zin[i];
     }
+    for (size_t p = 0; p < z.size(); ++p) z_doc[p] = z[c->slot_tok[p]];
     EZ_TRY_HIP(c, hipMemcpy(c->s.z, z.data(), z.size() * sizeof(uint16_t), hipMemcpyHostToDevice));
+    EZ_TRY_HIP(c, hipMemcpy(c->s.z_doc, z_doc.data(), z_doc.size() * sizeof(uint16_t), hipMemcpyHostToDevice));
     c->counts_fresh = false;
     return EZLDA_OK;
 }
@@ -1007,11 +1019,14 @@ int ezlda_get_dt_raw(ezlda_ctx *c, uint32_t *entries, int32_t *nnz) {
     std::vector<uint32_t> dt(c->dt_cap);
     if (s.D) EZ_TRY_HIP(c, hipMemcpy(di.data(), s.doc_info, di.size() * sizeof(uint2), hipMemcpyDeviceToHost));
     if (!dt.empty()) EZ_TRY_HIP(c, hipMemcpy(dt.data(), s.dt, dt.size() * 4, hipMemcpyDeviceToHost));
+    std::vector<int32_t> qof(s.D);
+    for (int q = 0; q < s.D; ++q) qof[c->dord[q]] = q;
     uint64_t canon = 0;  // rows packed in input doc order at sum_{e<d} min(n_e, K)
     for (int d = 0; d < s.D; ++d) {
-        nnz[d] = (int32_t)(di[d].y & 0xFFFFu);
+        const uint2 x = di[qof[d]];
+        nnz[d] = (int32_t)(x.y & 0xFFFFu);
         for (int32_t i = 0; i < nnz[d]; ++i) {  // device rows hold topic << 2
-            const uint32_t e = dt[di[d].x + i];
+            const uint32_t e = dt[x.x + i];
             entries[canon + i] = ((e & 0xFFFFu) >> 2) | (e & 0xFFFF0000u);
         }
         canon += std::min<uint32_t>(c->doc_off[d + 1] - c->doc_off[d], (uint32_t)s.K);
@@ -1034,9 +1049,10 @@ int ezlda_get_top(ezlda_ctx *c, uint16_t *top) {
 
 int ezlda_get_aux(ezlda_ctx *c, uint32_t *aux) {
     if (int r = need_loaded(c)) return r;
-    std::vector<uint32_t> a(c->s.N);
-    EZ_TRY_HIP(c, hipMemcpy(a.data(), c->s.aux, a.size() * 4, hipMemcpyDeviceToHost));
-    for (size_t i = 0; i < a.size(); ++i) aux[i] = a[c->doc_tok[i]];
+    std::vector<uint32_t> a(c->s.N), at(c->s.N);
+    EZ_TRY_HIP(c, hipMemcpy(a.data(), c->s.aux_doc, a.size() * 4, hipMemcpyDeviceToHost));
+    for (size_t p = 0; p < a.size(); ++p) at[c->slot_tok[p]] = a[p];
+    for (size_t i = 0; i < a.size(); ++i) aux[i] = at[c->doc_tok[i]];
     return EZLDA_OK;
 }
 
