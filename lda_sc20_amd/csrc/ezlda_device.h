@@ -72,6 +72,9 @@ hipError_t launch_foldin_step(const uint4 *slots, int nslots, int K, int sweeps_
 
 // Bytes of dynamic LDS the sampler needs for K topics and block size bd.
 size_t sample_lds_bytes(int K, int bd);
+// Static LDS of the sampler kernels: must be 0 (phi is read at LDS address
+// 0); (size_t)-1 if it cannot be queried.
+size_t sampler_static_lds();
 // Largest LDS any launch of the library asks for at this K and block size.
 size_t max_lds_bytes(int K, int bd);
 

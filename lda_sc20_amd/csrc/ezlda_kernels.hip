@@ -63,6 +63,12 @@ __device__ __forceinline__ float phi_of(const float *phi, uint32_t e) {
     return *reinterpret_cast<const float *>(reinterpret_cast<const char *>(phi) + (e & 0xFFFCu));
 }
 
+// The same read for a phi table at LDS address 0 (the sampler's dynamic LDS
+// when the kernel has no static LDS, which sampler_static_lds() lets the host
+// check): the entry's low half is the address itself, no base to add.
+typedef __attribute__((address_space(3))) const float lds_cfloat;
+__device__ __forceinline__ float phi_at0(uint32_t e) { return *(lds_cfloat *)(e & 0xFFFCu); }
+
 // 16-bit counters, two per LDS word (counts < 65536 by construction).
 __device__ __forceinline__ void h16_add(uint32_t *h, uint32_t k) {
     atomicAdd(&h[k >> 1], 1u << ((k & 1u) << 4));
@@ -300,16 +306,16 @@ static constexpr int kBlkU4 = EZ_SAMPLE_BLK;
 
 // Running FMA sum over one block of a DT row: kBlkU4 16-byte loads issued
 // together, then the entries in order.
-__device__ __forceinline__ float block_sum(const uint4 *__restrict__ p, const float *phi, float s) {
+__device__ __forceinline__ float block_sum(const uint4 *__restrict__ p, float s) {
     uint4 e[kBlkU4];
 #pragma unroll
     for (int i = 0; i < kBlkU4; ++i) e[i] = p[i];
 #pragma unroll
     for (int i = 0; i < kBlkU4; ++i) {
-        s = __builtin_fmaf(dt_count(e[i].x), phi_of(phi, e[i].x), s);
-        s = __builtin_fmaf(dt_count(e[i].y), phi_of(phi, e[i].y), s);
-        s = __builtin_fmaf(dt_count(e[i].z), phi_of(phi, e[i].z), s);
-        s = __builtin_fmaf(dt_count(e[i].w), phi_of(phi, e[i].w), s);
+        s = __builtin_fmaf(dt_count(e[i].x), phi_at0(e[i].x), s);
+        s = __builtin_fmaf(dt_count(e[i].y), phi_at0(e[i].y), s);
+        s = __builtin_fmaf(dt_count(e[i].z), phi_at0(e[i].z), s);
+        s = __builtin_fmaf(dt_count(e[i].w), phi_at0(e[i].w), s);
     }
     return s;
 }
@@ -447,10 +453,10 @@ __device__ __forceinline__ void sample_item(const SampleArgs &A, uint32_t item, 
         float S = 0.0f;
 #pragma unroll
         for (int b = 0; b < kCP; ++b) {
-            if ((uint32_t)b < nb) S = block_sum(rp + kBlkU4 * b, phi, S);
+            if ((uint32_t)b < nb) S = block_sum(rp + kBlkU4 * b, S);
             cp[b] = S;
         }
-        for (uint32_t b = kCP; b < nb; ++b) S = block_sum(rp + kBlkU4 * b, phi, S);
+        for (uint32_t b = kCP; b < nb; ++b) S = block_sum(rp + kBlkU4 * b, S);
         const float Aa = P1 + S;
         const float total = Aa + Q;
         while (need) {
@@ -485,10 +491,10 @@ __device__ __forceinline__ void sample_item(const SampleArgs &A, uint32_t item, 
                 bool found = false;
                 for (uint32_t g = b0 * kBlkU4; g < b1 * kBlkU4 && !found; ++g) {
                     const uint4 e = rp[g];
-                    const float r0 = __builtin_fmaf(dt_count(e.x), phi_of(phi, e.x), run);
-                    const float r1 = __builtin_fmaf(dt_count(e.y), phi_of(phi, e.y), r0);
-                    const float r2 = __builtin_fmaf(dt_count(e.z), phi_of(phi, e.z), r1);
-                    const float r3 = __builtin_fmaf(dt_count(e.w), phi_of(phi, e.w), r2);
+                    const float r0 = __builtin_fmaf(dt_count(e.x), phi_at0(e.x), run);
+                    const float r1 = __builtin_fmaf(dt_count(e.y), phi_at0(e.y), r0);
+                    const float r2 = __builtin_fmaf(dt_count(e.z), phi_at0(e.z), r1);
+                    const float r3 = __builtin_fmaf(dt_count(e.w), phi_at0(e.w), r2);
                     if (r3 > t2) {
                         newz = dt_topic(r0 > t2 ? e.x : r1 > t2 ? e.y : r2 > t2 ? e.z : e.w);
                         found = true;
@@ -498,7 +504,7 @@ __device__ __forceinline__ void sample_item(const SampleArgs &A, uint32_t item, 
                 if (!found) {  // rounding left every running sum at or below t2
                     const uint32_t *rr = A.dt + di.x;
                     for (uint32_t i = 0; i < nnz; ++i)
-                        if (dt_count(rr[i]) > 0.0f && phi_of(phi, rr[i]) > 0.0f) newz = dt_topic(rr[i]);
+                        if (dt_count(rr[i]) > 0.0f && phi_at0(rr[i]) > 0.0f) newz = dt_topic(rr[i]);
                 }
             } else {
                 const float t2 = target - Aa;
@@ -547,9 +553,12 @@ __device__ __forceinline__ void sample_item(const SampleArgs &A, uint32_t item, 
 // others.  Every workgroup leaves once every queue is past its last item.
 template <bool SHORTCUT>
 __global__ void __launch_bounds__(1024) k_sample(const SampleArgs A) {
+    // the only LDS of the kernel, so phi starts at address 0 and a DT entry's
+    // low half is directly the LDS address of its phi value
     extern __shared__ float smf[];
-    __shared__ float f123[3];
-    __shared__ uint32_t s_item;
+    const int NS = (A.K + 63) >> 6;
+    float *f123 = smf + 2 * NS * 64 + 2 * NS;  // after phi, qs, qt, qacc
+    uint32_t &s_item = *reinterpret_cast<uint32_t *>(f123 + 3);
     const uint32_t sweep = A.counters[0];
     uint32_t n_full = 0, n_scanned = 0;
     if (A.n_queues == 0) {
@@ -944,7 +953,15 @@ hipError_t launch_build_dt(const DevState &s, hipStream_t st) {
 size_t sample_lds_bytes(int K, int bd) {
     const int NS = (K + 63) >> 6;
     (void)bd;
-    return (2 * (size_t)NS * 64 + 2 * (size_t)NS) * sizeof(float);
+    return (2 * (size_t)NS * 64 + 2 * (size_t)NS + 4) * sizeof(float);  // + f123[3], item
+}
+
+size_t sampler_static_lds() {
+    hipFuncAttributes a1, a0;
+    if (hipFuncGetAttributes(&a1, reinterpret_cast<const void *>(&k_sample<true>)) != hipSuccess ||
+        hipFuncGetAttributes(&a0, reinterpret_cast<const void *>(&k_sample<false>)) != hipSuccess)
+        return (size_t)-1;
+    return std::max(a1.sharedSizeBytes, a0.sharedSizeBytes);
 }
 
 size_t max_lds_bytes(int K, int bd) {

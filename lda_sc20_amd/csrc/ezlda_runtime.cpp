@@ -406,6 +406,10 @@ ezlda_ctx *ezlda_create(const ezlda_args *args) {
             fail(nullptr, EZLDA_EHIP, "cannot query the device's LDS size");
             return nullptr;
         }
+        if (ez::sampler_static_lds() != 0) {
+            fail(nullptr, EZLDA_EHIP, "the sampler kernel has static LDS; its phi table must start at LDS address 0");
+            return nullptr;
+        }
         if (lds > (size_t)lmax) {
             fail(nullptr, EZLDA_EINVAL,
                  "K=" + std::to_string(a.K) + " needs " + std::to_string(lds) + " bytes of LDS per block; the device "
