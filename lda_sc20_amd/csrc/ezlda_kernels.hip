@@ -726,6 +726,22 @@ __global__ void __launch_bounds__(256) k_foldin_tables(int V, int VD, int K, flo
     }
 }
 
+static constexpr int kFB = 8, kFCP = 16;  // fold-in S blocks and checkpoints
+
+// Running FMA sum over kFB row entries (LDS) with phi gathered from the
+// frozen table: the gathers are issued together, the FMAs in order.
+__device__ __forceinline__ float foldin_block(const uint32_t *row, const float *__restrict__ ph, float s) {
+    uint32_t e[kFB];
+    float f[kFB];
+#pragma unroll
+    for (int u = 0; u < kFB; ++u) e[u] = row[u];
+#pragma unroll
+    for (int u = 0; u < kFB; ++u) f[u] = ph[e[u] & 0xFFFFu];
+#pragma unroll
+    for (int u = 0; u < kFB; ++u) s = __builtin_fmaf((float)(e[u] >> 16), f[u], s);
+    return s;
+}
+
 // One fold-in sweep for every active request slot (rid, first token, n,
 // sweep): one wavefront per slot.  The doc's topic counts are histogrammed
 // in LDS from its current z and compacted into an ascending row; then one
@@ -740,8 +756,8 @@ __global__ void __launch_bounds__(256) k_foldin_step(
     extern __shared__ uint32_t shf[];
     const int NS = (K + 63) >> 6;
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6, nwv = blockDim.x >> 6;
-    const int KH = (K + 1) >> 1;
-    uint32_t *h = shf + wv * (KH + K);
+    const int KH = (K + 1) >> 1, KR = (K + kFB - 1) / kFB * kFB;
+    uint32_t *h = shf + wv * (KH + KR);
     uint32_t *row = h + KH;
     for (int k = lane; k < KH; k += 64) h[k] = 0;
     wave_lds_sync();
@@ -766,6 +782,8 @@ __global__ void __launch_bounds__(256) k_foldin_step(
             wave_lds_sync();
             if (k < K && !(k & 1)) h[k >> 1] = 0;
         }
+        // zero entries up to the next block: count 0 adds exactly nothing
+        if ((uint32_t)lane < ((kFB - (nz % kFB)) % kFB)) row[nz + lane] = 0u;
         wave_lds_sync();
         for (uint32_t base = 0; base < n; base += 64) {
             const uint32_t i = base + (uint32_t)lane;
@@ -774,27 +792,51 @@ __global__ void __launch_bounds__(256) k_foldin_step(
                 const float *ph = phi + (size_t)w * K;
                 const float *qa = qacc + (size_t)w * NS;
                 const float u = ez_u24(ez_philox_x0(seed, key | i, sweep, EZ_TAG_FOLDIN));
+                // S over the row in blocks of kFB entries (8 independent phi
+                // gathers in flight per lane), zero padding adding nothing;
+                // cp[b] = running sum after block b, for the first kFCP blocks
+                const uint32_t nb = (nz + kFB - 1) / kFB;
+                float cp[kFCP];
                 float S = 0.0f;
-                for (uint32_t j = 0; j < nz; ++j) {
-                    const uint32_t e = row[j];
-                    S = __builtin_fmaf((float)(e >> 16), ph[e & 0xFFFFu], S);
+#pragma unroll
+                for (int b = 0; b < kFCP; ++b) {
+                    if ((uint32_t)b < nb) S = foldin_block(row + kFB * b, ph, S);
+                    cp[b] = S;
                 }
+                for (uint32_t b = kFCP; b < nb; ++b) S = foldin_block(row + kFB * b, ph, S);
                 const float target = u * (S + qa[NS - 1]);
                 int zz = -1;
                 if (target < S) {
+                    // the first block whose running sum exceeds the target,
+                    // then its entries in order (a running sum above the
+                    // target exists: the last one is S)
+                    int bs = -1;
+#pragma unroll
+                    for (int b = kFCP - 1; b >= 0; --b)
+                        if (cp[b] > target) bs = b;
                     float run = 0.0f;
-                    int last = -1;
-                    for (uint32_t j = 0; j < nz; ++j) {
+                    uint32_t g0 = 0, g1 = nz;
+                    if (bs >= 0) {
+#pragma unroll
+                        for (int b = 0; b < kFCP - 1; ++b)
+                            if (b + 1 == bs) run = cp[b];
+                        g0 = (uint32_t)bs * kFB;
+                        g1 = min(nz, g0 + kFB);
+                    } else {
+                        run = cp[kFCP - 1];
+                        g0 = kFCP * kFB;
+                    }
+                    for (uint32_t j = g0; j < g1; ++j) {
                         const uint32_t e = row[j];
-                        const float c = (float)(e >> 16), f = ph[e & 0xFFFFu];
-                        run = __builtin_fmaf(c, f, run);
+                        run = __builtin_fmaf((float)(e >> 16), ph[e & 0xFFFFu], run);
                         if (run > target) {
                             zz = (int)(e & 0xFFFFu);
                             break;
                         }
-                        if (c > 0.0f && f > 0.0f) last = (int)(e & 0xFFFFu);
                     }
-                    if (zz < 0) zz = last;
+                    if (zz < 0)  // not reached: kept as the canonical fallback
+                        for (uint32_t j = 0; j < nz; ++j)
+                            if (row[j] >> 16 && ph[row[j] & 0xFFFFu] > 0.0f) zz = (int)(row[j] & 0xFFFFu);
                 } else {
                     const float t2 = target - S;
                     int lo = 0, hi = NS;
@@ -853,8 +895,10 @@ hipError_t launch_foldin_tables(const DevState &s, float *phi, float *qs, float 
     return hipGetLastError();
 }
 
+static size_t foldin_wave_lds(int K) { return (size_t)(((K + 1) >> 1) + (K + kFB - 1) / kFB * kFB) * sizeof(uint32_t); }
+
 int foldin_waves_per_block(int K) {
-    const size_t per_wave = (size_t)(((K + 1) >> 1) + K) * sizeof(uint32_t);
+    const size_t per_wave = foldin_wave_lds(K);
     int w = 4;
     while (w > 1 && per_wave * w > 64 * 1024) --w;
     return w;
@@ -865,7 +909,7 @@ hipError_t launch_foldin_step(const uint4 *slots, int nslots, int K, int sweeps_
                               int32_t *counts_out, uint64_t seed, hipStream_t st) {
     if (nslots <= 0) return hipSuccess;
     const int nwv = foldin_waves_per_block(K);
-    const size_t lds = (size_t)nwv * (((K + 1) >> 1) + K) * sizeof(uint32_t);
+    const size_t lds = (size_t)nwv * foldin_wave_lds(K);
     hipLaunchKernelGGL(k_foldin_step, dim3(grid_for(nslots, nwv, 1 << 16)), dim3(64 * nwv), lds, st, slots, nslots, K,
                        sweeps_total, alpha, phi, qs, qacc, wpool, zpool, counts_out, seed);
     return hipGetLastError();
@@ -969,7 +1013,7 @@ size_t max_lds_bytes(int K, int bd) {
     m = std::max(m, 4 * wave_hist_bytes(K));                         // WT tail, top-3, DT build
     m = std::max(m, (size_t)K * sizeof(double));                     // log-likelihood
     m = std::max(m, (size_t)K * sizeof(float) + 1024 * sizeof(float));  // fold-in tables
-    m = std::max(m, (size_t)foldin_waves_per_block(K) * (((K + 1) >> 1) + K) * sizeof(uint32_t));
+    m = std::max(m, (size_t)foldin_waves_per_block(K) * foldin_wave_lds(K));
     return m;
 }
 
