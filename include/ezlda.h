@@ -78,7 +78,9 @@ typedef struct ezlda_info {
     int32_t num_docs;        /* D */
     int32_t num_head_words;  /* V_D */
     int32_t K;
-    int64_t dt_capacity;     /* sum_d min(n_d, K) */
+    int64_t dt_capacity;     /* entries of ezlda_get_dt_raw: sum_d of min(n_d, K),
+                                plus n_d / 63 when K <= 1024 (rows then hold
+                                a count in pieces of at most 63) */
     int64_t wt_tail_capacity;/* sum_tail min(n_w, K) */
     int32_t num_sample_items;/* sampler work items (word segments) */
     uint32_t sweep;          /* sweeps done (the Philox sweep counter) */
@@ -188,8 +190,10 @@ int ezlda_autotune(ezlda_ctx *ctx, int32_t sweeps_per_trial, ezlda_args *chosen,
 int ezlda_get_nk(ezlda_ctx *ctx, int32_t *nk);
 /* Dense V x K word-topic counts, rows by input word id. */
 int ezlda_get_wt_dense(ezlda_ctx *ctx, int32_t *wt);
-/* DT rows as stored: entries[dt_offset(d) + i] = topic | count << 16 for
- * i < nnz[d]; dt_offset(d) = sum_{e<d} min(n_e, K). */
+/* DT rows as the sampler walks them: entries[dt_offset(d) + i] = topic |
+ * count << 16 for i < nnz[d], ascending topic; for K <= 1024 a count above
+ * 63 appears as pieces of 63 and a remainder.  dt_offset(d) = sum_{e<d}
+ * cap(n_e), cap(n) = min(n, K) (+ n / 63, at most n, when K <= 1024). */
 int ezlda_get_dt_raw(ezlda_ctx *ctx, uint32_t *entries, int32_t *nnz);
 /* Top-3 topics per word (3 x uint16 per input word id; 0xFFFF = none). */
 int ezlda_get_top(ezlda_ctx *ctx, uint16_t *top);

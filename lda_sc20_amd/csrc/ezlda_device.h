@@ -25,7 +25,8 @@ struct DevState {
     uint32_t *aux_doc = nullptr;    // N: n_dk1 | n_dk2 << 16 of the slot's word
     // doc-topic rows
     uint2 *doc_info = nullptr;      // D, by doc q: x = DT row offset, y = nnz | len << 16
-    uint32_t *dt = nullptr;         // sum_d min(n_d,K): topic | count << 16
+    uint32_t *dt = nullptr;         // rows: u32 (topic << 2) | count << 16, or (compact) u16 topic | piece << 10
+    bool compact = false;           // K <= 1024: 16-bit entries, counts in pieces of <= 63
     // word-topic: dense head rows + sorted sparse tail rows
     int32_t *wt_head = nullptr;     // VD*K
     uint32_t *tail_off = nullptr;   // V-VD+1 (capacity min(n_w,K))

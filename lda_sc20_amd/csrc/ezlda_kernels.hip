@@ -63,11 +63,19 @@ __device__ __forceinline__ float phi_of(const float *phi, uint32_t e) {
     return *reinterpret_cast<const float *>(reinterpret_cast<const char *>(phi) + (e & 0xFFFCu));
 }
 
+// Compact DT rows (K <= kCompactK): 16-bit entries topic | piece << 10, a
+// count stored in pieces of at most kPiece.
+static constexpr uint32_t kCompactK = 1024, kPiece = 63;
+__device__ __forceinline__ float c16_count(uint32_t e) { return (float)((e >> 10) & 63u); }
+__device__ __forceinline__ uint32_t c16_topic(uint32_t e) { return e & 0x3FFu; }
+
 // The same read for a phi table at LDS address 0 (the sampler's dynamic LDS
 // when the kernel has no static LDS, which sampler_static_lds() lets the host
 // check): the entry's low half is the address itself, no base to add.
 typedef __attribute__((address_space(3))) const float lds_cfloat;
 __device__ __forceinline__ float phi_at0(uint32_t e) { return *(lds_cfloat *)(e & 0xFFFCu); }
+// phi of a compact entry (the 16-bit entry in the low half of e)
+__device__ __forceinline__ float phi_c16(uint32_t e) { return *(lds_cfloat *)((e & 0x3FFu) << 2); }
 
 // 16-bit counters, two per LDS word (counts < 65536 by construction).
 __device__ __forceinline__ void h16_add(uint32_t *h, uint32_t k) {
@@ -257,6 +265,7 @@ __global__ void k_word_top(int V, int VD, int K, float beta, const int32_t *__re
 // word's top-2 topics (the re-pack of :125-161), and compacts the row in
 // ascending topic order (:183-238).  A row starts on a 128-byte boundary and
 // is zero-padded to a multiple of 32 entries.
+template <bool COMPACT>
 __global__ void k_dt_build(int D, int K, const uint32_t *__restrict__ soff, const uint32_t *__restrict__ slot_wrd,
                            const uint16_t *__restrict__ z_doc, const ushort4 *__restrict__ wtop,
                            uint32_t *__restrict__ aux_doc, uint2 *__restrict__ doc_info, uint32_t *__restrict__ dt) {
@@ -280,19 +289,44 @@ __global__ void k_dt_build(int D, int K, const uint32_t *__restrict__ soff, cons
         const uint32_t base = doc_info[q].x;
         uint32_t nz = 0;
         if (e > b) {
-            for (int j = 0; j < K; j += 64) {
-                const int k = j + lane;
-                const uint32_t c = k < K ? h16_get(h, (uint32_t)k) : 0u;
-                const uint64_t m = __ballot(c != 0);
-                if (c) dt[base + nz + __popcll(m & lanemask_lt(lane))] = ((uint32_t)k << 2) | (c << 16);
-                nz += (uint32_t)__popcll(m);
+            if (COMPACT) {
+                // 16-bit entries topic | piece << 10, a count in pieces of at
+                // most kPiece (each topic's pieces in order)
+                uint16_t *dt16 = reinterpret_cast<uint16_t *>(dt) + base;
+                for (int j = 0; j < K; j += 64) {
+                    const int k = j + lane;
+                    const uint32_t c = k < K ? h16_get(h, (uint32_t)k) : 0u;
+                    const uint32_t m = (c + kPiece - 1u) / kPiece;  // entries of this topic
+                    uint32_t incl = m;                               // inclusive scan over lanes
+                    for (int d = 1; d < 64; d <<= 1) {
+                        const uint32_t v = __shfl_up(incl, d);
+                        if (lane >= d) incl += v;
+                    }
+                    uint32_t pos = nz + incl - m, left = c;
+                    for (uint32_t i = 0; i + 1 < m; ++i, left -= kPiece) dt16[pos++] = (uint16_t)(k | (kPiece << 10));
+                    if (m) dt16[pos] = (uint16_t)(k | (left << 10));
+                    nz += __shfl(incl, 63);
+                    wave_lds_sync();
+                    if (k < K && !(k & 1)) h[k >> 1] = 0;
+                }
                 wave_lds_sync();
-                if (k < K && !(k & 1)) h[k >> 1] = 0;
+                // zero entries up to the next 128 bytes: a zero entry adds nothing
+                if ((uint32_t)lane < ((64u - (nz & 63u)) & 63u)) dt16[nz + lane] = 0;
+            } else {
+                for (int j = 0; j < K; j += 64) {
+                    const int k = j + lane;
+                    const uint32_t c = k < K ? h16_get(h, (uint32_t)k) : 0u;
+                    const uint64_t m = __ballot(c != 0);
+                    if (c) dt[base + nz + __popcll(m & lanemask_lt(lane))] = ((uint32_t)k << 2) | (c << 16);
+                    nz += (uint32_t)__popcll(m);
+                    wave_lds_sync();
+                    if (k < K && !(k & 1)) h[k >> 1] = 0;
+                }
+                wave_lds_sync();
+                // zero the row up to the next multiple of 32 entries: the sampler
+                // reads rows in 128-byte blocks and a zero entry adds nothing
+                if ((uint32_t)lane < ((32u - (nz & 31u)) & 31u)) dt[base + nz + lane] = 0u;
             }
-            wave_lds_sync();
-            // zero the row up to the next multiple of 32 entries: the sampler
-            // reads rows in 128-byte blocks and a zero entry adds nothing
-            if ((uint32_t)lane < ((32u - (nz & 31u)) & 31u)) dt[base + nz + lane] = 0u;
         }
         if (lane == 0) doc_info[q].y = nz | ((e - b) << 16);
     }
@@ -306,16 +340,26 @@ static constexpr int kBlkU4 = EZ_SAMPLE_BLK;
 
 // Running FMA sum over one block of a DT row: kBlkU4 16-byte loads issued
 // together, then the entries in order.
+template <bool COMPACT>
 __device__ __forceinline__ float block_sum(const uint4 *__restrict__ p, float s) {
     uint4 e[kBlkU4];
 #pragma unroll
     for (int i = 0; i < kBlkU4; ++i) e[i] = p[i];
 #pragma unroll
     for (int i = 0; i < kBlkU4; ++i) {
-        s = __builtin_fmaf(dt_count(e[i].x), phi_at0(e[i].x), s);
-        s = __builtin_fmaf(dt_count(e[i].y), phi_at0(e[i].y), s);
-        s = __builtin_fmaf(dt_count(e[i].z), phi_at0(e[i].z), s);
-        s = __builtin_fmaf(dt_count(e[i].w), phi_at0(e[i].w), s);
+        if (COMPACT) {  // two 16-bit entries per dword, low half first
+            const uint32_t w[4] = {e[i].x, e[i].y, e[i].z, e[i].w};
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                s = __builtin_fmaf(c16_count(w[q]), phi_c16(w[q]), s);
+                s = __builtin_fmaf((float)(w[q] >> 26), phi_c16(w[q] >> 16), s);
+            }
+        } else {
+            s = __builtin_fmaf(dt_count(e[i].x), phi_at0(e[i].x), s);
+            s = __builtin_fmaf(dt_count(e[i].y), phi_at0(e[i].y), s);
+            s = __builtin_fmaf(dt_count(e[i].z), phi_at0(e[i].z), s);
+            s = __builtin_fmaf(dt_count(e[i].w), phi_at0(e[i].w), s);
+        }
     }
     return s;
 }
@@ -355,7 +399,7 @@ struct SampleArgs {
     uint64_t seed;
 };
 
-template <bool SHORTCUT>
+template <bool SHORTCUT, bool COMPACT>
 __device__ __forceinline__ void sample_item(const SampleArgs &A, uint32_t item, float *smf, float *f123,
                                             uint32_t sweep, uint32_t &n_full, uint32_t &n_scanned) {
     const int K = A.K, NS = (K + 63) >> 6;
@@ -441,9 +485,12 @@ __device__ __forceinline__ void sample_item(const SampleArgs &A, uint32_t item, 
             }
         }
         if (!need) continue;
-        const uint4 *rp = reinterpret_cast<const uint4 *>(A.dt + di.x);
+        // the row: 32-bit entries, or 16-bit entries at a 16-bit offset
+        const uint4 *rp = COMPACT ? reinterpret_cast<const uint4 *>(reinterpret_cast<const uint16_t *>(A.dt) + di.x)
+                                  : reinterpret_cast<const uint4 *>(A.dt + di.x);
         const uint32_t nnz = di.y & 0xFFFFu;
-        const uint32_t nb = (nnz + 4u * kBlkU4 - 1u) / (4u * kBlkU4);  // blocks of the row
+        constexpr uint32_t kEPB = (COMPACT ? 8u : 4u) * kBlkU4;  // entries per block
+        const uint32_t nb = (nnz + kEPB - 1u) / kEPB;            // blocks of the row
         n_full += (uint32_t)__popc(need);
         n_scanned += nnz;
         // S: one FMA per entry, left to right; cp[b] = running sum at the
@@ -453,10 +500,10 @@ __device__ __forceinline__ void sample_item(const SampleArgs &A, uint32_t item, 
         float S = 0.0f;
 #pragma unroll
         for (int b = 0; b < kCP; ++b) {
-            if ((uint32_t)b < nb) S = block_sum(rp + kBlkU4 * b, S);
+            if ((uint32_t)b < nb) S = block_sum<COMPACT>(rp + kBlkU4 * b, S);
             cp[b] = S;
         }
-        for (uint32_t b = kCP; b < nb; ++b) S = block_sum(rp + kBlkU4 * b, S);
+        for (uint32_t b = kCP; b < nb; ++b) S = block_sum<COMPACT>(rp + kBlkU4 * b, S);
         const float Aa = P1 + S;
         const float total = Aa + Q;
         while (need) {
@@ -491,20 +538,44 @@ __device__ __forceinline__ void sample_item(const SampleArgs &A, uint32_t item, 
                 bool found = false;
                 for (uint32_t g = b0 * kBlkU4; g < b1 * kBlkU4 && !found; ++g) {
                     const uint4 e = rp[g];
-                    const float r0 = __builtin_fmaf(dt_count(e.x), phi_at0(e.x), run);
-                    const float r1 = __builtin_fmaf(dt_count(e.y), phi_at0(e.y), r0);
-                    const float r2 = __builtin_fmaf(dt_count(e.z), phi_at0(e.z), r1);
-                    const float r3 = __builtin_fmaf(dt_count(e.w), phi_at0(e.w), r2);
-                    if (r3 > t2) {
-                        newz = dt_topic(r0 > t2 ? e.x : r1 > t2 ? e.y : r2 > t2 ? e.z : e.w);
-                        found = true;
+                    if (COMPACT) {
+                        const uint32_t w[4] = {e.x, e.y, e.z, e.w};
+#pragma unroll
+                        for (int q = 0; q < 4; ++q) {
+                            if (found) break;
+                            const float ra = __builtin_fmaf(c16_count(w[q]), phi_c16(w[q]), run);
+                            const float rb = __builtin_fmaf((float)(w[q] >> 26), phi_c16(w[q] >> 16), ra);
+                            if (ra > t2) {
+                                newz = c16_topic(w[q]);
+                                found = true;
+                            } else if (rb > t2) {
+                                newz = c16_topic(w[q] >> 16);
+                                found = true;
+                            }
+                            run = rb;
+                        }
+                    } else {
+                        const float r0 = __builtin_fmaf(dt_count(e.x), phi_at0(e.x), run);
+                        const float r1 = __builtin_fmaf(dt_count(e.y), phi_at0(e.y), r0);
+                        const float r2 = __builtin_fmaf(dt_count(e.z), phi_at0(e.z), r1);
+                        const float r3 = __builtin_fmaf(dt_count(e.w), phi_at0(e.w), r2);
+                        if (r3 > t2) {
+                            newz = dt_topic(r0 > t2 ? e.x : r1 > t2 ? e.y : r2 > t2 ? e.z : e.w);
+                            found = true;
+                        }
+                        run = r3;
                     }
-                    run = r3;
                 }
                 if (!found) {  // rounding left every running sum at or below t2
-                    const uint32_t *rr = A.dt + di.x;
-                    for (uint32_t i = 0; i < nnz; ++i)
-                        if (dt_count(rr[i]) > 0.0f && phi_at0(rr[i]) > 0.0f) newz = dt_topic(rr[i]);
+                    if (COMPACT) {
+                        const uint16_t *rr = reinterpret_cast<const uint16_t *>(A.dt) + di.x;
+                        for (uint32_t i = 0; i < nnz; ++i)
+                            if (c16_count(rr[i]) > 0.0f && phi_c16(rr[i]) > 0.0f) newz = c16_topic(rr[i]);
+                    } else {
+                        const uint32_t *rr = A.dt + di.x;
+                        for (uint32_t i = 0; i < nnz; ++i)
+                            if (dt_count(rr[i]) > 0.0f && phi_at0(rr[i]) > 0.0f) newz = dt_topic(rr[i]);
+                    }
                 }
             } else {
                 const float t2 = target - Aa;
@@ -551,7 +622,7 @@ __device__ __forceinline__ void sample_item(const SampleArgs &A, uint32_t item, 
 // n_queues (the items of one XCD when items are dealt round-robin to XCDs),
 // a workgroup drains its own queue (blockIdx mod n_queues) and then the
 // others.  Every workgroup leaves once every queue is past its last item.
-template <bool SHORTCUT>
+template <bool SHORTCUT, bool COMPACT>
 __global__ void __launch_bounds__(1024) k_sample(const SampleArgs A) {
     // the only LDS of the kernel, so phi starts at address 0 and a DT entry's
     // low half is directly the LDS address of its phi value
@@ -562,7 +633,7 @@ __global__ void __launch_bounds__(1024) k_sample(const SampleArgs A) {
     const uint32_t sweep = A.counters[0];
     uint32_t n_full = 0, n_scanned = 0;
     if (A.n_queues == 0) {
-        sample_item<SHORTCUT>(A, blockIdx.x, smf, f123, sweep, n_full, n_scanned);
+        sample_item<SHORTCUT, COMPACT>(A, blockIdx.x, smf, f123, sweep, n_full, n_scanned);
     } else {
         const uint32_t nq = A.n_queues, x0 = blockIdx.x % nq;
         for (uint32_t h = 0; h < nq; ++h) {
@@ -573,7 +644,7 @@ __global__ void __launch_bounds__(1024) k_sample(const SampleArgs A) {
                 __syncthreads();
                 const uint64_t item = (uint64_t)q + (uint64_t)s_item * nq;
                 if (item >= A.n_items) break;
-                sample_item<SHORTCUT>(A, (uint32_t)item, smf, f123, sweep, n_full, n_scanned);
+                sample_item<SHORTCUT, COMPACT>(A, (uint32_t)item, smf, f123, sweep, n_full, n_scanned);
             }
         }
     }
@@ -601,7 +672,7 @@ __global__ void __launch_bounds__(1024) k_ll(const uint4 *__restrict__ segs, con
                                              const uint32_t *__restrict__ wt_tail,
                                              const uint32_t *__restrict__ tail_nnz, const int32_t *__restrict__ nk,
                                              const uint2 *__restrict__ doc_info, const uint32_t *__restrict__ dt,
-                                             double *__restrict__ part) {
+                                             int compact, double *__restrict__ part) {
     extern __shared__ double smd[];
     __shared__ double red[16];
     const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6, nwv = blockDim.x >> 6;
@@ -642,9 +713,17 @@ __global__ void __launch_bounds__(1024) k_ll(const uint4 *__restrict__ segs, con
         const uint2 di = doc_info[rn.z];
         const uint32_t nnz = di.y & 0xFFFFu;
         double s = 0.0;
-        for (uint32_t i = lane; i < nnz; i += 64) {
-            const uint32_t e = dt[di.x + i];
-            s += (double)(e >> 16) * phi[dt_topic(e)];
+        if (compact) {
+            const uint16_t *r16 = reinterpret_cast<const uint16_t *>(dt) + di.x;
+            for (uint32_t i = lane; i < nnz; i += 64) {
+                const uint32_t e = r16[i];
+                s += (double)(e >> 10) * phi[e & 0x3FFu];
+            }
+        } else {
+            for (uint32_t i = lane; i < nnz; i += 64) {
+                const uint32_t e = dt[di.x + i];
+                s += (double)(e >> 16) * phi[dt_topic(e)];
+            }
         }
         for (int off = 32; off >= 1; off >>= 1) s += __shfl_xor(s, off);
         acc += (double)rn.w * log((s + q) / ((double)(di.y >> 16) + (double)K * alpha));
@@ -989,8 +1068,14 @@ hipError_t launch_topk(const DevState &s, hipStream_t st) {
 hipError_t launch_build_dt(const DevState &s, hipStream_t st) {
     if (s.D == 0) return hipSuccess;
     const int nwv = 4;
-    hipLaunchKernelGGL(k_dt_build, dim3(grid_for(s.D, nwv, 1 << 18)), dim3(64 * nwv), nwv * wave_hist_bytes(s.K), st,
-                       s.D, s.K, s.soff, s.slot_wrd, s.z_doc, s.wtop, s.aux_doc, s.doc_info, s.dt);
+    if (s.compact)
+        hipLaunchKernelGGL(k_dt_build<true>, dim3(grid_for(s.D, nwv, 1 << 18)), dim3(64 * nwv),
+                           nwv * wave_hist_bytes(s.K), st, s.D, s.K, s.soff, s.slot_wrd, s.z_doc, s.wtop, s.aux_doc,
+                           s.doc_info, s.dt);
+    else
+        hipLaunchKernelGGL(k_dt_build<false>, dim3(grid_for(s.D, nwv, 1 << 18)), dim3(64 * nwv),
+                           nwv * wave_hist_bytes(s.K), st, s.D, s.K, s.soff, s.slot_wrd, s.z_doc, s.wtop, s.aux_doc,
+                           s.doc_info, s.dt);
     return hipGetLastError();
 }
 
@@ -1001,11 +1086,17 @@ size_t sample_lds_bytes(int K, int bd) {
 }
 
 size_t sampler_static_lds() {
-    hipFuncAttributes a1, a0;
-    if (hipFuncGetAttributes(&a1, reinterpret_cast<const void *>(&k_sample<true>)) != hipSuccess ||
-        hipFuncGetAttributes(&a0, reinterpret_cast<const void *>(&k_sample<false>)) != hipSuccess)
-        return (size_t)-1;
-    return std::max(a1.sharedSizeBytes, a0.sharedSizeBytes);
+    const void *fs[4] = {reinterpret_cast<const void *>(&k_sample<true, true>),
+                         reinterpret_cast<const void *>(&k_sample<true, false>),
+                         reinterpret_cast<const void *>(&k_sample<false, true>),
+                         reinterpret_cast<const void *>(&k_sample<false, false>)};
+    size_t m = 0;
+    for (const void *f : fs) {
+        hipFuncAttributes a;
+        if (hipFuncGetAttributes(&a, f) != hipSuccess) return (size_t)-1;
+        m = std::max(m, a.sharedSizeBytes);
+    }
+    return m;
 }
 
 size_t max_lds_bytes(int K, int bd) {
@@ -1052,10 +1143,14 @@ hipError_t launch_sample(const DevState &s, bool shortcut, hipStream_t st) {
         hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(64), 0, st, s.counters + 8, (size_t)A.n_queues, 0u);
         grid = (unsigned)s.grid_dim;
     }
-    if (shortcut)
-        hipLaunchKernelGGL(k_sample<true>, dim3(grid), dim3(bd), lds, st, A);
+    if (shortcut && s.compact)
+        hipLaunchKernelGGL((k_sample<true, true>), dim3(grid), dim3(bd), lds, st, A);
+    else if (shortcut)
+        hipLaunchKernelGGL((k_sample<true, false>), dim3(grid), dim3(bd), lds, st, A);
+    else if (s.compact)
+        hipLaunchKernelGGL((k_sample<false, true>), dim3(grid), dim3(bd), lds, st, A);
     else
-        hipLaunchKernelGGL(k_sample<false>, dim3(grid), dim3(bd), lds, st, A);
+        hipLaunchKernelGGL((k_sample<false, false>), dim3(grid), dim3(bd), lds, st, A);
     return hipGetLastError();
 }
 
@@ -1073,7 +1168,7 @@ hipError_t launch_loglik(const DevState &s, hipStream_t st) {
     hipLaunchKernelGGL(k_ll, dim3(s.n_smp_segs), dim3(bd), (size_t)s.K * sizeof(double), st, s.smp_segs, s.runs, s.VD,
                        s.K,
                        (double)s.alpha, (double)s.beta, (double)s.V * (double)s.beta, s.wt_head, s.tail_off,
-                       s.wt_tail, s.tail_nnz, s.nk, s.doc_info, s.dt, s.ll_part);
+                       s.wt_tail, s.tail_nnz, s.nk, s.doc_info, s.dt, (int)s.compact, s.ll_part);
     hipLaunchKernelGGL(k_ll_final, dim3(1), dim3(256), 0, st, s.ll_part, s.n_smp_segs, s.ll_out);
     return hipGetLastError();
 }

@@ -44,6 +44,18 @@ void parallel_for(int64_t n, F f) {
     for (auto &x : th) x.join();
 }
 
+// DT rows: 16-bit entries for K <= kCompactK, counts in pieces of at most
+// kPiece (the oracle's ORC_COMPACT_K / ORC_PIECE).
+constexpr uint32_t kCompactK = 1024, kPiece = 63;
+
+// Entries of a doc's DT row at most: min(n, K), plus one per full piece in
+// compact rows.
+uint32_t dt_cap(uint32_t n, int K) {
+    const uint32_t m = std::min<uint32_t>(n, (uint32_t)K);
+    if ((uint32_t)K > kCompactK) return m;
+    return std::min<uint32_t>(m + n / kPiece, n);
+}
+
 // Compute units of the current device (256 on MI355X).
 int device_cus() {
     int dev = 0, n = 0;
@@ -561,24 +573,28 @@ int ezlda_load_docs(ezlda_ctx *c, int32_t V, int32_t D, const int64_t *doc_off, 
     parallel_for(N, [&](int64_t b, int64_t e, int) {
         for (int64_t p = b; p < e; ++p) z_doc[p] = z[c->slot_tok[p]];
     });
-    // DT rows in doc order q, each 128-byte aligned with capacity min(n_d, K)
-    // rounded up to 32 entries (the build zero-fills the rest)
+    // DT rows in doc order q, each 128-byte aligned (the build zero-fills
+    // the rest): K <= 1024 keeps 16-bit entries with counts in pieces of at
+    // most 63 (capacity dt_cap(n, K) rounded up to 64 entries), larger K
+    // 32-bit entries (min(n_d, K) rounded up to 32)
+    const bool compact = K <= (int)kCompactK;
     std::vector<uint2> doc_info(D);
     uint64_t dtc = 0;
     for (int q = 0; q < D; ++q) {
         const uint32_t n = c->soff[q + 1] - c->soff[q];
         doc_info[q] = make_uint2((uint32_t)dtc, n << 16);
-        dtc += (std::min<uint32_t>(n, (uint32_t)K) + 31u) & ~31u;
+        dtc += compact ? (dt_cap(n, K) + 63u) & ~63u : (dt_cap(n, K) + 31u) & ~31u;
     }
     if (dtc >= 0xFFFFFFF0ull) return fail(c, EZLDA_EINVAL, "DT capacity exceeds 2^32 entries");
     ez::DevState &s = c->s;
+    s.compact = compact;
     s.K = K; s.V = V; s.D = D; s.N = N;
     s.alpha = c->a.alpha; s.beta = c->a.beta;
     s.vbeta = (float)((double)V * (double)c->a.beta);
     s.seed = c->a.seed;
     c->dt_cap = (int64_t)dtc;
     c->dt_cap_canon = 0;
-    for (int d = 0; d < D; ++d) c->dt_cap_canon += std::min<uint32_t>(c->doc_off[d + 1] - c->doc_off[d], (uint32_t)K);
+    for (int d = 0; d < D; ++d) c->dt_cap_canon += dt_cap(c->doc_off[d + 1] - c->doc_off[d], K);
     int r = 0;
     if ((r = upload(c, &s.word_off, c->word_off))) return r;
     if ((r = upload(c, &s.z, z))) return r;
@@ -588,8 +604,9 @@ int ezlda_load_docs(ezlda_ctx *c, int32_t V, int32_t D, const int64_t *doc_off, 
     if ((r = dalloc(c, &s.aux_doc, N))) return r;
     if ((r = upload(c, &s.doc_info, doc_info))) return r;
     // 256 entries of zero padding past the last row
-    if ((r = dalloc(c, &s.dt, dtc + 256))) return r;
-    EZ_TRY_HIP(c, hipMemset(s.dt, 0, (dtc + 256) * sizeof(uint32_t)));
+    const size_t dt_words = compact ? (dtc + 256 + 1) / 2 : dtc + 256;
+    if ((r = dalloc(c, &s.dt, dt_words))) return r;
+    EZ_TRY_HIP(c, hipMemset(s.dt, 0, dt_words * sizeof(uint32_t)));
     if ((r = dalloc(c, &s.nk, K))) return r;
     if ((r = dalloc(c, &s.inv, K))) return r;
     if ((r = dalloc(c, &s.wtop, V))) return r;
@@ -1020,20 +1037,26 @@ int ezlda_get_dt_raw(ezlda_ctx *c, uint32_t *entries, int32_t *nnz) {
     if (int r = need_loaded(c)) return r;
     const ez::DevState &s = c->s;
     std::vector<uint2> di(s.D);
-    std::vector<uint32_t> dt(c->dt_cap);
+    std::vector<uint32_t> dt(s.compact ? (c->dt_cap + 1) / 2 : c->dt_cap);
     if (s.D) EZ_TRY_HIP(c, hipMemcpy(di.data(), s.doc_info, di.size() * sizeof(uint2), hipMemcpyDeviceToHost));
     if (!dt.empty()) EZ_TRY_HIP(c, hipMemcpy(dt.data(), s.dt, dt.size() * 4, hipMemcpyDeviceToHost));
+    const uint16_t *dt16 = reinterpret_cast<const uint16_t *>(dt.data());
     std::vector<int32_t> qof(s.D);
     for (int q = 0; q < s.D; ++q) qof[c->dord[q]] = q;
     uint64_t canon = 0;  // rows packed in input doc order at sum_{e<d} min(n_e, K)
     for (int d = 0; d < s.D; ++d) {
         const uint2 x = di[qof[d]];
         nnz[d] = (int32_t)(x.y & 0xFFFFu);
-        for (int32_t i = 0; i < nnz[d]; ++i) {  // device rows hold topic << 2
-            const uint32_t e = dt[x.x + i];
-            entries[canon + i] = ((e & 0xFFFFu) >> 2) | (e & 0xFFFF0000u);
+        for (int32_t i = 0; i < nnz[d]; ++i) {  // device rows hold topic << 2, or 16-bit entries
+            if (s.compact) {
+                const uint32_t e = dt16[x.x + i];
+                entries[canon + i] = (e & 0x3FFu) | ((e >> 10) << 16);
+            } else {
+                const uint32_t e = dt[x.x + i];
+                entries[canon + i] = ((e & 0xFFFFu) >> 2) | (e & 0xFFFF0000u);
+            }
         }
-        canon += std::min<uint32_t>(c->doc_off[d + 1] - c->doc_off[d], (uint32_t)s.K);
+        canon += dt_cap(c->doc_off[d + 1] - c->doc_off[d], s.K);
     }
     return EZLDA_OK;
 }

@@ -491,7 +491,18 @@ def selftest_copy(nbytes: int, reps: int = 10) -> float:
     return ms.value
 
 
+COMPACT_K, PIECE = 1024, 63
+
+
+def dt_capacity(n: np.ndarray, K: int) -> np.ndarray:
+    """Entries of a doc's DT row at most: min(n, K), plus one per full piece
+    of 63 when K <= 1024 (rows then store counts in pieces of at most 63)."""
+    n = np.asarray(n, dtype=np.int64)
+    m = np.minimum(n, K)
+    return m if K > COMPACT_K else np.minimum(m + n // PIECE, n)
+
+
 def dt_offsets(doc_off: Sequence[int], K: int) -> np.ndarray:
-    """DT row offsets: sum over earlier docs of min(n_d, K)."""
+    """Row offsets of the DT getters: sum over earlier docs of the capacity."""
     n = np.diff(np.asarray(doc_off, dtype=np.int64))
-    return np.concatenate([[0], np.cumsum(np.minimum(n, K))])
+    return np.concatenate([[0], np.cumsum(dt_capacity(n, K))])

@@ -165,8 +165,10 @@ class Oracle:
         return o
 
     def get_dt_raw(self):
-        n = np.diff(self.doc_off)
-        cap = int(np.minimum(n, self.K).sum())
+        n = np.diff(self.doc_off).astype(np.int64)
+        m = np.minimum(n, self.K)
+        # dt_cap() of the oracle: compact rows (K <= 1024) add one entry per full piece of 63
+        cap = int((m if self.K > 1024 else np.minimum(m + n // 63, n)).sum())
         ent = np.zeros(max(cap, 1), np.uint32)
         nnz = np.zeros(max(self.D, 1), np.int32)
         self.L.orc_get_dt_raw(self.h, _p(ent), _p(nnz))

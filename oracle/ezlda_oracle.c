@@ -64,6 +64,20 @@
 
 #define ORC_MARGIN 0.998046875f /* 1 - 2^-9 */
 #define ORC_NONE 0xFFFFu
+/* Compact DT rows (K <= ORC_COMPACT_K): a topic's count is stored in pieces
+ * of at most ORC_PIECE, one entry per piece, in order; every S sum and
+ * running sum walks the pieces.  Larger K keeps one entry per topic. */
+#define ORC_COMPACT_K 1024
+#define ORC_PIECE 63
+
+/* Entries of a doc's DT row at most: min(n, K), plus one per full piece in
+ * compact rows. */
+static int64_t dt_cap(int64_t n, int K) {
+    const int64_t m = n < K ? n : K;
+    if (K > ORC_COMPACT_K) return m;
+    const int64_t c = m + n / ORC_PIECE;
+    return c < n ? c : n;
+}
 
 /* ---------------------------------------------------------------- Philox */
 /* Salmon et al., SC'11, Philox4x32 with 10 rounds (Random123 constants). */
@@ -203,7 +217,7 @@ void *orc_create(int K, float alpha, float beta, uint64_t seed, int V, int D,
     o->dt_off = (int64_t *)xcalloc(D + 1, sizeof(int64_t));
     for (int d = 0; d < D; ++d) {
         int64_t n = doc_off[d + 1] - doc_off[d];
-        o->dt_off[d + 1] = o->dt_off[d] + (n < K ? n : K);
+        o->dt_off[d + 1] = o->dt_off[d] + dt_cap(n, K);
     }
     o->dt = (uint32_t *)xcalloc(o->dt_off[D], sizeof(uint32_t));
     o->dt_nnz = (int32_t *)xcalloc(D, sizeof(int32_t));
@@ -292,7 +306,9 @@ void orc_topk(void *h) {
 /* ---------------------------------------- 4. DT rows and per-token aux */
 /* DT_Update_Kernel, utility.cu:79-284: histogram of the doc's topics,
  * per-token (n_dk1, n_dk2) re-pack (:125-161), ascending-topic compaction
- * (:183-238). */
+ * (:183-238); for K <= ORC_COMPACT_K a count above ORC_PIECE becomes
+ * pieces of ORC_PIECE and a remainder (the reference packs the count in 16
+ * bits and never splits). */
 void orc_build_dt(void *h) {
     orc_t *o = (orc_t *)h;
     const int K = o->K;
@@ -312,7 +328,10 @@ void orc_build_dt(void *h) {
             uint32_t *row = o->dt + o->dt_off[d];
             for (int k = 0; k < K; ++k)
                 if (hist[k]) {
-                    row[nz++] = (uint32_t)k | ((uint32_t)hist[k] << 16);
+                    uint32_t c = (uint32_t)hist[k];
+                    if (K <= ORC_COMPACT_K)
+                        for (; c > ORC_PIECE; c -= ORC_PIECE) row[nz++] = (uint32_t)k | ((uint32_t)ORC_PIECE << 16);
+                    row[nz++] = (uint32_t)k | (c << 16);
                     hist[k] = 0;
                 }
             o->dt_nnz[d] = nz;
@@ -614,7 +633,7 @@ void orc_get_dt_dense(void *h, int32_t *out) {
     for (int d = 0; d < o->D; ++d)
         for (int i = 0; i < o->dt_nnz[d]; ++i) {
             uint32_t e = o->dt[o->dt_off[d] + i];
-            out[(size_t)d * o->K + (e & 0xFFFFu)] = (int32_t)(e >> 16);
+            out[(size_t)d * o->K + (e & 0xFFFFu)] += (int32_t)(e >> 16);
         }
 }
 /* Packed DT rows exactly as stored: out has dt_off[D] entries; nnz per doc. */

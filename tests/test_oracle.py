@@ -14,6 +14,7 @@ import numpy as np
 import pytest
 
 from conftest import ROOT, make_corpus
+from lda_sc20_amd import ezlda
 from oracle import binding as ob
 
 ALPHA = lambda K: 50.0 / K  # noqa: E731
@@ -112,14 +113,45 @@ def test_counts_bruteforce():
     assert np.array_equal(o.get_wt_dense(), wt)
     assert np.array_equal(o.get_nk(), wt.sum(0))
     assert np.array_equal(o.get_dt_dense(), dt)
+    _check_rows(o, off, K)
+
+
+def _check_rows(o, off, K):
+    """Rows hold each topic once in ascending order (compact rows: in pieces
+    of 63 and a remainder), counts summing to n_d, within the capacity."""
     ent, nnz = o.get_dt_raw()
-    doff = np.concatenate([[0], np.cumsum(np.minimum(np.diff(off), K))])
+    doff = ezlda.dt_offsets(off, K)
+    dense = o.get_dt_dense()
     for d in range(len(off) - 1):
         row = ent[doff[d]: doff[d] + nnz[d]]
-        tk, ct = row & 0xFFFF, row >> 16
-        assert np.all(np.diff(tk.astype(np.int64)) > 0) and np.all(ct > 0)
-        assert nnz[d] <= min(off[d + 1] - off[d], K)
+        tk, ct = (row & 0xFFFF).astype(np.int64), (row >> 16).astype(np.int64)
+        assert np.all(np.diff(tk) >= 0) and np.all(ct > 0)
+        assert nnz[d] <= doff[d + 1] - doff[d]
         assert ct.sum() == off[d + 1] - off[d]
+        exp = []
+        for k in np.nonzero(dense[d])[0]:
+            c = int(dense[d, k])
+            if K <= ezlda.COMPACT_K:
+                exp += [(k, 63)] * ((c - 1) // 63) + [(k, c - 63 * ((c - 1) // 63))]
+            else:
+                exp.append((k, c))
+        assert list(zip(tk.tolist(), ct.tolist())) == exp
+
+
+@pytest.mark.parametrize("K", [2, 5, 1025])
+def test_dt_rows_split_large_counts(K):
+    # long docs over few topics: counts far above 63 (split when K <= 1024)
+    off, words = make_corpus(7, 12, 30, 600)
+    o = ob.Oracle(K, float(np.float32(ALPHA(K))), BETA, 3, 30, off, words,
+                  z0=(np.arange(len(words)) % 2).astype(np.uint16))
+    o.build_wt()
+    o.build_dt()
+    assert o.get_dt_dense().max() > 130
+    _check_rows(o, off, K)
+    o.sweep(2)
+    o.build_wt()
+    o.build_dt()
+    _check_rows(o, off, K)
 
 
 def _phi32(wt, nk, V, K):
