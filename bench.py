@@ -312,21 +312,23 @@ def main() -> int:
     value = N * K_steps / (total_ms / 1e3)
     log(f"timed {K_steps} sweeps: {total_ms:.2f} ms (wall {wall * 1e3:.1f} ms), stages {stage_ms}")
 
-    # algorithmic bytes of one sampler launch: the z write per token; per run
-    # (a word's tokens in one doc) its entry, token->doc id, max-topic counts
-    # and doc info (8 + 4 + 4 + 8); phi staging once per word; one read of
-    # every scanned DT entry (one scan per run with a full draw)
+    # algorithmic bytes of one sampler launch: the two z writes per token (by
+    # token and by doc slot); per run (a word's tokens in one doc) its entry,
+    # max-topic counts and doc info (16 + 4 + 8); phi staging once per word;
+    # one read of every scanned DT entry (one scan per run with a full draw;
+    # 2-byte entries when K <= 1024, else 4)
     VD = int(info.num_head_words)
     R = int(info.num_runs)
+    eb = 2 if K <= 1024 else 4
     s_scanned = n_scanned / K_steps
-    b_sample = 2 * N + 24 * R + 4 * K * VD + 4 * tail_nnz + 4 * K + 4 * s_scanned
+    b_sample = 4 * N + 28 * R + 4 * K * VD + 4 * tail_nnz + 4 * K + eb * s_scanned
     t_sample = stage_ms["sample"] / K_steps / 1e3
     achieved = b_sample / t_sample / 1e9
     # whole-sweep algorithmic bytes (same accounting for every stage)
     b_sweep = (b_sample                                                      # sample
                + 2 * N + 2 * N + 4 * K * VD + 4 * tail_nnz                  # n_k + WT rebuild
                + 4 * K * VD + 4 * tail_nnz + 8 * info.num_words             # top-3
-               + 8 * N + 2 * N + 4 * N + 4 * dt_nnz + 8 * info.num_docs)    # DT + aux
+               + 2 * N + 4 * N + 4 * N + eb * dt_nnz + 8 * info.num_docs)   # DT + aux (z_doc, word, aux)
     graph_ms = None
     if not args.no_graph:
         gms, _ = ctx.time_sweeps(K_steps, use_graph=True)
