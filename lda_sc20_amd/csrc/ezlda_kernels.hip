@@ -20,7 +20,7 @@
 #define EZ_SAMPLE_BLK 4
 #endif
 #ifndef EZ_SAMPLE_CP
-#define EZ_SAMPLE_CP 32
+#define EZ_SAMPLE_CP 12
 #endif
 
 namespace ez {
@@ -340,11 +340,20 @@ static constexpr int kBlkU4 = EZ_SAMPLE_BLK;
 
 // Running FMA sum over one block of a DT row: kBlkU4 16-byte loads issued
 // together, then the entries in order.
-template <bool COMPACT>
-__device__ __forceinline__ float block_sum(const uint4 *__restrict__ p, float s) {
+struct RowBlock {
     uint4 e[kBlkU4];
+};
+
+__device__ __forceinline__ RowBlock load_block(const uint4 *__restrict__ p) {
+    RowBlock r;
 #pragma unroll
-    for (int i = 0; i < kBlkU4; ++i) e[i] = p[i];
+    for (int i = 0; i < kBlkU4; ++i) r.e[i] = p[i];
+    return r;
+}
+
+template <bool COMPACT>
+__device__ __forceinline__ float sum_block(const RowBlock &r, float s) {
+    const uint4 *e = r.e;
 #pragma unroll
     for (int i = 0; i < kBlkU4; ++i) {
         if (COMPACT) {  // two 16-bit entries per dword, low half first
@@ -496,14 +505,25 @@ __device__ __forceinline__ void sample_item(const SampleArgs &A, uint32_t item, 
         // S: one FMA per entry, left to right; cp[b] = running sum at the
         // end of block b (the first kCP blocks), so the S-bucket search
         // re-reads one block instead of the row up to the target
+        // Block b + 1 is loaded before block b is summed (only by lanes
+        // whose row has it, so a wavefront stops loading at its longest row).
         float cp[kCP];
         float S = 0.0f;
+        RowBlock cur = load_block(rp);
 #pragma unroll
         for (int b = 0; b < kCP; ++b) {
-            if ((uint32_t)b < nb) S = block_sum<COMPACT>(rp + kBlkU4 * b, S);
+            RowBlock nxt;
+            if ((uint32_t)b + 1u < nb) nxt = load_block(rp + kBlkU4 * (b + 1));
+            if ((uint32_t)b < nb) S = sum_block<COMPACT>(cur, S);
             cp[b] = S;
+            cur = nxt;
         }
-        for (uint32_t b = kCP; b < nb; ++b) S = block_sum<COMPACT>(rp + kBlkU4 * b, S);
+        for (uint32_t b = kCP; b < nb; ++b) {
+            RowBlock nxt;
+            if (b + 1u < nb) nxt = load_block(rp + kBlkU4 * (b + 1u));
+            S = sum_block<COMPACT>(cur, S);
+            cur = nxt;
+        }
         const float Aa = P1 + S;
         const float total = Aa + Q;
         while (need) {
