@@ -21,7 +21,8 @@ struct DevState {
     // [soff[q], soff[q+1]), its tokens grouped by word rank, then position
     uint32_t *soff = nullptr;       // D+1
     uint32_t *slot_wrd = nullptr;   // N: word rank
-    uint16_t *z_doc = nullptr;      // N: z by slot, written with z by the sampler
+    uint32_t *slot_tok = nullptr;   // N: token id of the slot
+    uint16_t *z_doc = nullptr;      // N: z by slot, kept by the sampler for head-word slots
     uint32_t *aux_doc = nullptr;    // N: n_dk1 | n_dk2 << 16 of the slot's word
     // doc-topic rows
     uint2 *doc_info = nullptr;      // D, by doc q: x = DT row offset, y = nnz | len << 16
@@ -55,6 +56,7 @@ hipError_t launch_zero_counts(const DevState &s, hipStream_t st);
 hipError_t launch_build_wt(const DevState &s, hipStream_t st);
 hipError_t launch_topk(const DevState &s, hipStream_t st);
 hipError_t launch_build_dt(const DevState &s, hipStream_t st);
+hipError_t launch_z_to_doc(const DevState &s, hipStream_t st);
 hipError_t launch_sample(const DevState &s, bool shortcut, hipStream_t st);
 hipError_t launch_advance_sweep(const DevState &s, hipStream_t st);
 hipError_t launch_loglik(const DevState &s, hipStream_t st);

@@ -256,6 +256,20 @@ __global__ void k_word_top(int V, int VD, int K, float beta, const int32_t *__re
     }
 }
 
+// Doc-slot copy of z from z (after a layout change: slots of words that
+// were tail words hold stale topics there).
+__global__ void k_z_to_doc(uint32_t N, const uint32_t *__restrict__ slot_tok, const uint16_t *__restrict__ z,
+                           uint16_t *__restrict__ z_doc) {
+    for (uint32_t p = blockIdx.x * blockDim.x + threadIdx.x; p < N; p += gridDim.x * blockDim.x)
+        z_doc[p] = z[slot_tok[p]];
+}
+
+hipError_t launch_z_to_doc(const DevState &s, hipStream_t st) {
+    if (s.N == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_z_to_doc, dim3(grid_for(s.N, 256, 1 << 16)), dim3(256), 0, st, s.N, s.slot_tok, s.z, s.z_doc);
+    return hipGetLastError();
+}
+
 // --------------------------------------------------- DT rows and aux
 // DT_Update_Kernel (utility.cu:79-284), on the doc-major copy of z: doc q
 // (docs in descending length order) owns slots [soff[q], soff[q+1]), its
@@ -266,8 +280,10 @@ __global__ void k_word_top(int V, int VD, int K, float beta, const int32_t *__re
 // ascending topic order (:183-238).  A row starts on a 128-byte boundary and
 // is zero-padded to a multiple of 32 entries.
 template <bool COMPACT>
-__global__ void k_dt_build(int D, int K, const uint32_t *__restrict__ soff, const uint32_t *__restrict__ slot_wrd,
-                           const uint16_t *__restrict__ z_doc, const ushort4 *__restrict__ wtop,
+__global__ void k_dt_build(int D, int K, int VD, const uint32_t *__restrict__ soff,
+                           const uint32_t *__restrict__ slot_wrd, const uint32_t *__restrict__ slot_tok,
+                           const uint16_t *__restrict__ z, const uint16_t *__restrict__ z_doc,
+                           const ushort4 *__restrict__ wtop,
                            uint32_t *__restrict__ aux_doc, uint2 *__restrict__ doc_info, uint32_t *__restrict__ dt) {
     extern __shared__ uint32_t sh[];
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6, nwv = blockDim.x >> 6;
@@ -277,7 +293,8 @@ __global__ void k_dt_build(int D, int K, const uint32_t *__restrict__ soff, cons
     wave_lds_sync();
     for (int q = blockIdx.x * nwv + wv; q < D; q += gridDim.x * nwv) {
         const uint32_t b = soff[q], e = soff[q + 1];
-        for (uint32_t i = b + lane; i < e; i += 64) h16_add(h, z_doc[i]);
+        for (uint32_t i = b + lane; i < e; i += 64)
+            h16_add(h, slot_wrd[i] < (uint32_t)VD ? z_doc[i] : z[slot_tok[i]]);
         wave_lds_sync();
         for (uint32_t i = b + lane; i < e; i += 64) {
             const ushort4 tp = wtop[slot_wrd[i]];
@@ -466,6 +483,10 @@ __device__ __forceinline__ void sample_item(const SampleArgs &A, uint32_t item, 
     const float Q = qacc[NS - 1];
     const float f1 = f123[0], f2 = f123[1], f3 = f123[2];
     const uint32_t k1 = tp.x;
+    // head words also write the doc-slot copy of z (their items share a doc
+    // tile, so the copy's lines fill in L2); tail tokens, in random docs,
+    // are read back by token id in the DT rebuild instead
+    const bool head = w < A.VD;
 
     for (uint32_t base = sg.y + (uint32_t)wv * 64u; base < sg.z; base += (uint32_t)nwv * 64u) {
         const uint32_t r = base + (uint32_t)lane;
@@ -490,7 +511,7 @@ __device__ __forceinline__ void sample_item(const SampleArgs &A, uint32_t item, 
                 need |= 1u << j;
             } else {
                 A.z[t0 + j] = (uint16_t)k1;
-                A.z_doc[p0 + j] = (uint16_t)k1;
+                if (head) A.z_doc[p0 + j] = (uint16_t)k1;
             }
         }
         if (!need) continue;
@@ -632,7 +653,7 @@ __device__ __forceinline__ void sample_item(const SampleArgs &A, uint32_t item, 
                 }
             }
             A.z[t] = (uint16_t)newz;
-            A.z_doc[p0 + j] = (uint16_t)newz;
+            if (head) A.z_doc[p0 + j] = (uint16_t)newz;
         }
     }
 }
@@ -1090,12 +1111,12 @@ hipError_t launch_build_dt(const DevState &s, hipStream_t st) {
     const int nwv = 4;
     if (s.compact)
         hipLaunchKernelGGL(k_dt_build<true>, dim3(grid_for(s.D, nwv, 1 << 18)), dim3(64 * nwv),
-                           nwv * wave_hist_bytes(s.K), st, s.D, s.K, s.soff, s.slot_wrd, s.z_doc, s.wtop, s.aux_doc,
-                           s.doc_info, s.dt);
+                           nwv * wave_hist_bytes(s.K), st, s.D, s.K, s.VD, s.soff, s.slot_wrd, s.slot_tok, s.z,
+                           s.z_doc, s.wtop, s.aux_doc, s.doc_info, s.dt);
     else
         hipLaunchKernelGGL(k_dt_build<false>, dim3(grid_for(s.D, nwv, 1 << 18)), dim3(64 * nwv),
-                           nwv * wave_hist_bytes(s.K), st, s.D, s.K, s.soff, s.slot_wrd, s.z_doc, s.wtop, s.aux_doc,
-                           s.doc_info, s.dt);
+                           nwv * wave_hist_bytes(s.K), st, s.D, s.K, s.VD, s.soff, s.slot_wrd, s.slot_tok, s.z,
+                           s.z_doc, s.wtop, s.aux_doc, s.doc_info, s.dt);
     return hipGetLastError();
 }
 

@@ -345,6 +345,11 @@ int build_layout(ezlda_ctx *c, const std::vector<uint32_t> &tok_doc, const std::
     s.grid_dim = c->a.grid_dim;
     c->tail_cap = tail_off.back();
     c->counts_fresh = false;
+    // the sampler keeps z_doc only for head words: bring every slot up to date
+    if (s.z_doc) {
+        EZ_TRY_HIP(c, ez::launch_z_to_doc(s, c->st));
+        EZ_TRY_HIP(c, hipStreamSynchronize(c->st));
+    }
     return EZLDA_OK;
 }
 
@@ -600,6 +605,7 @@ int ezlda_load_docs(ezlda_ctx *c, int32_t V, int32_t D, const int64_t *doc_off, 
     if ((r = upload(c, &s.z, z))) return r;
     if ((r = upload(c, &s.soff, c->soff))) return r;
     if ((r = upload(c, &s.slot_wrd, slot_wrd))) return r;
+    if ((r = upload(c, &s.slot_tok, c->slot_tok))) return r;
     if ((r = upload(c, &s.z_doc, z_doc))) return r;
     if ((r = dalloc(c, &s.aux_doc, N))) return r;
     if ((r = upload(c, &s.doc_info, doc_info))) return r;
