@@ -156,7 +156,13 @@ void pool_free(ezlda_server *s, uint32_t off, uint32_t n) {
 
 // Worker: admit, step, retire, until stopped.
 void worker_main(ezlda_server *s) {
-    hipSetDevice(s->device);
+    if (hipSetDevice(s->device) != hipSuccess) {
+        std::lock_guard<std::mutex> lk(s->mu);
+        s->worker_rc = EZLDA_EHIP;
+        s->err = "fold-in worker: hipSetDevice failed";
+        s->cv_done.notify_all();
+        return;
+    }
     std::vector<int> act;
     std::vector<uint16_t> zbuf;
     while (true) {
@@ -177,13 +183,13 @@ void worker_main(ezlda_server *s) {
                 fs->n = n;
                 fs->sweep = 0;
                 fs->t_submit = rq.t_submit;
-                if (n && hipMemcpyAsync(s->wpool + off, rq.words.data(), n * sizeof(uint32_t), hipMemcpyHostToDevice,
-                                        s->st) != hipSuccess) {
+                // the pageable source must stay valid until the copy is done
+                if (n && (hipMemcpyAsync(s->wpool + off, rq.words.data(), n * sizeof(uint32_t), hipMemcpyHostToDevice,
+                                         s->st) != hipSuccess ||
+                          hipStreamSynchronize(s->st) != hipSuccess)) {
                     s->worker_rc = EZLDA_EHIP;
                     s->err = "fold-in upload failed";
                 }
-                // the pageable source must stay valid until the copy is done
-                if (n) hipStreamSynchronize(s->st);
             }
             act.clear();
             for (int i = 0; i < s->max_slots; ++i)
@@ -222,9 +228,16 @@ void worker_main(ezlda_server *s) {
             r.z.resize(x.n);
             r.counts.resize(s->K);
             lk.unlock();
-            if (x.n) hipMemcpy(r.z.data(), s->zpool + x.off, x.n * sizeof(uint16_t), hipMemcpyDeviceToHost);
-            hipMemcpy(r.counts.data(), s->counts + j * (size_t)s->K, s->K * sizeof(int32_t), hipMemcpyDeviceToHost);
+            hipError_t ce = hipSuccess;
+            if (x.n) ce = hipMemcpy(r.z.data(), s->zpool + x.off, x.n * sizeof(uint16_t), hipMemcpyDeviceToHost);
+            if (ce == hipSuccess)
+                ce = hipMemcpy(r.counts.data(), s->counts + j * (size_t)s->K, s->K * sizeof(int32_t),
+                               hipMemcpyDeviceToHost);
             lk.lock();
+            if (ce != hipSuccess) {
+                s->worker_rc = EZLDA_EHIP;
+                s->err = std::string("fold-in result copy failed: ") + hipGetErrorString(ce);
+            }
             const double t = now_ms();
             r.latency_ms = t - x.t_submit;
             s->lat.push_back(r.latency_ms);
@@ -250,7 +263,7 @@ ezlda_server *ezlda_server_create(ezlda_ctx *ctx, const ezlda_server_args *a) {
     }
     if (ezc::refresh_counts(ctx) != EZLDA_OK) return nullptr;
     auto *s = new ezlda_server();
-    hipGetDevice(&s->device);
+    if (hipGetDevice(&s->device) != hipSuccess) s->device = 0;
     s->K = ctx->s.K;
     s->V = ctx->s.V;
     s->NS = (s->K + 63) / 64;
@@ -295,12 +308,13 @@ void ezlda_server_destroy(ezlda_server *s) {
         s->cv_work.notify_all();
         s->worker.join();
     }
-    if (s->st) hipStreamSynchronize(s->st);
-    if (s->pst) hipStreamSynchronize(s->pst);
-    for (void *p : s->allocs) hipFree(p);
-    if (s->hslots) hipHostFree(s->hslots);
-    if (s->st) hipStreamDestroy(s->st);
-    if (s->pst) hipStreamDestroy(s->pst);
+    // release: nothing to do about a failure here
+    if (s->st) (void)hipStreamSynchronize(s->st);
+    if (s->pst) (void)hipStreamSynchronize(s->pst);
+    for (void *p : s->allocs) (void)hipFree(p);
+    if (s->hslots) (void)hipHostFree(s->hslots);
+    if (s->st) (void)hipStreamDestroy(s->st);
+    if (s->pst) (void)hipStreamDestroy(s->pst);
     delete s;
 }
 

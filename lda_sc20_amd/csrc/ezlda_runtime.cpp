@@ -118,13 +118,14 @@ int upload(ezlda_ctx *c, T **p, const std::vector<T> &v) {
 }
 
 void free_device(ezlda_ctx *c) {
-    if (c->graph_exec) hipGraphExecDestroy(c->graph_exec);
-    if (c->graph) hipGraphDestroy(c->graph);
+    // release: nothing to do about a failure here
+    if (c->graph_exec) (void)hipGraphExecDestroy(c->graph_exec);
+    if (c->graph) (void)hipGraphDestroy(c->graph);
     c->graph_exec = nullptr;
     c->graph = nullptr;
-    for (void *p : c->allocs) hipFree(p);
+    for (void *p : c->allocs) (void)hipFree(p);
     c->allocs.clear();
-    for (void *p : c->layout_allocs) hipFree(p);
+    for (void *p : c->layout_allocs) (void)hipFree(p);
     c->layout_allocs.clear();
     c->dev_bytes = 0;
     c->layout_bytes = 0;
@@ -295,12 +296,12 @@ int build_layout(ezlda_ctx *c, const std::vector<uint32_t> &tok_doc, const std::
                  const std::vector<uint32_t> &slot_wrd) {
     ez::DevState &s = c->s;
     const int V = s.V, K = s.K, D = s.D;
-    if (c->graph_exec) hipGraphExecDestroy(c->graph_exec);
-    if (c->graph) hipGraphDestroy(c->graph);
+    if (c->graph_exec) (void)hipGraphExecDestroy(c->graph_exec);
+    if (c->graph) (void)hipGraphDestroy(c->graph);
     c->graph_exec = nullptr;
     c->graph = nullptr;
     EZ_TRY_HIP(c, hipStreamSynchronize(c->st));
-    for (void *p : c->layout_allocs) hipFree(p);
+    for (void *p : c->layout_allocs) (void)hipFree(p);
     c->layout_allocs.clear();
     c->layout_bytes = 0;
     // head / tail split (words by rank: count desc)
@@ -452,9 +453,9 @@ ezlda_ctx *ezlda_create(const ezlda_args *args) {
 
 void ezlda_destroy(ezlda_ctx *c) {
     if (!c) return;
-    if (c->st) hipStreamSynchronize(c->st);
+    if (c->st) (void)hipStreamSynchronize(c->st);
     free_device(c);
-    if (c->st) hipStreamDestroy(c->st);
+    if (c->st) (void)hipStreamDestroy(c->st);
     delete c;
 }
 
@@ -728,7 +729,7 @@ static int ensure_graph(ezlda_ctx *c) {
     hipGraph_t g = nullptr;
     hipError_t e = hipStreamEndCapture(c->st, &g);
     if (r) {
-        if (g) hipGraphDestroy(g);
+        if (g) (void)hipGraphDestroy(g);
         return r;
     }
     if (e != hipSuccess) return fail(c, EZLDA_EHIP, std::string("hipStreamEndCapture: ") + hipGetErrorString(e));
@@ -894,12 +895,13 @@ int ezlda_time_sweeps(ezlda_ctx *c, int32_t n, int32_t use_graph, double *total_
         if (hipEventCreate(&e) != hipSuccess) rc = fail(c, EZLDA_EHIP, "hipEventCreate failed");
     if (rc == EZLDA_OK) {
         if (use_graph) {
-            hipEventRecord(ev[0], c->st);
+            if (hipEventRecord(ev[0], c->st) != hipSuccess) rc = fail(c, EZLDA_EHIP, "hipEventRecord failed");
             for (int i = 0; i < n && rc == EZLDA_OK; ++i) {
                 if (hipGraphLaunch(c->graph_exec, c->st) != hipSuccess) rc = fail(c, EZLDA_EHIP, "hipGraphLaunch failed");
                 c->sweep++;
             }
-            hipEventRecord(ev[1], c->st);
+            if (rc == EZLDA_OK && hipEventRecord(ev[1], c->st) != hipSuccess)
+                rc = fail(c, EZLDA_EHIP, "hipEventRecord failed");
         } else {
             for (int i = 0; i < n && rc == EZLDA_OK; ++i) {
                 rc = enqueue_sweep(c, &ev[(size_t)i * per]);
@@ -911,21 +913,23 @@ int ezlda_time_sweeps(ezlda_ctx *c, int32_t n, int32_t use_graph, double *total_
     if (rc == EZLDA_OK) {
         float ms = 0.f;
         if (use_graph) {
-            hipEventElapsedTime(&ms, ev[0], ev[1]);
+            if (hipEventElapsedTime(&ms, ev[0], ev[1]) != hipSuccess) rc = fail(c, EZLDA_EHIP, "hipEventElapsedTime failed");
             tot = ms;
             for (auto &x : st) x = -1.0;
         } else {
-            hipEventElapsedTime(&ms, ev[0], ev[nev - 1]);
+            if (hipEventElapsedTime(&ms, ev[0], ev[nev - 1]) != hipSuccess)
+                rc = fail(c, EZLDA_EHIP, "hipEventElapsedTime failed");
             tot = ms;
             for (int i = 0; i < n; ++i)
                 for (int k = 0; k < EZLDA_NUM_STAGES; ++k) {
-                    hipEventElapsedTime(&ms, ev[(size_t)i * per + k], ev[(size_t)i * per + k + 1]);
+                    if (hipEventElapsedTime(&ms, ev[(size_t)i * per + k], ev[(size_t)i * per + k + 1]) != hipSuccess)
+                        rc = fail(c, EZLDA_EHIP, "hipEventElapsedTime failed");
                     st[k] += ms;
                 }
         }
     }
     for (auto &e : ev)
-        if (e) hipEventDestroy(e);
+        if (e) (void)hipEventDestroy(e);
     c->counts_fresh = false;
     if (rc) return rc;
     if (total_ms) *total_ms = tot;
@@ -1099,9 +1103,9 @@ int ezlda_selftest_philox(const uint32_t *ctr, const uint32_t *key, uint32_t *ou
     EZ_TRY_HIP(nullptr, hipMemcpy(dk, key, (size_t)n * 8, hipMemcpyHostToDevice));
     EZ_TRY_HIP(nullptr, ez::launch_selftest_philox(dc, dk, dout, n, nullptr));
     EZ_TRY_HIP(nullptr, hipMemcpy(out, dout, (size_t)n * 16, hipMemcpyDeviceToHost));
-    hipFree(dc);
-    hipFree(dk);
-    hipFree(dout);
+    (void)hipFree(dc);
+    (void)hipFree(dk);
+    (void)hipFree(dout);
     return EZLDA_OK;
 }
 
@@ -1122,12 +1126,13 @@ int ezlda_selftest_copy(int64_t bytes, int32_t reps, double *ms) {
     EZ_TRY_HIP(nullptr, hipEventRecord(e1, nullptr));
     EZ_TRY_HIP(nullptr, hipEventSynchronize(e1));
     float t = 0.f;
-    hipEventElapsedTime(&t, e0, e1);
+    const hipError_t et = hipEventElapsedTime(&t, e0, e1);
     if (ms) *ms = t / reps;
-    hipEventDestroy(e0);
-    hipEventDestroy(e1);
-    hipFree(a);
-    hipFree(b);
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    (void)hipFree(a);
+    (void)hipFree(b);
+    if (et != hipSuccess) return fail(nullptr, EZLDA_EHIP, "hipEventElapsedTime failed");
     return EZLDA_OK;
 }
 
