@@ -54,6 +54,8 @@ static int grid_for(size_t n, int bd, int cap) {
 }
 
 static size_t wave_hist_bytes(int K) { return (size_t)((K + 1) >> 1) * sizeof(uint32_t); }
+// DT rebuild: 16-bit counts plus a topic bitmap per wavefront
+static size_t dt_wave_bytes(int K) { return wave_hist_bytes(K) + (size_t)((K + 31) >> 5) * sizeof(uint32_t); }
 
 // A DT entry is (topic << 2) | (count << 16): its low half is the byte
 // offset of phi[topic] in a float array, so a gather needs one AND.
@@ -287,14 +289,18 @@ __global__ void k_dt_build(int D, int K, int VD, const uint32_t *__restrict__ so
                            uint32_t *__restrict__ aux_doc, uint2 *__restrict__ doc_info, uint32_t *__restrict__ dt) {
     extern __shared__ uint32_t sh[];
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6, nwv = blockDim.x >> 6;
-    const int KH = (K + 1) >> 1;
-    uint32_t *h = sh + wv * KH;
-    for (int k = lane; k < KH; k += 64) h[k] = 0;
+    const int KH = (K + 1) >> 1, NW = (K + 31) >> 5;
+    uint32_t *h = sh + wv * (KH + NW);  // 16-bit counts
+    uint32_t *bm = h + KH;              // bitmap of the doc's topics
+    for (int k = lane; k < KH + NW; k += 64) h[k] = 0;
     wave_lds_sync();
     for (int q = blockIdx.x * nwv + wv; q < D; q += gridDim.x * nwv) {
         const uint32_t b = soff[q], e = soff[q + 1];
-        for (uint32_t i = b + lane; i < e; i += 64)
-            h16_add(h, slot_wrd[i] < (uint32_t)VD ? z_doc[i] : z[slot_tok[i]]);
+        for (uint32_t i = b + lane; i < e; i += 64) {
+            const uint32_t t = slot_wrd[i] < (uint32_t)VD ? z_doc[i] : z[slot_tok[i]];
+            h16_add(h, t);
+            atomicOr(&bm[t >> 5], 1u << (t & 31u));
+        }
         wave_lds_sync();
         for (uint32_t i = b + lane; i < e; i += 64) {
             const ushort4 tp = wtop[slot_wrd[i]];
@@ -303,45 +309,57 @@ __global__ void k_dt_build(int D, int K, int VD, const uint32_t *__restrict__ so
             aux_doc[i] = n1 | (n2 << 16);
         }
         wave_lds_sync();
+        // Compaction over the bitmap: lane j owns topics [32j, 32j + 32) of
+        // each 64-word chunk, counts its entries, takes its place from a
+        // wave prefix sum and writes them in ascending topic order; then it
+        // clears its counts and bitmap word.  Cost follows the doc's topics,
+        // not K.
         const uint32_t base = doc_info[q].x;
         uint32_t nz = 0;
+        for (int j = 0; j < NW; j += 64) {
+            const int wi = j + lane;
+            const uint32_t word = wi < NW ? bm[wi] : 0u;
+            uint32_t m = 0;  // entries of this lane's topics
+            if (COMPACT) {
+                for (uint32_t x = word; x; x &= x - 1u) {
+                    const uint32_t k = ((uint32_t)wi << 5) + (uint32_t)__ffs(x) - 1u;
+                    m += (h16_get(h, k) + kPiece - 1u) / kPiece;
+                }
+            } else {
+                m = (uint32_t)__popc(word);
+            }
+            uint32_t incl = m;  // inclusive scan over lanes
+            for (int d = 1; d < 64; d <<= 1) {
+                const uint32_t v = __shfl_up(incl, d);
+                if (lane >= d) incl += v;
+            }
+            uint32_t pos = nz + incl - m;
+            for (uint32_t x = word; x; x &= x - 1u) {
+                const uint32_t k = ((uint32_t)wi << 5) + (uint32_t)__ffs(x) - 1u;
+                uint32_t c = h16_get(h, k);
+                if (COMPACT) {
+                    uint16_t *dt16 = reinterpret_cast<uint16_t *>(dt) + base;
+                    for (; c > kPiece; c -= kPiece) dt16[pos++] = (uint16_t)(k | (kPiece << 10));
+                    dt16[pos++] = (uint16_t)(k | (c << 10));
+                } else {
+                    dt[base + pos++] = (k << 2) | (c << 16);
+                }
+            }
+            for (uint32_t x = word; x; x &= x - 1u) {  // the lane's own 16 count words
+                const uint32_t k = ((uint32_t)wi << 5) + (uint32_t)__ffs(x) - 1u;
+                h[k >> 1] = 0;
+            }
+            if (wi < NW) bm[wi] = 0;
+            nz += __shfl(incl, 63);
+        }
+        wave_lds_sync();
+        // zero entries up to the next 128 bytes: the sampler reads rows in
+        // 128-byte blocks and a zero entry adds nothing
         if (e > b) {
             if (COMPACT) {
-                // 16-bit entries topic | piece << 10, a count in pieces of at
-                // most kPiece (each topic's pieces in order)
                 uint16_t *dt16 = reinterpret_cast<uint16_t *>(dt) + base;
-                for (int j = 0; j < K; j += 64) {
-                    const int k = j + lane;
-                    const uint32_t c = k < K ? h16_get(h, (uint32_t)k) : 0u;
-                    const uint32_t m = (c + kPiece - 1u) / kPiece;  // entries of this topic
-                    uint32_t incl = m;                               // inclusive scan over lanes
-                    for (int d = 1; d < 64; d <<= 1) {
-                        const uint32_t v = __shfl_up(incl, d);
-                        if (lane >= d) incl += v;
-                    }
-                    uint32_t pos = nz + incl - m, left = c;
-                    for (uint32_t i = 0; i + 1 < m; ++i, left -= kPiece) dt16[pos++] = (uint16_t)(k | (kPiece << 10));
-                    if (m) dt16[pos] = (uint16_t)(k | (left << 10));
-                    nz += __shfl(incl, 63);
-                    wave_lds_sync();
-                    if (k < K && !(k & 1)) h[k >> 1] = 0;
-                }
-                wave_lds_sync();
-                // zero entries up to the next 128 bytes: a zero entry adds nothing
                 if ((uint32_t)lane < ((64u - (nz & 63u)) & 63u)) dt16[nz + lane] = 0;
             } else {
-                for (int j = 0; j < K; j += 64) {
-                    const int k = j + lane;
-                    const uint32_t c = k < K ? h16_get(h, (uint32_t)k) : 0u;
-                    const uint64_t m = __ballot(c != 0);
-                    if (c) dt[base + nz + __popcll(m & lanemask_lt(lane))] = ((uint32_t)k << 2) | (c << 16);
-                    nz += (uint32_t)__popcll(m);
-                    wave_lds_sync();
-                    if (k < K && !(k & 1)) h[k >> 1] = 0;
-                }
-                wave_lds_sync();
-                // zero the row up to the next multiple of 32 entries: the sampler
-                // reads rows in 128-byte blocks and a zero entry adds nothing
                 if ((uint32_t)lane < ((32u - (nz & 31u)) & 31u)) dt[base + nz + lane] = 0u;
             }
         }
@@ -1111,11 +1129,11 @@ hipError_t launch_build_dt(const DevState &s, hipStream_t st) {
     const int nwv = 4;
     if (s.compact)
         hipLaunchKernelGGL(k_dt_build<true>, dim3(grid_for(s.D, nwv, 1 << 18)), dim3(64 * nwv),
-                           nwv * wave_hist_bytes(s.K), st, s.D, s.K, s.VD, s.soff, s.slot_wrd, s.slot_tok, s.z,
+                           nwv * dt_wave_bytes(s.K), st, s.D, s.K, s.VD, s.soff, s.slot_wrd, s.slot_tok, s.z,
                            s.z_doc, s.wtop, s.aux_doc, s.doc_info, s.dt);
     else
         hipLaunchKernelGGL(k_dt_build<false>, dim3(grid_for(s.D, nwv, 1 << 18)), dim3(64 * nwv),
-                           nwv * wave_hist_bytes(s.K), st, s.D, s.K, s.VD, s.soff, s.slot_wrd, s.slot_tok, s.z,
+                           nwv * dt_wave_bytes(s.K), st, s.D, s.K, s.VD, s.soff, s.slot_wrd, s.slot_tok, s.z,
                            s.z_doc, s.wtop, s.aux_doc, s.doc_info, s.dt);
     return hipGetLastError();
 }
@@ -1142,7 +1160,7 @@ size_t sampler_static_lds() {
 
 size_t max_lds_bytes(int K, int bd) {
     size_t m = sample_lds_bytes(K, bd);
-    m = std::max(m, 4 * wave_hist_bytes(K));                         // WT tail, top-3, DT build
+    m = std::max(m, 4 * dt_wave_bytes(K));                           // WT tail, top-3, DT build
     m = std::max(m, (size_t)K * sizeof(double));                     // log-likelihood
     m = std::max(m, (size_t)K * sizeof(float) + 1024 * sizeof(float));  // fold-in tables
     m = std::max(m, (size_t)foldin_waves_per_block(K) * foldin_wave_lds(K));
