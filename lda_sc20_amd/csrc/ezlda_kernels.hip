@@ -881,7 +881,7 @@ __device__ __forceinline__ float foldin_block(const uint32_t *row, const float *
 }
 
 // One fold-in sweep for every active request slot (rid, first token, n,
-// sweep): one wavefront per slot.  The doc's topic counts are histogrammed
+// sweep): one workgroup per slot.  The doc's topic counts are histogrammed
 // in LDS from its current z and compacted into an ascending row; then one
 // lane per token draws from S (the row, FMA running sum, phi gathered from
 // the frozen table) and Q (the word's precomputed running sums).  The first
@@ -892,39 +892,46 @@ __global__ void __launch_bounds__(256) k_foldin_step(
     const uint32_t *__restrict__ wpool, uint16_t *__restrict__ zpool, int32_t *__restrict__ counts_out,
     uint64_t seed) {
     extern __shared__ uint32_t shf[];
+    __shared__ uint32_t s_nz;
     const int NS = (K + 63) >> 6;
-    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6, nwv = blockDim.x >> 6;
+    const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6, bd = blockDim.x;
     const int KH = (K + 1) >> 1, KR = (K + kFB - 1) / kFB * kFB;
-    uint32_t *h = shf + wv * (KH + KR);
-    uint32_t *row = h + KH;
-    for (int k = lane; k < KH; k += 64) h[k] = 0;
-    wave_lds_sync();
-    for (int si = blockIdx.x * nwv + wv; si < nslots; si += gridDim.x * nwv) {
+    uint32_t *h = shf;        // the doc's 16-bit topic counts
+    uint32_t *row = h + KH;   // its sorted row, zero padded to a block
+    for (int k = tid; k < KH; k += bd) h[k] = 0;
+    __syncthreads();
+    for (int si = blockIdx.x; si < nslots; si += gridDim.x) {
         const uint4 sl = slots[si];
         const uint32_t rid = sl.x, off = sl.y, n = sl.z, sweep = sl.w;
         const uint64_t key = (uint64_t)rid << 32;
         uint16_t *z = zpool + off;
         const uint32_t *wd = wpool + off;
-        if (sweep == 0)
-            for (uint32_t i = lane; i < n; i += 64)
+        if (sweep == 0) {
+            for (uint32_t i = tid; i < n; i += bd)
                 z[i] = (uint16_t)(ez_philox_x0(seed, key | i, 0u, EZ_TAG_FOLDIN_INIT) % (uint32_t)K);
-        for (uint32_t i = lane; i < n; i += 64) h16_add(h, z[i]);
-        wave_lds_sync();
-        uint32_t nz = 0;
-        for (int j = 0; j < K; j += 64) {
-            const int k = j + lane;
-            const uint32_t c = k < K ? h16_get(h, (uint32_t)k) : 0u;
-            const uint64_t m = __ballot(c != 0);
-            if (c) row[nz + __popcll(m & lanemask_lt(lane))] = (uint32_t)k | (c << 16);
-            nz += (uint32_t)__popcll(m);
-            wave_lds_sync();
-            if (k < K && !(k & 1)) h[k >> 1] = 0;
+            __syncthreads();
         }
-        // zero entries up to the next block: count 0 adds exactly nothing
-        if ((uint32_t)lane < ((kFB - (nz % kFB)) % kFB)) row[nz + lane] = 0u;
-        wave_lds_sync();
-        for (uint32_t base = 0; base < n; base += 64) {
-            const uint32_t i = base + (uint32_t)lane;
+        for (uint32_t i = tid; i < n; i += bd) h16_add(h, z[i]);
+        __syncthreads();
+        if (wv == 0) {  // one wavefront compacts the row (and clears the counts)
+            uint32_t nz = 0;
+            for (int j = 0; j < K; j += 64) {
+                const int k = j + lane;
+                const uint32_t c = k < K ? h16_get(h, (uint32_t)k) : 0u;
+                const uint64_t m = __ballot(c != 0);
+                if (c) row[nz + __popcll(m & lanemask_lt(lane))] = (uint32_t)k | (c << 16);
+                nz += (uint32_t)__popcll(m);
+                wave_lds_sync();
+                if (k < K && !(k & 1)) h[k >> 1] = 0;
+            }
+            // zero entries up to the next block: count 0 adds exactly nothing
+            if ((uint32_t)lane < ((kFB - (nz % kFB)) % kFB)) row[nz + lane] = 0u;
+            if (lane == 0) s_nz = nz;
+        }
+        __syncthreads();
+        const uint32_t nz = s_nz;
+        for (uint32_t base = 0; base < n; base += (uint32_t)bd) {
+            const uint32_t i = base + (uint32_t)tid;
             if (i < n) {
                 const uint32_t w = wd[i];
                 const float *ph = phi + (size_t)w * K;
@@ -1014,15 +1021,15 @@ __global__ void __launch_bounds__(256) k_foldin_step(
             }
         }
         if (sweep + 1 == (uint32_t)sweeps_total) {  // final counts of the doc
-            wave_lds_sync();
-            for (uint32_t i = lane; i < n; i += 64) h16_add(h, z[i]);
-            wave_lds_sync();
+            __syncthreads();
+            for (uint32_t i = tid; i < n; i += bd) h16_add(h, z[i]);
+            __syncthreads();
             int32_t *co = counts_out + (size_t)si * K;
-            for (int k = lane; k < K; k += 64) co[k] = (int32_t)h16_get(h, (uint32_t)k);
-            wave_lds_sync();
-            for (int k = lane; k < KH; k += 64) h[k] = 0;
+            for (int k = tid; k < K; k += bd) co[k] = (int32_t)h16_get(h, (uint32_t)k);
+            __syncthreads();
+            for (int k = tid; k < KH; k += bd) h[k] = 0;
         }
-        wave_lds_sync();
+        __syncthreads();
     }
 }
 
@@ -1036,10 +1043,8 @@ hipError_t launch_foldin_tables(const DevState &s, float *phi, float *qs, float 
 static size_t foldin_wave_lds(int K) { return (size_t)(((K + 1) >> 1) + (K + kFB - 1) / kFB * kFB) * sizeof(uint32_t); }
 
 int foldin_waves_per_block(int K) {
-    const size_t per_wave = foldin_wave_lds(K);
-    int w = 4;
-    while (w > 1 && per_wave * w > 64 * 1024) --w;
-    return w;
+    (void)K;
+    return 4;  // one workgroup of 4 wavefronts per document
 }
 
 hipError_t launch_foldin_step(const uint4 *slots, int nslots, int K, int sweeps_total, float alpha, const float *phi,
@@ -1047,8 +1052,8 @@ hipError_t launch_foldin_step(const uint4 *slots, int nslots, int K, int sweeps_
                               int32_t *counts_out, uint64_t seed, hipStream_t st) {
     if (nslots <= 0) return hipSuccess;
     const int nwv = foldin_waves_per_block(K);
-    const size_t lds = (size_t)nwv * foldin_wave_lds(K);
-    hipLaunchKernelGGL(k_foldin_step, dim3(grid_for(nslots, nwv, 1 << 16)), dim3(64 * nwv), lds, st, slots, nslots, K,
+    const size_t lds = foldin_wave_lds(K);  // one document per workgroup
+    hipLaunchKernelGGL(k_foldin_step, dim3(grid_for(nslots, 1, 1 << 16)), dim3(64 * nwv), lds, st, slots, nslots, K,
                        sweeps_total, alpha, phi, qs, qacc, wpool, zpool, counts_out, seed);
     return hipGetLastError();
 }
@@ -1163,7 +1168,7 @@ size_t max_lds_bytes(int K, int bd) {
     m = std::max(m, 4 * dt_wave_bytes(K));                           // WT tail, top-3, DT build
     m = std::max(m, (size_t)K * sizeof(double));                     // log-likelihood
     m = std::max(m, (size_t)K * sizeof(float) + 1024 * sizeof(float));  // fold-in tables
-    m = std::max(m, (size_t)foldin_waves_per_block(K) * foldin_wave_lds(K));
+    m = std::max(m, foldin_wave_lds(K) + 16);
     return m;
 }
 
