@@ -190,6 +190,13 @@ int ezlda_autotune(ezlda_ctx *ctx, int32_t sweeps_per_trial, ezlda_args *chosen,
 int ezlda_get_nk(ezlda_ctx *ctx, int32_t *nk);
 /* Dense V x K word-topic counts, rows by input word id. */
 int ezlda_get_wt_dense(ezlda_ctx *ctx, int32_t *wt);
+/* Model export of the current state (the reference keeps only debug dumps,
+ * WT.cu:242-280, DT.cu:154-168): dir/model.txt ("K V D alpha beta sweep"),
+ * dir/phi.f32 (V x K little-endian float32, rows by input word id, phi =
+ * (n_wk + beta) / (n_k + V beta) as the sampler computes it) and
+ * dir/theta.txt (per input doc: "n_d topic:count ..." in ascending topic).
+ * The directory must exist. */
+int ezlda_export_model(ezlda_ctx *ctx, const char *dir);
 /* DT rows as the sampler walks them: entries[dt_offset(d) + i] = topic |
  * count << 16 for i < nnz[d], ascending topic; for K <= 1024 a count above
  * 63 appears as pieces of 63 and a remainder.  dt_offset(d) = sum_{e<d}

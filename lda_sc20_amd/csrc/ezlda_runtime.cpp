@@ -1043,6 +1043,61 @@ int ezlda_get_wt_dense(ezlda_ctx *c, int32_t *wt) {
     return EZLDA_OK;
 }
 
+int ezlda_export_model(ezlda_ctx *c, const char *dir) {
+    if (int r = need_loaded(c)) return r;
+    if (!dir) return fail(c, EZLDA_EINVAL, "null directory");
+    if (int r = ezc::refresh_counts(c)) return r;
+    EZ_TRY_HIP(c, ez::launch_build_dt(c->s, c->st));
+    EZ_TRY_HIP(c, hipStreamSynchronize(c->st));
+    const ez::DevState &s = c->s;
+    const std::string d(dir);
+    std::vector<int32_t> wt((size_t)s.V * s.K), nk(s.K);
+    std::vector<float> inv(s.K);
+    if (int r = ezlda_get_wt_dense(c, wt.data())) return r;
+    EZ_TRY_HIP(c, hipMemcpy(inv.data(), s.inv, sizeof(float) * s.K, hipMemcpyDeviceToHost));
+    {
+        std::FILE *f = std::fopen((d + "/phi.f32").c_str(), "wb");
+        if (!f) return fail(c, EZLDA_EIO, "cannot write " + d + "/phi.f32");
+        std::vector<float> row(s.K);
+        bool ok = true;
+        for (int w = 0; w < s.V && ok; ++w) {
+            for (int k = 0; k < s.K; ++k) row[k] = ((float)wt[(size_t)w * s.K + k] + s.beta) * inv[k];
+            ok = std::fwrite(row.data(), sizeof(float), s.K, f) == (size_t)s.K;
+        }
+        ok = (std::fclose(f) == 0) && ok;
+        if (!ok) return fail(c, EZLDA_EIO, "short write to " + d + "/phi.f32");
+    }
+    {
+        std::vector<uint32_t> ent((size_t)std::max<int64_t>(c->dt_cap_canon, 1));
+        std::vector<int32_t> nnz(std::max(s.D, 1));
+        if (int r = ezlda_get_dt_raw(c, ent.data(), nnz.data())) return r;
+        std::FILE *f = std::fopen((d + "/theta.txt").c_str(), "w");
+        if (!f) return fail(c, EZLDA_EIO, "cannot write " + d + "/theta.txt");
+        uint64_t off = 0;
+        bool ok = true;
+        for (int doc = 0; doc < s.D && ok; ++doc) {  // input doc order
+            const uint32_t n = c->doc_off[doc + 1] - c->doc_off[doc];
+            ok = std::fprintf(f, "%u", n) > 0;
+            for (int32_t i = 0; i < nnz[doc];) {  // merge a topic's pieces
+                const uint32_t k = ent[off + i] & 0xFFFFu;
+                uint32_t cnt = 0;
+                for (; i < nnz[doc] && (ent[off + i] & 0xFFFFu) == k; ++i) cnt += ent[off + i] >> 16;
+                ok = ok && std::fprintf(f, " %u:%u", k, cnt) > 0;
+            }
+            ok = ok && std::fputc('\n', f) != EOF;
+            off += dt_cap(n, s.K);
+        }
+        ok = (std::fclose(f) == 0) && ok;
+        if (!ok) return fail(c, EZLDA_EIO, "short write to " + d + "/theta.txt");
+    }
+    std::FILE *f = std::fopen((d + "/model.txt").c_str(), "w");
+    if (!f) return fail(c, EZLDA_EIO, "cannot write " + d + "/model.txt");
+    const bool ok = std::fprintf(f, "%d %d %d %.9g %.9g %u\n", s.K, s.V, s.D, (double)s.alpha, (double)s.beta,
+                                 c->sweep) > 0;
+    if ((std::fclose(f) != 0) || !ok) return fail(c, EZLDA_EIO, "short write to " + d + "/model.txt");
+    return EZLDA_OK;
+}
+
 int ezlda_get_dt_raw(ezlda_ctx *c, uint32_t *entries, int32_t *nnz) {
     if (int r = need_loaded(c)) return r;
     const ez::DevState &s = c->s;

@@ -32,6 +32,7 @@ EXPORTS = (
     "ezlda_reset_stats", "ezlda_server_create", "ezlda_server_destroy", "ezlda_server_last_error",
     "ezlda_server_submit", "ezlda_server_wait", "ezlda_server_drain", "ezlda_server_get_stats",
     "ezlda_server_reset_stats", "ezlda_foldin_plain", "ezlda_set_layout", "ezlda_autotune",
+    "ezlda_export_model",
 )
 
 
@@ -138,6 +139,7 @@ def lib() -> C.CDLL:
         "ezlda_get_stats": (C.c_int, [P, P]),
         "ezlda_reset_stats": (C.c_int, [P]),
         "ezlda_set_layout": (C.c_int, [P, C.POINTER(Args)]),
+        "ezlda_export_model": (C.c_int, [P, C.c_char_p]),
         "ezlda_autotune": (C.c_int, [P, C.c_int32, C.POINTER(Args), C.POINTER(C.c_double), P, C.c_int32,
                                      C.POINTER(C.c_int32)]),
         "ezlda_server_create": (P, [P, C.POINTER(ServerArgs)]),
@@ -329,6 +331,10 @@ class Context:
 
     def reset_stats(self):
         self._chk(self._L.ezlda_reset_stats(self._h))
+
+    def export_model(self, directory: str):
+        """Write model.txt, phi.f32 (V x K float32) and theta.txt into directory."""
+        self._chk(self._L.ezlda_export_model(self._h, directory.encode()))
 
     # layout tuning (topics never depend on it)
     def set_layout(self, **kw):

@@ -127,6 +127,7 @@ public:
         t.resize(std::min<size_t>((size_t)n, t.size()));
         return t;
     }
+    void export_model(const std::string &dir) { check(ezlda_export_model(h_, dir.c_str()), h_); }
     void save_checkpoint(const std::string &p) { check(ezlda_save_checkpoint(h_, p.c_str()), h_); }
     void load_checkpoint(const std::string &p) { check(ezlda_load_checkpoint(h_, p.c_str()), h_); }
     ezlda_ctx *handle() { return h_; }

@@ -50,6 +50,7 @@ const char *kUsage =
     "             [--block-dim N] [--grid-dim N] [--streams N] [--token-segment N] [--seed S]\n"
     "             [--dense-threshold N] [--dense-words N] [--doc-tiles N] [--run-cap N] [--plain]\n"
     "             [--graph] [--autotune] [--ll-every N] [--checkpoint FILE] [--resume FILE] [--record FILE]\n"
+    "             [--export DIR]\n"
     "  foldin     train options, plus --requests DOCWORD [--foldin-sweeps 20] [--slots N]\n"
     "             [--budget-mb N] [--out FILE]: folds the documents of a docword file into the\n"
     "             trained model and writes each one's topic counts\n"
@@ -160,6 +161,7 @@ int run_training(const Opts &o, ezlda::Context &ctx) {
     std::printf("%d sweeps in %.1f ms (%.3e tokens/s); log-likelihood %.6e; perplexity %.4f\n", iters, total,
                 inf.num_tokens * (double)iters / (total / 1e3), ll, std::exp(-ll / (double)inf.num_tokens));
     if (o.has("checkpoint")) ctx.save_checkpoint(o.str("checkpoint"));
+    if (o.has("export")) ctx.export_model(o.str("export"));
     return 0;
 }
 

@@ -244,3 +244,28 @@ def test_autotune_sweeps_are_ordinary_sweeps():
     ctx.sweep(1)
     orc.sweep(1)
     assert np.array_equal(ctx.get_topics(), orc.get_topics())
+
+
+def test_export_model(tmp_path):
+    off, words = make_corpus(8, 120, 200, 90, zipf=1.2)
+    ctx, orc = _pair(off, words, 200, 40)
+    ctx.sweep(3)
+    orc.sweep(3)
+    ctx.export_model(str(tmp_path))
+    K, V, D = (int(x) for x in open(tmp_path / "model.txt").read().split()[:3])
+    assert (K, V, D) == (40, 200, 120)
+    phi = np.fromfile(tmp_path / "phi.f32", dtype=np.float32).reshape(V, K)
+    orc.build_wt()
+    orc.topk()
+    wt, inv = orc.get_wt_dense(), orc.get_inv()
+    assert np.array_equal(phi, ((wt.astype(np.float32) + np.float32(BETA)) * inv).astype(np.float32))
+    orc.build_dt()
+    dt = orc.get_dt_dense()
+    for d, line in enumerate(open(tmp_path / "theta.txt").read().splitlines()):
+        f = line.split()
+        assert int(f[0]) == off[d + 1] - off[d]
+        row = np.zeros(K, np.int32)
+        for kc in f[1:]:
+            k, c = kc.split(":")
+            row[int(k)] = int(c)
+        assert np.array_equal(row, dt[d])
