@@ -79,8 +79,10 @@ typedef struct ezlda_info {
     int32_t num_head_words;  /* V_D */
     int32_t K;
     int64_t dt_capacity;     /* entries of ezlda_get_dt_raw: sum_d of min(n_d, K),
-                                plus n_d / 63 when K <= 1024 (rows then hold
-                                a count in pieces of at most 63) */
+                                plus n_d / p when K <= 8192 (rows then hold a
+                                count in pieces of at most p = 2^(16 - tb) - 1,
+                                tb = max(10, ceil(log2 K)): 63 up to K = 1024,
+                                31, 15, 7 up to 2048, 4096, 8192) */
     int64_t wt_tail_capacity;/* sum_tail min(n_w, K) */
     int32_t num_sample_items;/* sampler work items (word segments) */
     uint32_t sweep;          /* sweeps done (the Philox sweep counter) */
@@ -198,9 +200,10 @@ int ezlda_get_wt_dense(ezlda_ctx *ctx, int32_t *wt);
  * The directory must exist. */
 int ezlda_export_model(ezlda_ctx *ctx, const char *dir);
 /* DT rows as the sampler walks them: entries[dt_offset(d) + i] = topic |
- * count << 16 for i < nnz[d], ascending topic; for K <= 1024 a count above
- * 63 appears as pieces of 63 and a remainder.  dt_offset(d) = sum_{e<d}
- * cap(n_e), cap(n) = min(n, K) (+ n / 63, at most n, when K <= 1024). */
+ * count << 16 for i < nnz[d], ascending topic; for K <= 8192 a count above
+ * p (see dt_capacity) appears as pieces of p and a remainder.
+ * dt_offset(d) = sum_{e<d} cap(n_e), cap(n) = min(n, K) (+ n / p, at most
+ * n, when K <= 8192). */
 int ezlda_get_dt_raw(ezlda_ctx *ctx, uint32_t *entries, int32_t *nnz);
 /* Top-3 topics per word (3 x uint16 per input word id; 0xFFFF = none). */
 int ezlda_get_top(ezlda_ctx *ctx, uint16_t *top);

@@ -27,7 +27,8 @@ struct DevState {
     // doc-topic rows
     uint2 *doc_info = nullptr;      // D, by doc q: x = DT row offset, y = nnz | len << 16
     uint32_t *dt = nullptr;         // rows: u32 (topic << 2) | count << 16, or (compact) u16 topic | piece << 10
-    bool compact = false;           // K <= 1024: 16-bit entries, counts in pieces of <= 63
+    bool compact = false;           // K <= 8192: 16-bit entries topic | piece << topic_bits
+    uint32_t topic_bits = 16;       // max(10, ceil(log2 K)); pieces of <= 2^(16 - topic_bits) - 1
     // word-topic: dense head rows + sorted sparse tail rows
     int32_t *wt_head = nullptr;     // VD*K
     uint32_t *tail_off = nullptr;   // V-VD+1 (capacity min(n_w,K))

@@ -65,11 +65,11 @@ __device__ __forceinline__ float phi_of(const float *phi, uint32_t e) {
     return *reinterpret_cast<const float *>(reinterpret_cast<const char *>(phi) + (e & 0xFFFCu));
 }
 
-// Compact DT rows (K <= kCompactK): 16-bit entries topic | piece << 10, a
-// count stored in pieces of at most kPiece.
-static constexpr uint32_t kCompactK = 1024, kPiece = 63;
-__device__ __forceinline__ float c16_count(uint32_t e) { return (float)((e >> 10) & 63u); }
-__device__ __forceinline__ uint32_t c16_topic(uint32_t e) { return e & 0x3FFu; }
+// Compact DT rows (K <= 8192): 16-bit entries topic | piece << tb, tb =
+// max(10, ceil(log2 K)) topic bits; a count stored in pieces of at most
+// 2^(16 - tb) - 1.  Decoders take the low 16 bits of e.
+__device__ __forceinline__ float c16_count(uint32_t e, uint32_t tb) { return (float)((e & 0xFFFFu) >> tb); }
+__device__ __forceinline__ uint32_t c16_topic(uint32_t e, uint32_t tb) { return e & ((1u << tb) - 1u); }
 
 // The same read for a phi table at LDS address 0 (the sampler's dynamic LDS
 // when the kernel has no static LDS, which sampler_static_lds() lets the host
@@ -77,7 +77,7 @@ __device__ __forceinline__ uint32_t c16_topic(uint32_t e) { return e & 0x3FFu; }
 typedef __attribute__((address_space(3))) const float lds_cfloat;
 __device__ __forceinline__ float phi_at0(uint32_t e) { return *(lds_cfloat *)(e & 0xFFFCu); }
 // phi of a compact entry (the 16-bit entry in the low half of e)
-__device__ __forceinline__ float phi_c16(uint32_t e) { return *(lds_cfloat *)((e & 0x3FFu) << 2); }
+__device__ __forceinline__ float phi_c16(uint32_t e, uint32_t tb) { return *(lds_cfloat *)(c16_topic(e, tb) << 2); }
 
 // 16-bit counters, two per LDS word (counts < 65536 by construction).
 __device__ __forceinline__ void h16_add(uint32_t *h, uint32_t k) {
@@ -282,7 +282,7 @@ hipError_t launch_z_to_doc(const DevState &s, hipStream_t st) {
 // ascending topic order (:183-238).  A row starts on a 128-byte boundary and
 // is zero-padded to a multiple of 32 entries.
 template <bool COMPACT>
-__global__ void k_dt_build(int D, int K, int VD, const uint32_t *__restrict__ soff,
+__global__ void k_dt_build(int D, int K, int VD, uint32_t tb, const uint32_t *__restrict__ soff,
                            const uint32_t *__restrict__ slot_wrd, const uint32_t *__restrict__ slot_tok,
                            const uint16_t *__restrict__ z, const uint16_t *__restrict__ z_doc,
                            const ushort4 *__restrict__ wtop,
@@ -290,6 +290,7 @@ __global__ void k_dt_build(int D, int K, int VD, const uint32_t *__restrict__ so
     extern __shared__ uint32_t sh[];
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6, nwv = blockDim.x >> 6;
     const int KH = (K + 1) >> 1, NW = (K + 31) >> 5;
+    const uint32_t piece = (1u << (16u - tb)) - 1u;  // compact rows: largest count of one entry
     uint32_t *h = sh + wv * (KH + NW);  // 16-bit counts
     uint32_t *bm = h + KH;              // bitmap of the doc's topics
     for (int k = lane; k < KH + NW; k += 64) h[k] = 0;
@@ -323,7 +324,7 @@ __global__ void k_dt_build(int D, int K, int VD, const uint32_t *__restrict__ so
             if (COMPACT) {
                 for (uint32_t x = word; x; x &= x - 1u) {
                     const uint32_t k = ((uint32_t)wi << 5) + (uint32_t)__ffs(x) - 1u;
-                    m += (h16_get(h, k) + kPiece - 1u) / kPiece;
+                    m += (h16_get(h, k) + piece - 1u) / piece;
                 }
             } else {
                 m = (uint32_t)__popc(word);
@@ -339,8 +340,8 @@ __global__ void k_dt_build(int D, int K, int VD, const uint32_t *__restrict__ so
                 uint32_t c = h16_get(h, k);
                 if (COMPACT) {
                     uint16_t *dt16 = reinterpret_cast<uint16_t *>(dt) + base;
-                    for (; c > kPiece; c -= kPiece) dt16[pos++] = (uint16_t)(k | (kPiece << 10));
-                    dt16[pos++] = (uint16_t)(k | (c << 10));
+                    for (; c > piece; c -= piece) dt16[pos++] = (uint16_t)(k | (piece << tb));
+                    dt16[pos++] = (uint16_t)(k | (c << tb));
                 } else {
                     dt[base + pos++] = (k << 2) | (c << 16);
                 }
@@ -387,7 +388,7 @@ __device__ __forceinline__ RowBlock load_block(const uint4 *__restrict__ p) {
 }
 
 template <bool COMPACT>
-__device__ __forceinline__ float sum_block(const RowBlock &r, float s) {
+__device__ __forceinline__ float sum_block(const RowBlock &r, float s, uint32_t tb) {
     const uint4 *e = r.e;
 #pragma unroll
     for (int i = 0; i < kBlkU4; ++i) {
@@ -395,8 +396,8 @@ __device__ __forceinline__ float sum_block(const RowBlock &r, float s) {
             const uint32_t w[4] = {e[i].x, e[i].y, e[i].z, e[i].w};
 #pragma unroll
             for (int q = 0; q < 4; ++q) {
-                s = __builtin_fmaf(c16_count(w[q]), phi_c16(w[q]), s);
-                s = __builtin_fmaf((float)(w[q] >> 26), phi_c16(w[q] >> 16), s);
+                s = __builtin_fmaf(c16_count(w[q], tb), phi_c16(w[q], tb), s);
+                s = __builtin_fmaf((float)(w[q] >> (16u + tb)), phi_c16(w[q] >> 16, tb), s);
             }
         } else {
             s = __builtin_fmaf(dt_count(e[i].x), phi_at0(e[i].x), s);
@@ -439,6 +440,7 @@ struct SampleArgs {
     const uint32_t *counters;
     uint32_t *queue;  // persistent mode: one item counter per queue
     uint32_t n_queues;
+    uint32_t tb;  // compact rows: topic bits
     unsigned long long *stats;
     uint64_t seed;
 };
@@ -505,6 +507,7 @@ __device__ __forceinline__ void sample_item(const SampleArgs &A, uint32_t item, 
     // tile, so the copy's lines fill in L2); tail tokens, in random docs,
     // are read back by token id in the DT rebuild instead
     const bool head = w < A.VD;
+    const uint32_t tb = A.tb;
 
     for (uint32_t base = sg.y + (uint32_t)wv * 64u; base < sg.z; base += (uint32_t)nwv * 64u) {
         const uint32_t r = base + (uint32_t)lane;
@@ -553,14 +556,14 @@ __device__ __forceinline__ void sample_item(const SampleArgs &A, uint32_t item, 
         for (int b = 0; b < kCP; ++b) {
             RowBlock nxt;
             if ((uint32_t)b + 1u < nb) nxt = load_block(rp + kBlkU4 * (b + 1));
-            if ((uint32_t)b < nb) S = sum_block<COMPACT>(cur, S);
+            if ((uint32_t)b < nb) S = sum_block<COMPACT>(cur, S, tb);
             cp[b] = S;
             cur = nxt;
         }
         for (uint32_t b = kCP; b < nb; ++b) {
             RowBlock nxt;
             if (b + 1u < nb) nxt = load_block(rp + kBlkU4 * (b + 1u));
-            S = sum_block<COMPACT>(cur, S);
+            S = sum_block<COMPACT>(cur, S, tb);
             cur = nxt;
         }
         const float Aa = P1 + S;
@@ -602,13 +605,14 @@ __device__ __forceinline__ void sample_item(const SampleArgs &A, uint32_t item, 
 #pragma unroll
                         for (int q = 0; q < 4; ++q) {
                             if (found) break;
-                            const float ra = __builtin_fmaf(c16_count(w[q]), phi_c16(w[q]), run);
-                            const float rb = __builtin_fmaf((float)(w[q] >> 26), phi_c16(w[q] >> 16), ra);
+                            const float ra = __builtin_fmaf(c16_count(w[q], tb), phi_c16(w[q], tb), run);
+                            const float rb =
+                                __builtin_fmaf((float)(w[q] >> (16u + tb)), phi_c16(w[q] >> 16, tb), ra);
                             if (ra > t2) {
-                                newz = c16_topic(w[q]);
+                                newz = c16_topic(w[q], tb);
                                 found = true;
                             } else if (rb > t2) {
-                                newz = c16_topic(w[q] >> 16);
+                                newz = c16_topic(w[q] >> 16, tb);
                                 found = true;
                             }
                             run = rb;
@@ -629,7 +633,7 @@ __device__ __forceinline__ void sample_item(const SampleArgs &A, uint32_t item, 
                     if (COMPACT) {
                         const uint16_t *rr = reinterpret_cast<const uint16_t *>(A.dt) + di.x;
                         for (uint32_t i = 0; i < nnz; ++i)
-                            if (c16_count(rr[i]) > 0.0f && phi_c16(rr[i]) > 0.0f) newz = c16_topic(rr[i]);
+                            if (c16_count(rr[i], tb) > 0.0f && phi_c16(rr[i], tb) > 0.0f) newz = c16_topic(rr[i], tb);
                     } else {
                         const uint32_t *rr = A.dt + di.x;
                         for (uint32_t i = 0; i < nnz; ++i)
@@ -731,7 +735,7 @@ __global__ void __launch_bounds__(1024) k_ll(const uint4 *__restrict__ segs, con
                                              const uint32_t *__restrict__ wt_tail,
                                              const uint32_t *__restrict__ tail_nnz, const int32_t *__restrict__ nk,
                                              const uint2 *__restrict__ doc_info, const uint32_t *__restrict__ dt,
-                                             int compact, double *__restrict__ part) {
+                                             int compact, uint32_t tb, double *__restrict__ part) {
     extern __shared__ double smd[];
     __shared__ double red[16];
     const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6, nwv = blockDim.x >> 6;
@@ -776,7 +780,7 @@ __global__ void __launch_bounds__(1024) k_ll(const uint4 *__restrict__ segs, con
             const uint16_t *r16 = reinterpret_cast<const uint16_t *>(dt) + di.x;
             for (uint32_t i = lane; i < nnz; i += 64) {
                 const uint32_t e = r16[i];
-                s += (double)(e >> 10) * phi[e & 0x3FFu];
+                s += (double)(e >> tb) * phi[e & ((1u << tb) - 1u)];
             }
         } else {
             for (uint32_t i = lane; i < nnz; i += 64) {
@@ -1134,11 +1138,11 @@ hipError_t launch_build_dt(const DevState &s, hipStream_t st) {
     const int nwv = 4;
     if (s.compact)
         hipLaunchKernelGGL(k_dt_build<true>, dim3(grid_for(s.D, nwv, 1 << 18)), dim3(64 * nwv),
-                           nwv * dt_wave_bytes(s.K), st, s.D, s.K, s.VD, s.soff, s.slot_wrd, s.slot_tok, s.z,
+                           nwv * dt_wave_bytes(s.K), st, s.D, s.K, s.VD, s.topic_bits, s.soff, s.slot_wrd, s.slot_tok, s.z,
                            s.z_doc, s.wtop, s.aux_doc, s.doc_info, s.dt);
     else
         hipLaunchKernelGGL(k_dt_build<false>, dim3(grid_for(s.D, nwv, 1 << 18)), dim3(64 * nwv),
-                           nwv * dt_wave_bytes(s.K), st, s.D, s.K, s.VD, s.soff, s.slot_wrd, s.slot_tok, s.z,
+                           nwv * dt_wave_bytes(s.K), st, s.D, s.K, s.VD, s.topic_bits, s.soff, s.slot_wrd, s.slot_tok, s.z,
                            s.z_doc, s.wtop, s.aux_doc, s.doc_info, s.dt);
     return hipGetLastError();
 }
@@ -1200,6 +1204,7 @@ hipError_t launch_sample(const DevState &s, bool shortcut, hipStream_t st) {
     A.n_queues = s.grid_dim > 0 ? (uint32_t)s.n_queues : 0u;
     A.stats = s.stats;
     A.seed = s.seed;
+    A.tb = s.topic_bits;
     unsigned grid = (unsigned)s.n_smp_segs;
     if (A.n_queues) {
         // queue heads start at zero (a fill kernel, so a captured sweep
@@ -1232,7 +1237,7 @@ hipError_t launch_loglik(const DevState &s, hipStream_t st) {
     hipLaunchKernelGGL(k_ll, dim3(s.n_smp_segs), dim3(bd), (size_t)s.K * sizeof(double), st, s.smp_segs, s.runs, s.VD,
                        s.K,
                        (double)s.alpha, (double)s.beta, (double)s.V * (double)s.beta, s.wt_head, s.tail_off,
-                       s.wt_tail, s.tail_nnz, s.nk, s.doc_info, s.dt, (int)s.compact, s.ll_part);
+                       s.wt_tail, s.tail_nnz, s.nk, s.doc_info, s.dt, (int)s.compact, s.topic_bits, s.ll_part);
     hipLaunchKernelGGL(k_ll_final, dim3(1), dim3(256), 0, st, s.ll_part, s.n_smp_segs, s.ll_out);
     return hipGetLastError();
 }

@@ -44,16 +44,25 @@ void parallel_for(int64_t n, F f) {
     for (auto &x : th) x.join();
 }
 
-// DT rows: 16-bit entries for K <= kCompactK, counts in pieces of at most
-// kPiece (the oracle's ORC_COMPACT_K / ORC_PIECE).
-constexpr uint32_t kCompactK = 1024, kPiece = 63;
+// DT rows: 16-bit entries for K <= kCompactK (the oracle's ORC_COMPACT_K),
+// tb = max(10, ceil(log2 K)) topic bits, counts in pieces of at most
+// 2^(16 - tb) - 1.
+constexpr uint32_t kCompactK = 8192;
+
+uint32_t topic_bits(int K) {
+    uint32_t b = 10;
+    while ((1 << b) < K) ++b;
+    return b;
+}
+
+uint32_t dt_piece(int K) { return (1u << (16u - topic_bits(K))) - 1u; }
 
 // Entries of a doc's DT row at most: min(n, K), plus one per full piece in
 // compact rows.
 uint32_t dt_cap(uint32_t n, int K) {
     const uint32_t m = std::min<uint32_t>(n, (uint32_t)K);
     if ((uint32_t)K > kCompactK) return m;
-    return std::min<uint32_t>(m + n / kPiece, n);
+    return std::min<uint32_t>(m + n / dt_piece(K), n);
 }
 
 // Compute units of the current device (256 on MI355X).
@@ -580,9 +589,9 @@ int ezlda_load_docs(ezlda_ctx *c, int32_t V, int32_t D, const int64_t *doc_off, 
         for (int64_t p = b; p < e; ++p) z_doc[p] = z[c->slot_tok[p]];
     });
     // DT rows in doc order q, each 128-byte aligned (the build zero-fills
-    // the rest): K <= 1024 keeps 16-bit entries with counts in pieces of at
-    // most 63 (capacity dt_cap(n, K) rounded up to 64 entries), larger K
-    // 32-bit entries (min(n_d, K) rounded up to 32)
+    // the rest): K <= 8192 keeps 16-bit entries with counts in pieces
+    // (capacity dt_cap(n, K) rounded up to 64 entries), larger K 32-bit
+    // entries (min(n_d, K) rounded up to 32)
     const bool compact = K <= (int)kCompactK;
     std::vector<uint2> doc_info(D);
     uint64_t dtc = 0;
@@ -594,6 +603,7 @@ int ezlda_load_docs(ezlda_ctx *c, int32_t V, int32_t D, const int64_t *doc_off, 
     if (dtc >= 0xFFFFFFF0ull) return fail(c, EZLDA_EINVAL, "DT capacity exceeds 2^32 entries");
     ez::DevState &s = c->s;
     s.compact = compact;
+    s.topic_bits = compact ? topic_bits(K) : 16u;
     s.K = K; s.V = V; s.D = D; s.N = N;
     s.alpha = c->a.alpha; s.beta = c->a.beta;
     s.vbeta = (float)((double)V * (double)c->a.beta);
@@ -1115,7 +1125,7 @@ int ezlda_get_dt_raw(ezlda_ctx *c, uint32_t *entries, int32_t *nnz) {
         for (int32_t i = 0; i < nnz[d]; ++i) {  // device rows hold topic << 2, or 16-bit entries
             if (s.compact) {
                 const uint32_t e = dt16[x.x + i];
-                entries[canon + i] = (e & 0x3FFu) | ((e >> 10) << 16);
+                entries[canon + i] = (e & ((1u << s.topic_bits) - 1u)) | ((e >> s.topic_bits) << 16);
             } else {
                 const uint32_t e = dt[x.x + i];
                 entries[canon + i] = ((e & 0xFFFFu) >> 2) | (e & 0xFFFF0000u);

@@ -497,15 +497,24 @@ def selftest_copy(nbytes: int, reps: int = 10) -> float:
     return ms.value
 
 
-COMPACT_K, PIECE = 1024, 63
+COMPACT_K = 8192
+
+
+def dt_piece(K: int) -> int:
+    """Largest count of one compact DT entry: 2^(16 - tb) - 1 with tb =
+    max(10, ceil(log2 K)) topic bits (63 for K <= 1024, 31 for K <= 2048...)."""
+    tb = 10
+    while (1 << tb) < K:
+        tb += 1
+    return (1 << (16 - tb)) - 1
 
 
 def dt_capacity(n: np.ndarray, K: int) -> np.ndarray:
     """Entries of a doc's DT row at most: min(n, K), plus one per full piece
-    of 63 when K <= 1024 (rows then store counts in pieces of at most 63)."""
+    when K <= 8192 (rows then store counts in pieces of at most dt_piece(K))."""
     n = np.asarray(n, dtype=np.int64)
     m = np.minimum(n, K)
-    return m if K > COMPACT_K else np.minimum(m + n // PIECE, n)
+    return m if K > COMPACT_K else np.minimum(m + n // dt_piece(K), n)
 
 
 def dt_offsets(doc_off: Sequence[int], K: int) -> np.ndarray:

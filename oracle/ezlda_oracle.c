@@ -64,18 +64,26 @@
 
 #define ORC_MARGIN 0.998046875f /* 1 - 2^-9 */
 #define ORC_NONE 0xFFFFu
-/* Compact DT rows (K <= ORC_COMPACT_K): a topic's count is stored in pieces
- * of at most ORC_PIECE, one entry per piece, in order; every S sum and
- * running sum walks the pieces.  Larger K keeps one entry per topic. */
-#define ORC_COMPACT_K 1024
-#define ORC_PIECE 63
+/* Compact DT rows (K <= ORC_COMPACT_K): 16-bit entries of tb topic bits,
+ * tb = max(10, ceil(log2 K)), and 16 - tb count bits; a topic's count is
+ * stored in pieces of at most piece = 2^(16 - tb) - 1, one entry per piece,
+ * in order, and every S sum and running sum walks the pieces.  Larger K
+ * keeps one entry per topic. */
+#define ORC_COMPACT_K 8192
+
+static int topic_bits(int K) {
+    int b = 10;
+    while ((1 << b) < K) ++b;
+    return b;
+}
+static uint32_t dt_piece(int K) { return K <= ORC_COMPACT_K ? (1u << (16 - topic_bits(K))) - 1u : 0xFFFFFFFFu; }
 
 /* Entries of a doc's DT row at most: min(n, K), plus one per full piece in
  * compact rows. */
 static int64_t dt_cap(int64_t n, int K) {
     const int64_t m = n < K ? n : K;
     if (K > ORC_COMPACT_K) return m;
-    const int64_t c = m + n / ORC_PIECE;
+    const int64_t c = m + n / (int64_t)dt_piece(K);
     return c < n ? c : n;
 }
 
@@ -306,12 +314,13 @@ void orc_topk(void *h) {
 /* ---------------------------------------- 4. DT rows and per-token aux */
 /* DT_Update_Kernel, utility.cu:79-284: histogram of the doc's topics,
  * per-token (n_dk1, n_dk2) re-pack (:125-161), ascending-topic compaction
- * (:183-238); for K <= ORC_COMPACT_K a count above ORC_PIECE becomes
- * pieces of ORC_PIECE and a remainder (the reference packs the count in 16
+ * (:183-238); for K <= ORC_COMPACT_K a count above dt_piece(K) becomes
+ * pieces of dt_piece(K) and a remainder (the reference packs the count in 16
  * bits and never splits). */
 void orc_build_dt(void *h) {
     orc_t *o = (orc_t *)h;
     const int K = o->K;
+    const uint32_t piece = dt_piece(K);
 #pragma omp parallel
     {
         int32_t *hist = (int32_t *)xcalloc(K, sizeof(int32_t));
@@ -330,7 +339,7 @@ void orc_build_dt(void *h) {
                 if (hist[k]) {
                     uint32_t c = (uint32_t)hist[k];
                     if (K <= ORC_COMPACT_K)
-                        for (; c > ORC_PIECE; c -= ORC_PIECE) row[nz++] = (uint32_t)k | ((uint32_t)ORC_PIECE << 16);
+                        for (; c > piece; c -= piece) row[nz++] = (uint32_t)k | (piece << 16);
                     row[nz++] = (uint32_t)k | (c << 16);
                     hist[k] = 0;
                 }

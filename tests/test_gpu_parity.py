@@ -269,3 +269,31 @@ def test_export_model(tmp_path):
             k, c = kc.split(":")
             row[int(k)] = int(c)
         assert np.array_equal(row, dt[d])
+
+
+@pytest.mark.parametrize("K", [1024, 2048, 4096, 8192])
+def test_compact_rows_with_pieces(K):
+    """Counts far above a compact entry's piece (63/31/15/7): rows, topics
+    and log-likelihood bit-exact with the oracle through several sweeps."""
+    off, words = make_corpus(9, 40, 300, 500, zipf=1.2)
+    z0 = (np.arange(len(words)) % 3 == 0).astype(np.uint16) * 5  # topics 0 and 5 only
+    ctx, orc = _pair(off, words, 300, K, z0=z0)
+    for s in range(3):
+        ctx.build_wt()
+        orc.build_wt()
+        ctx.topk()
+        orc.topk()
+        ctx.build_dt()
+        orc.build_dt()
+        ge, gn = ctx.get_dt_raw()
+        oe, on = orc.get_dt_raw()
+        assert np.array_equal(gn, on), s
+        doff = ezlda.dt_offsets(off, K)
+        for d in range(len(gn)):
+            assert np.array_equal(ge[doff[d]: doff[d] + gn[d]], oe[doff[d]: doff[d] + on[d]]), (s, d)
+        if s == 0:
+            assert (ge >> 16).max() == ezlda.dt_piece(K)  # pieces are present
+        ctx.sample()
+        orc.sample()
+        assert np.array_equal(ctx.get_topics(), orc.get_topics()), s
+    assert ctx.loglik() == pytest.approx(orc.loglik(), rel=1e-9)

@@ -132,13 +132,14 @@ def _check_rows(o, off, K):
         for k in np.nonzero(dense[d])[0]:
             c = int(dense[d, k])
             if K <= ezlda.COMPACT_K:
-                exp += [(k, 63)] * ((c - 1) // 63) + [(k, c - 63 * ((c - 1) // 63))]
+                p = ezlda.dt_piece(K)
+                exp += [(k, p)] * ((c - 1) // p) + [(k, c - p * ((c - 1) // p))]
             else:
                 exp.append((k, c))
         assert list(zip(tk.tolist(), ct.tolist())) == exp
 
 
-@pytest.mark.parametrize("K", [2, 5, 1025])
+@pytest.mark.parametrize("K", [2, 5, 1025, 2048, 4097, 8193])
 def test_dt_rows_split_large_counts(K):
     # long docs over few topics: counts far above 63 (split when K <= 1024)
     off, words = make_corpus(7, 12, 30, 600)
