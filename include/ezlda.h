@@ -256,7 +256,10 @@ const char *ezlda_server_last_error(const ezlda_server *srv);
 /* Queues one document (input word ids). */
 int ezlda_server_submit(ezlda_server *srv, uint32_t request_id, const int32_t *words, int32_t n);
 /* Blocks until the request is done (timeout_ms < 0: no limit) and copies its
- * final topics (n values) and topic counts (K values); either may be NULL. */
+ * final topics (n values) and topic counts (K values); either may be NULL.
+ * The result is then released.  An id that is neither queued, resident nor
+ * done is EZLDA_EINVAL; so is submitting an id that is in flight or has an
+ * unclaimed result. */
 int ezlda_server_wait(ezlda_server *srv, uint32_t request_id, uint16_t *z_out, int32_t *counts_out,
                       double *latency_ms, int32_t timeout_ms);
 /* Blocks until no request is queued or resident. */

@@ -24,6 +24,7 @@
 #include <string>
 #include <thread>
 #include <unordered_map>
+#include <unordered_set>
 #include <vector>
 
 #include "../../include/ezlda.h"
@@ -81,6 +82,7 @@ struct ezlda_server {
     std::vector<Slot> slots;
     std::deque<Request> pending;
     std::unordered_map<uint32_t, Result> done;
+    std::unordered_set<uint32_t> inflight;  // queued or resident request ids
     std::mutex mu;
     std::condition_variable cv_work, cv_done;
     std::thread worker;
@@ -244,6 +246,7 @@ void worker_main(ezlda_server *s) {
             s->completed++;
             s->t_last = t;
             s->done[x.rid] = std::move(r);
+            s->inflight.erase(x.rid);
             pool_free(s, x.off, x.n);
             x.active = false;
             any = true;
@@ -337,6 +340,8 @@ int ezlda_server_submit(ezlda_server *s, uint32_t request_id, const int32_t *wor
     }
     if (s->worker_rc != EZLDA_OK) return sfail(s, s->worker_rc, s->err);
     if (s->done.count(request_id)) return sfail(s, EZLDA_EINVAL, "request id already has an unclaimed result");
+    if (s->inflight.count(request_id)) return sfail(s, EZLDA_EINVAL, "request id already in flight");
+    s->inflight.insert(request_id);
     rq.t_submit = now_ms();
     if (s->t_first < 0) s->t_first = rq.t_submit;
     s->submitted++;
@@ -349,6 +354,8 @@ int ezlda_server_wait(ezlda_server *s, uint32_t request_id, uint16_t *z_out, int
                       int32_t timeout_ms) {
     if (!s) return sfail(nullptr, EZLDA_EINVAL, "null server");
     std::unique_lock<std::mutex> lk(s->mu);
+    if (!s->done.count(request_id) && !s->inflight.count(request_id))
+        return sfail(s, EZLDA_EINVAL, "unknown fold-in request " + std::to_string(request_id));
     auto ready = [&] { return s->done.count(request_id) || s->worker_rc != EZLDA_OK; };
     if (timeout_ms < 0)
         s->cv_done.wait(lk, ready);

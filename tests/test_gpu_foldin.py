@@ -113,9 +113,15 @@ def test_budget_rejects_oversized_and_bad_requests():
     with pytest.raises(ezlda.EzldaError):
         srv.submit(3, np.zeros(70000, np.int32))
     srv.submit(4, np.zeros(100, np.int32))  # exactly the budget
+    with pytest.raises(ezlda.EzldaError):
+        srv.submit(4, np.zeros(3, np.int32))  # the same id while in flight
     z, c, _ = srv.wait(4, 100, timeout_ms=30000)
     assert c.sum() == 100 and len(z) == 100
     assert srv.stats().rejected == 1
+    with pytest.raises(ezlda.EzldaError):
+        srv.wait(4, 100, timeout_ms=1000)  # already claimed
+    with pytest.raises(ezlda.EzldaError):
+        srv.wait(12345, 1, timeout_ms=-1)  # never submitted: no wait
     srv.close()
 
 
