@@ -161,6 +161,7 @@ def serving_bench(ctx, ezlda, wl: dict, args) -> dict:
                 _wait_until(at[j])
             srv.submit(rid, docs[rid % len(docs)])
             rid += 1
+        submit_s = time.perf_counter() - t0
         srv.drain()
         wall = time.perf_counter() - t0
         st = srv.stats()
@@ -170,14 +171,15 @@ def serving_bench(ctx, ezlda, wl: dict, args) -> dict:
             out[r] = c
         lvl = {"offered_qps": qps, "docs": n, "docs_per_s": st.docs_per_s, "p50_ms": st.p50_ms,
                "p99_ms": st.p99_ms, "max_ms": st.max_ms, "mean_ms": st.mean_ms, "max_active": st.max_active,
-               "launches": st.steps, "wall_s": wall}
+               "launches": st.steps, "wall_s": wall, "submit_s": submit_s}
         return lvl, out
 
     run(min(256, n_burst), None)  # warm-up
     burst, burst_counts = run(n_burst, None)
     cap = burst["docs_per_s"]
     log(f"serving burst: {n_burst} docs, {cap:.0f} docs/s, p50 {burst['p50_ms']:.1f} ms, "
-        f"p99 {burst['p99_ms']:.1f} ms, max active {burst['max_active']}")
+        f"p99 {burst['p99_ms']:.1f} ms, max active {burst['max_active']}, submitted in {burst['submit_s'] * 1e3:.1f} ms, "
+        f"{burst['launches']} steps")
     levels = []
     for f in fr:
         qps = f * cap

@@ -234,12 +234,17 @@ int ezlda_load_checkpoint(ezlda_ctx *ctx, const char *path);
  * not depend on what else is in flight.  Thread-safe: submit and wait may be
  * called from any thread; one internal worker thread drives the device. */
 typedef struct ezlda_server ezlda_server;
+/* Device bytes per resident token: words, topics, row pool, word-major
+ * token list, result staging. */
+#define EZLDA_FOLDIN_TOKEN_BYTES 16
 typedef struct ezlda_server_args {
     int32_t max_slots;        /* documents resident at once */
     int32_t sweeps;           /* fold-in sweeps per document (e.g. 20) */
     int64_t hbm_budget_bytes; /* device bytes for resident request state
-                                 (6 bytes per token); admission waits for room,
-                                 a document larger than all of it is rejected */
+                                 (EZLDA_FOLDIN_TOKEN_BYTES per token; slots
+                                 and per-word step buffers come on top);
+                                 admission waits for room, a document larger
+                                 than all of it is rejected */
     uint64_t seed;
 } ezlda_server_args;
 
@@ -266,7 +271,7 @@ int ezlda_server_wait(ezlda_server *srv, uint32_t request_id, uint16_t *z_out, i
 int ezlda_server_drain(ezlda_server *srv);
 int ezlda_server_get_stats(ezlda_server *srv, ezlda_server_stats *stats);
 int ezlda_server_reset_stats(ezlda_server *srv);
-/* The plain path: one document, one launch per sweep, synchronous, on its
+/* The plain path: one document, one step per sweep, synchronous, on its
  * own stream; same results as the server. */
 int ezlda_foldin_plain(ezlda_server *srv, uint32_t request_id, const int32_t *words, int32_t n, uint16_t *z_out,
                        int32_t *counts_out);

@@ -70,9 +70,33 @@ hipError_t launch_copy(const void *src, void *dst, size_t bytes, hipStream_t st)
 // 64-topic slabs (qs, V x K) and over slab totals (qacc, V x ceil(K/64)).
 hipError_t launch_foldin_tables(const DevState &s, float *phi, float *qs, float *qacc, hipStream_t st);
 // One sweep of every slot (rid, first token, n, sweep) in the pools.
-hipError_t launch_foldin_step(const uint4 *slots, int nslots, int K, int sweeps_total, float alpha, const float *phi,
-                              const float *qs, const float *qacc, const uint32_t *wpool, uint16_t *zpool,
-                              int32_t *counts_out, uint64_t seed, hipStream_t st);
+// Buffers of the word-major fold-in step.
+struct FoldinBatch {
+    uint32_t *rows = nullptr;       // row pool, parallel to the token pool: topic | count << 16
+    uint32_t *row_nnz = nullptr;    // per slot
+    uint32_t *wcount = nullptr;     // V: tokens per word in the step
+    uint32_t *wcursor = nullptr;    // V: bucket cursors
+    uint32_t *bsum = nullptr;       // 3 x ceil(V / 1024): per-block tokens, big and small items
+    uint32_t *tokw = nullptr;       // resident tokens: slot << 16 | position, by word
+    uint4 *items = nullptr;         // foldin_max_items(V, tokens) big items: (word, first, end, 0)
+    uint4 *small = nullptr;         // V small items
+    uint32_t *ctr = nullptr;        // [0] big items, [2] small items
+    int32_t *counts_out = nullptr;  // per finishing document: K topic counts
+    uint16_t *zout = nullptr;       // finishing documents' topics, packed
+    int grid = 0;                   // persistent sampling workgroups
+};
+size_t foldin_lds_bytes(int K);
+uint32_t foldin_max_items(int V, uint32_t max_tokens);
+// Copies admitted documents (source offset, pool offset, tokens, 0) from an
+// upload buffer into the word pool.
+hipError_t launch_foldin_admit(const uint4 *adm, int nadm, const uint32_t *src, uint32_t *wpool, hipStream_t st);
+// One fold-in sweep of every slot (request id, pool offset (a multiple of 8),
+// tokens, sweep); fin[j] = (output index or ~0, topic offset) for the
+// documents that finish with this sweep; ntok = their tokens.  nslots <=
+// 65535 documents of <= 65535 tokens.
+hipError_t launch_foldin_step(const FoldinBatch &fb, const uint4 *slots, const uint2 *fin, int nslots, uint32_t ntok,
+                              int V, int K, float alpha, const float *phi, const float *qs, const float *qacc,
+                              const uint32_t *wpool, uint16_t *zpool, uint64_t seed, hipStream_t st);
 
 // Bytes of dynamic LDS the sampler needs for K topics and block size bd.
 size_t sample_lds_bytes(int K, int bd);

@@ -6,12 +6,14 @@
 // keeps it on the device with the per-word Q running sums.  Requests carry a
 // document; a worker thread admits them into device slots while slots and
 // the HBM token pool (the budget) allow, and advances every resident request
-// by one fold-in sweep per launch (k_foldin_step), so requests that arrive
-// while others run join the next launch.  A request leaves after `sweeps`
-// sweeps; its pool range and slot are reused.  Draws are keyed by (seed,
-// request id, token, sweep), never by slot or position, so a result is the
-// same alone and in any mix.  ezlda_foldin_plain runs one document per launch
-// on its own stream, the comparison path.
+// by one fold-in sweep per step, so requests that arrive while others run
+// join the next step.  A step is one upload (slot table, finishing table,
+// admitted documents' words), the word-major fold-in kernels, and one copy
+// back of the finishing documents' counts and topics.  A request leaves
+// after `sweeps` sweeps; its pool range and slot are reused.  Draws are keyed
+// by (seed, request id, token, sweep), never by slot or position, so a result
+// is the same alone and in any mix.  ezlda_foldin_plain runs one document
+// per step on its own stream, the comparison path.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -55,6 +57,26 @@ struct Result {
     double latency_ms = 0.0;
 };
 
+// A document finishing with a step: where its counts and topics land in the
+// step's result copy.
+struct Finishing {
+    uint32_t rid = 0, n = 0, fidx = 0, zoff = 0;
+    double t_submit = 0.0;
+};
+
+// Host side of one step in flight: pinned upload (slot table, finishing
+// table, admitted documents), pinned result staging (grown on demand), the
+// event after its result copies, and the documents it finishes.
+struct StepRec {
+    uint8_t *up_h = nullptr;
+    int32_t *cnt_h = nullptr;
+    uint16_t *z_h = nullptr;
+    size_t cnt_cap = 0, z_cap = 0;
+    hipEvent_t ev = nullptr;
+    std::vector<Finishing> fins;
+    bool busy = false;
+};
+
 }  // namespace
 
 struct ezlda_server {
@@ -62,24 +84,28 @@ struct ezlda_server {
     float alpha = 0.f;
     uint64_t seed = 0;
     std::vector<int32_t> rank_of;
-    // device: frozen model, pools, slot table, per-slot counts
+    // device: frozen model, pools (in tokens; ranges are multiples of 8),
+    // step buffers, upload buffer
     float *phi = nullptr, *qs = nullptr, *qacc = nullptr;
     uint32_t *wpool = nullptr;
     uint16_t *zpool = nullptr;
-    int32_t *counts = nullptr;
-    uint4 *dslots = nullptr;
-    uint4 *hslots = nullptr;  // pinned
-    uint32_t pool_tokens = 0;
+    ez::FoldinBatch fb;
+    uint8_t *up_d = nullptr;
+    size_t up_bytes = 0;
+    uint32_t budget_tokens = 0, pool_tokens = 0, up_cap = 0;
+    StepRec rec[2];     // two steps in flight: host buffers and finishing documents
+    int64_t retiring = 0;  // documents done on the device, results not yet published
     // plain path buffers
     uint32_t *pw = nullptr;
     uint16_t *pz = nullptr;
-    int32_t *pc = nullptr;
-    uint4 *pslot = nullptr;
+    uint8_t *phdr = nullptr;
+    ez::FoldinBatch pfb;
     hipStream_t st = nullptr, pst = nullptr;
     std::vector<void *> allocs;
     // token pool allocator: free ranges offset -> length
     std::map<uint32_t, uint32_t> free_ranges;
     std::vector<Slot> slots;
+    std::vector<int> free_slots, active;
     std::deque<Request> pending;
     std::unordered_map<uint32_t, Result> done;
     std::unordered_set<uint32_t> inflight;  // queued or resident request ids
@@ -156,7 +182,78 @@ void pool_free(ezlda_server *s, uint32_t off, uint32_t n) {
     }
 }
 
-// Worker: admit, step, retire, until stopped.
+uint32_t round8(uint32_t n) { return (n + 7u) & ~7u; }
+size_t align16(size_t n) { return (n + 15u) & ~(size_t)15u; }
+
+// Grows a pinned host buffer to hold `need` elements (contents not kept).
+template <class T>
+bool ensure_pinned(T **p, size_t *cap, size_t need) {
+    if (need <= *cap) return true;
+    const size_t n = std::max(need, *cap * 2);
+    if (*p) (void)hipHostFree(*p);
+    *p = nullptr;
+    *cap = 0;
+    if (hipHostMalloc((void **)p, n * sizeof(T), hipHostMallocDefault) != hipSuccess) {
+        *p = nullptr;
+        return false;
+    }
+    *cap = n;
+    return true;
+}
+
+// Upload buffer layout for ns slots and na admitted documents:
+// [slots: ns uint4][fin: ns uint2][pad to 16][adm: na uint4][words]
+struct UpLayout {
+    size_t fin, adm, words, end;
+};
+UpLayout up_layout(size_t ns, size_t na, size_t nw) {
+    UpLayout l;
+    l.fin = ns * sizeof(uint4);
+    l.adm = align16(l.fin + ns * sizeof(uint2));
+    l.words = l.adm + na * sizeof(uint4);
+    l.end = l.words + nw * sizeof(uint32_t);
+    return l;
+}
+
+// Waits for a step's result copies and publishes its finishing documents.
+bool publish(ezlda_server *s, StepRec &R) {
+    const hipError_t e = hipEventSynchronize(R.ev);
+    R.busy = false;
+    if (e != hipSuccess) {
+        std::lock_guard<std::mutex> lk(s->mu);
+        s->worker_rc = EZLDA_EHIP;
+        s->err = std::string("fold-in step failed: ") + hipGetErrorString(e);
+        s->cv_done.notify_all();
+        return false;
+    }
+    std::vector<Result> res(R.fins.size());
+    for (size_t f = 0; f < R.fins.size(); ++f) {
+        const Finishing &x = R.fins[f];
+        res[f].z.assign(R.z_h + x.zoff, R.z_h + x.zoff + x.n);
+        res[f].counts.assign(R.cnt_h + (size_t)x.fidx * s->K, R.cnt_h + (size_t)(x.fidx + 1) * s->K);
+    }
+    std::lock_guard<std::mutex> lk(s->mu);
+    const double t = now_ms();
+    for (size_t f = 0; f < R.fins.size(); ++f) {
+        Result &r = res[f];
+        r.latency_ms = t - R.fins[f].t_submit;
+        s->lat.push_back(r.latency_ms);
+        s->completed++;
+        s->t_last = t;
+        s->done[R.fins[f].rid] = std::move(r);
+        s->inflight.erase(R.fins[f].rid);
+    }
+    s->retiring -= (int64_t)R.fins.size();
+    if (!R.fins.empty()) s->cv_done.notify_all();
+    R.fins.clear();
+    return true;
+}
+
+// Worker: admit, build and enqueue a step, publish the previous one, until
+// stopped.  Two steps are in flight: the host builds step s + 1 while the
+// device runs step s.  Documents finishing with a step give back their slot
+// and pool range when the step is built: the stream runs the step (and its
+// result copies) before any later step's upload or admission touches them.
 void worker_main(ezlda_server *s) {
     if (hipSetDevice(s->device) != hipSuccess) {
         std::lock_guard<std::mutex> lk(s->mu);
@@ -165,94 +262,133 @@ void worker_main(ezlda_server *s) {
         s->cv_done.notify_all();
         return;
     }
-    std::vector<int> act;
-    std::vector<uint16_t> zbuf;
+    std::vector<Request> adm;
+    std::vector<uint4> adm_rec;
+    int cur = 0;
     while (true) {
+        StepRec &R = s->rec[cur], &P = s->rec[cur ^ 1];
+        adm.clear();
+        adm_rec.clear();
+        uint32_t up_words = 0, ns = 0, nfin = 0, zsum = 0, ntok = 0;
+        UpLayout L{};
         {
             std::unique_lock<std::mutex> lk(s->mu);
-            // admission: free slot and pool room, in arrival order
-            while (!s->pending.empty()) {
-                auto fs = std::find_if(s->slots.begin(), s->slots.end(), [](const Slot &x) { return !x.active; });
-                if (fs == s->slots.end()) break;
-                uint32_t off = 0;
+            // admission: a free slot, pool room and upload room, in arrival order
+            while (!s->pending.empty() && !s->free_slots.empty()) {
                 const uint32_t n = (uint32_t)s->pending.front().words.size();
-                if (!pool_alloc(s, n, &off)) break;
-                Request rq = std::move(s->pending.front());
+                if (up_words + n > s->up_cap) break;
+                uint32_t off = 0;
+                if (!pool_alloc(s, round8(n), &off)) break;
+                adm.push_back(std::move(s->pending.front()));
                 s->pending.pop_front();
-                fs->active = true;
-                fs->rid = rq.rid;
-                fs->off = off;
-                fs->n = n;
-                fs->sweep = 0;
-                fs->t_submit = rq.t_submit;
-                // the pageable source must stay valid until the copy is done
-                if (n && (hipMemcpyAsync(s->wpool + off, rq.words.data(), n * sizeof(uint32_t), hipMemcpyHostToDevice,
-                                         s->st) != hipSuccess ||
-                          hipStreamSynchronize(s->st) != hipSuccess)) {
-                    s->worker_rc = EZLDA_EHIP;
-                    s->err = "fold-in upload failed";
-                }
+                const int j = s->free_slots.back();
+                s->free_slots.pop_back();
+                Slot &x = s->slots[j];
+                x.active = true;
+                x.rid = adm.back().rid;
+                x.off = off;
+                x.n = n;
+                x.sweep = 0;
+                x.t_submit = adm.back().t_submit;
+                s->active.push_back(j);
+                adm_rec.push_back(make_uint4(up_words, off, n, 0u));
+                up_words += n;
             }
-            act.clear();
-            for (int i = 0; i < s->max_slots; ++i)
-                if (s->slots[i].active) act.push_back(i);
-            s->max_active = std::max(s->max_active, (int)act.size());
-            if (act.empty()) {
+            s->max_active = std::max(s->max_active, (int)s->active.size());
+            if (s->active.empty() && !P.busy) {
                 if (s->stop) return;
                 s->cv_work.wait_for(lk, std::chrono::milliseconds(50));
                 continue;
             }
-        }
-        for (size_t j = 0; j < act.size(); ++j) {
-            const Slot &x = s->slots[act[j]];
-            s->hslots[j] = make_uint4(x.rid, x.off, x.n, x.sweep);
-        }
-        const int ns = (int)act.size();
-        hipError_t e = hipMemcpyAsync(s->dslots, s->hslots, ns * sizeof(uint4), hipMemcpyHostToDevice, s->st);
-        if (e == hipSuccess)
-            e = ez::launch_foldin_step(s->dslots, ns, s->K, s->sweeps, s->alpha, s->phi, s->qs, s->qacc, s->wpool,
-                                       s->zpool, s->counts, s->seed, s->st);
-        if (e == hipSuccess) e = hipStreamSynchronize(s->st);
-        std::unique_lock<std::mutex> lk(s->mu);
-        s->steps++;
-        if (e != hipSuccess) {
-            s->worker_rc = EZLDA_EHIP;
-            s->err = std::string("fold-in step failed: ") + hipGetErrorString(e);
-            for (int i : act) s->slots[i].active = false;
-            s->cv_done.notify_all();
-            return;
-        }
-        bool any = false;
-        for (size_t j = 0; j < act.size(); ++j) {
-            Slot &x = s->slots[act[j]];
-            if (++x.sweep < (uint32_t)s->sweeps) continue;
-            Result r;
-            r.z.resize(x.n);
-            r.counts.resize(s->K);
-            lk.unlock();
-            hipError_t ce = hipSuccess;
-            if (x.n) ce = hipMemcpy(r.z.data(), s->zpool + x.off, x.n * sizeof(uint16_t), hipMemcpyDeviceToHost);
-            if (ce == hipSuccess)
-                ce = hipMemcpy(r.counts.data(), s->counts + j * (size_t)s->K, s->K * sizeof(int32_t),
-                               hipMemcpyDeviceToHost);
-            lk.lock();
-            if (ce != hipSuccess) {
-                s->worker_rc = EZLDA_EHIP;
-                s->err = std::string("fold-in result copy failed: ") + hipGetErrorString(ce);
+            if (!s->active.empty()) {
+                // the slot table, the finishing table and the bookkeeping of
+                // this step: sweeps advance, finishing documents retire
+                ns = (uint32_t)s->active.size();
+                L = up_layout(ns, adm_rec.size(), up_words);
+                uint4 *hs = (uint4 *)R.up_h;
+                uint2 *hf = (uint2 *)(R.up_h + L.fin);
+                R.fins.clear();
+                for (uint32_t j = 0; j < ns; ++j) {
+                    const int si = s->active[j];
+                    Slot &x = s->slots[si];
+                    hs[j] = make_uint4(x.rid, x.off, x.n, x.sweep);
+                    ntok += x.n;
+                    if (++x.sweep == (uint32_t)s->sweeps) {
+                        Finishing f;
+                        f.rid = x.rid;
+                        f.n = x.n;
+                        f.fidx = nfin++;
+                        f.zoff = zsum;
+                        f.t_submit = x.t_submit;
+                        R.fins.push_back(f);
+                        hf[j] = make_uint2(f.fidx, f.zoff);
+                        zsum += x.n;
+                        pool_free(s, x.off, round8(x.n));
+                        x.active = false;
+                        s->free_slots.push_back(si);
+                    } else {
+                        hf[j] = make_uint2(0xFFFFFFFFu, 0u);
+                    }
+                }
+                if (nfin) {
+                    s->active.erase(std::remove_if(s->active.begin(), s->active.end(),
+                                                   [&](int i) { return !s->slots[i].active; }),
+                                    s->active.end());
+                    s->retiring += nfin;
+                }
             }
-            const double t = now_ms();
-            r.latency_ms = t - x.t_submit;
-            s->lat.push_back(r.latency_ms);
-            s->completed++;
-            s->t_last = t;
-            s->done[x.rid] = std::move(r);
-            s->inflight.erase(x.rid);
-            pool_free(s, x.off, x.n);
-            x.active = false;
-            any = true;
         }
-        if (any) s->cv_done.notify_all();
+        if (ns) {
+            if (!adm_rec.empty()) std::memcpy(R.up_h + L.adm, adm_rec.data(), adm_rec.size() * sizeof(uint4));
+            for (size_t k = 0; k < adm.size(); ++k)
+                if (!adm[k].words.empty())
+                    std::memcpy(R.up_h + L.words + (size_t)adm_rec[k].x * sizeof(uint32_t), adm[k].words.data(),
+                                adm[k].words.size() * sizeof(uint32_t));
+            hipError_t e = hipSuccess;
+            if (!ensure_pinned(&R.cnt_h, &R.cnt_cap, (size_t)nfin * s->K) ||
+                !ensure_pinned(&R.z_h, &R.z_cap, (size_t)zsum + 1))
+                e = hipErrorOutOfMemory;
+            if (e == hipSuccess) e = hipMemcpyAsync(s->up_d, R.up_h, L.end, hipMemcpyHostToDevice, s->st);
+            if (e == hipSuccess)
+                e = ez::launch_foldin_admit((const uint4 *)(s->up_d + L.adm), (int)adm_rec.size(),
+                                            (const uint32_t *)(s->up_d + L.words), s->wpool, s->st);
+            if (e == hipSuccess)
+                e = ez::launch_foldin_step(s->fb, (const uint4 *)s->up_d, (const uint2 *)(s->up_d + L.fin), (int)ns,
+                                           ntok, s->V, s->K, s->alpha, s->phi, s->qs, s->qacc, s->wpool, s->zpool,
+                                           s->seed, s->st);
+            if (e == hipSuccess && nfin)
+                e = hipMemcpyAsync(R.cnt_h, s->fb.counts_out, (size_t)nfin * s->K * sizeof(int32_t),
+                                   hipMemcpyDeviceToHost, s->st);
+            if (e == hipSuccess && zsum)
+                e = hipMemcpyAsync(R.z_h, s->fb.zout, (size_t)zsum * sizeof(uint16_t), hipMemcpyDeviceToHost, s->st);
+            if (e == hipSuccess) e = hipEventRecord(R.ev, s->st);
+            {
+                std::lock_guard<std::mutex> lk(s->mu);
+                s->steps++;
+                if (e != hipSuccess) {
+                    s->worker_rc = EZLDA_EHIP;
+                    s->err = std::string("fold-in step failed: ") + hipGetErrorString(e);
+                    s->cv_done.notify_all();
+                    return;
+                }
+            }
+            R.busy = true;
+        }
+        if (P.busy && !publish(s, P)) return;
+        if (ns) cur ^= 1;
     }
+}
+
+// Device buffers of one fold-in batch: `tokens` pool tokens, `nslots` slots.
+bool alloc_batch(ezlda_server *s, ez::FoldinBatch *b, uint32_t tokens, int nslots, int grid) {
+    const size_t nb = ((size_t)s->V + 1023) / 1024;
+    b->grid = grid;
+    return salloc(s, &b->rows, tokens) == EZLDA_OK && salloc(s, &b->row_nnz, nslots) == EZLDA_OK &&
+           salloc(s, &b->wcount, s->V) == EZLDA_OK && salloc(s, &b->wcursor, s->V) == EZLDA_OK &&
+           salloc(s, &b->bsum, 3 * nb) == EZLDA_OK && salloc(s, &b->tokw, tokens) == EZLDA_OK &&
+           salloc(s, &b->items, ez::foldin_max_items(s->V, tokens)) == EZLDA_OK &&
+           salloc(s, &b->small, s->V) == EZLDA_OK && salloc(s, &b->ctr, 4) == EZLDA_OK && salloc(s, &b->counts_out, (size_t)nslots * s->K) == EZLDA_OK &&
+           salloc(s, &b->zout, tokens) == EZLDA_OK;
 }
 
 }  // namespace
@@ -260,13 +396,17 @@ void worker_main(ezlda_server *s) {
 extern "C" {
 
 ezlda_server *ezlda_server_create(ezlda_ctx *ctx, const ezlda_server_args *a) {
-    if (!ctx || !a || a->max_slots < 1 || a->sweeps < 1 || a->hbm_budget_bytes < 6) {
-        ezc::set_error("bad fold-in server arguments");
+    if (!ctx || !a || a->max_slots < 1 || a->max_slots > 65535 || a->sweeps < 1 ||
+        a->hbm_budget_bytes < EZLDA_FOLDIN_TOKEN_BYTES) {
+        ezc::set_error("bad fold-in server arguments (max_slots 1..65535, sweeps >= 1, budget >= one token)");
         return nullptr;
     }
     if (ezc::refresh_counts(ctx) != EZLDA_OK) return nullptr;
     auto *s = new ezlda_server();
     if (hipGetDevice(&s->device) != hipSuccess) s->device = 0;
+    int cus = 0;
+    if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, s->device) != hipSuccess || cus < 1)
+        cus = 1;
     s->K = ctx->s.K;
     s->V = ctx->s.V;
     s->NS = (s->K + 63) / 64;
@@ -275,18 +415,25 @@ ezlda_server *ezlda_server_create(ezlda_ctx *ctx, const ezlda_server_args *a) {
     s->sweeps = a->sweeps;
     s->max_slots = a->max_slots;
     s->rank_of = ctx->rank_of;
-    const uint64_t tok = std::min<uint64_t>((uint64_t)a->hbm_budget_bytes / 6u, 0xFFFFFFF0ull);
-    s->pool_tokens = (uint32_t)tok;
+    // the pool holds the budget's tokens plus 8 per slot, the most that
+    // rounding every resident document up to a multiple of 8 can add
+    const uint64_t budget = std::min<uint64_t>((uint64_t)a->hbm_budget_bytes / EZLDA_FOLDIN_TOKEN_BYTES, 0xF0000000ull);
+    s->budget_tokens = (uint32_t)budget;
+    s->pool_tokens = round8(s->budget_tokens) + 8u * (uint32_t)s->max_slots;
+    s->up_cap = std::min<uint32_t>(s->pool_tokens, 1u << 22);
+    s->up_bytes = up_layout((size_t)s->max_slots, (size_t)s->max_slots, s->up_cap).end;
     bool ok = hipStreamCreateWithFlags(&s->st, hipStreamNonBlocking) == hipSuccess &&
               hipStreamCreateWithFlags(&s->pst, hipStreamNonBlocking) == hipSuccess;
     ok = ok && salloc(s, &s->phi, (size_t)s->V * s->K) == EZLDA_OK &&
          salloc(s, &s->qs, (size_t)s->V * s->K) == EZLDA_OK && salloc(s, &s->qacc, (size_t)s->V * s->NS) == EZLDA_OK &&
          salloc(s, &s->wpool, s->pool_tokens) == EZLDA_OK && salloc(s, &s->zpool, s->pool_tokens) == EZLDA_OK &&
-         salloc(s, &s->counts, (size_t)s->max_slots * s->K) == EZLDA_OK &&
-         salloc(s, &s->dslots, s->max_slots) == EZLDA_OK && salloc(s, &s->pw, 65536) == EZLDA_OK &&
-         salloc(s, &s->pz, 65536) == EZLDA_OK && salloc(s, &s->pc, s->K) == EZLDA_OK &&
-         salloc(s, &s->pslot, 1) == EZLDA_OK;
-    ok = ok && hipHostMalloc((void **)&s->hslots, s->max_slots * sizeof(uint4), hipHostMallocDefault) == hipSuccess;
+         salloc(s, &s->up_d, s->up_bytes) == EZLDA_OK &&
+         alloc_batch(s, &s->fb, s->pool_tokens, s->max_slots, 8 * cus) &&
+         salloc(s, &s->pw, 65536) == EZLDA_OK && salloc(s, &s->pz, 65536) == EZLDA_OK &&
+         salloc(s, &s->phdr, up_layout(1, 0, 0).end) == EZLDA_OK && alloc_batch(s, &s->pfb, 65536, 1, 8 * cus);
+    for (StepRec &R : s->rec)
+        ok = ok && hipHostMalloc((void **)&R.up_h, s->up_bytes, hipHostMallocDefault) == hipSuccess &&
+             hipEventCreateWithFlags(&R.ev, hipEventDisableTiming) == hipSuccess;
     ok = ok && ez::launch_foldin_tables(ctx->s, s->phi, s->qs, s->qacc, s->st) == hipSuccess &&
          hipStreamSynchronize(s->st) == hipSuccess;
     if (!ok) {
@@ -295,6 +442,7 @@ ezlda_server *ezlda_server_create(ezlda_ctx *ctx, const ezlda_server_args *a) {
         return nullptr;
     }
     s->slots.resize(s->max_slots);
+    for (int j = s->max_slots - 1; j >= 0; --j) s->free_slots.push_back(j);
     s->free_ranges[0] = s->pool_tokens;
     s->worker = std::thread(worker_main, s);
     return s;
@@ -315,7 +463,12 @@ void ezlda_server_destroy(ezlda_server *s) {
     if (s->st) (void)hipStreamSynchronize(s->st);
     if (s->pst) (void)hipStreamSynchronize(s->pst);
     for (void *p : s->allocs) (void)hipFree(p);
-    if (s->hslots) (void)hipHostFree(s->hslots);
+    for (StepRec &R : s->rec) {
+        if (R.up_h) (void)hipHostFree(R.up_h);
+        if (R.cnt_h) (void)hipHostFree(R.cnt_h);
+        if (R.z_h) (void)hipHostFree(R.z_h);
+        if (R.ev) (void)hipEventDestroy(R.ev);
+    }
     if (s->st) (void)hipStreamDestroy(s->st);
     if (s->pst) (void)hipStreamDestroy(s->pst);
     delete s;
@@ -334,7 +487,7 @@ int ezlda_server_submit(ezlda_server *s, uint32_t request_id, const int32_t *wor
         rq.words[i] = (uint32_t)s->rank_of[words[i]];
     }
     std::lock_guard<std::mutex> lk(s->mu);
-    if ((uint32_t)n > s->pool_tokens) {
+    if ((uint32_t)n > s->budget_tokens) {
         s->rejected++;
         return sfail(s, EZLDA_ENOMEM, "document larger than the server's HBM token budget");
     }
@@ -375,10 +528,7 @@ int ezlda_server_drain(ezlda_server *s) {
     std::unique_lock<std::mutex> lk(s->mu);
     s->cv_done.wait(lk, [&] {
         if (s->worker_rc != EZLDA_OK) return true;
-        if (!s->pending.empty()) return false;
-        for (const Slot &x : s->slots)
-            if (x.active) return false;
-        return true;
+        return s->pending.empty() && s->active.empty() && s->retiring == 0;
     });
     return s->worker_rc;
 }
@@ -393,7 +543,7 @@ int ezlda_server_get_stats(ezlda_server *s, ezlda_server_stats *st) {
     st->steps = s->steps;
     st->max_active = s->max_active;
     int act = 0;
-    for (const Slot &x : s->slots) act += x.active;
+    act = (int)s->active.size();
     st->active = act;
     st->pending = (int32_t)s->pending.size();
     if (!s->lat.empty()) {
@@ -433,15 +583,24 @@ int ezlda_foldin_plain(ezlda_server *s, uint32_t request_id, const int32_t *word
     }
     std::lock_guard<std::mutex> lk(s->plain_mu);
     if (n) SV_HIP(s, hipMemcpyAsync(s->pw, r.data(), n * sizeof(uint32_t), hipMemcpyHostToDevice, s->pst));
+    const UpLayout L = up_layout(1, 0, 0);
+    struct {
+        uint4 slot;
+        uint2 fin;
+    } hdr;
+    static_assert(sizeof(hdr) >= sizeof(uint4) + sizeof(uint2), "slot header layout");
     for (int sw = 0; sw < s->sweeps; ++sw) {
-        const uint4 sl = make_uint4(request_id, 0u, (uint32_t)n, (uint32_t)sw);
-        SV_HIP(s, hipMemcpyAsync(s->pslot, &sl, sizeof(uint4), hipMemcpyHostToDevice, s->pst));
-        SV_HIP(s, ez::launch_foldin_step(s->pslot, 1, s->K, s->sweeps, s->alpha, s->phi, s->qs, s->qacc, s->pw, s->pz,
-                                         s->pc, s->seed, s->pst));
+        hdr.slot = make_uint4(request_id, 0u, (uint32_t)n, (uint32_t)sw);
+        hdr.fin = sw + 1 == s->sweeps ? make_uint2(0u, 0u) : make_uint2(0xFFFFFFFFu, 0u);
+        SV_HIP(s, hipMemcpyAsync(s->phdr, &hdr.slot, sizeof(uint4), hipMemcpyHostToDevice, s->pst));
+        SV_HIP(s, hipMemcpyAsync(s->phdr + L.fin, &hdr.fin, sizeof(uint2), hipMemcpyHostToDevice, s->pst));
+        SV_HIP(s, ez::launch_foldin_step(s->pfb, (const uint4 *)s->phdr, (const uint2 *)(s->phdr + L.fin), 1,
+                                         (uint32_t)n, s->V, s->K, s->alpha, s->phi, s->qs, s->qacc, s->pw, s->pz, s->seed, s->pst));
         SV_HIP(s, hipStreamSynchronize(s->pst));
     }
-    if (n && z_out) SV_HIP(s, hipMemcpy(z_out, s->pz, n * sizeof(uint16_t), hipMemcpyDeviceToHost));
-    if (counts_out) SV_HIP(s, hipMemcpy(counts_out, s->pc, s->K * sizeof(int32_t), hipMemcpyDeviceToHost));
+    if (n && z_out) SV_HIP(s, hipMemcpy(z_out, s->pfb.zout, n * sizeof(uint16_t), hipMemcpyDeviceToHost));
+    if (counts_out)
+        SV_HIP(s, hipMemcpy(counts_out, s->pfb.counts_out, s->K * sizeof(int32_t), hipMemcpyDeviceToHost));
     return EZLDA_OK;
 }
 

@@ -868,59 +868,61 @@ __global__ void __launch_bounds__(256) k_foldin_tables(int V, int VD, int K, flo
     }
 }
 
-static constexpr int kFB = 8, kFCP = 16;  // fold-in S blocks and checkpoints
+static constexpr int kFB = 8, kFCP = 16;     // fold-in S blocks and checkpoints
+static constexpr uint32_t kFoldSeg = 256;     // tokens per big fold-in work item at most
+static constexpr uint32_t kFoldHot = 2048;    // word ranks counted in LDS before the global atomics
+static constexpr uint32_t kFoldStageQ = 64;   // tokens of a word in a step that make big items
 
-// Running FMA sum over kFB row entries (LDS) with phi gathered from the
-// frozen table: the gathers are issued together, the FMAs in order.
-__device__ __forceinline__ float foldin_block(const uint32_t *row, const float *__restrict__ ph, float s) {
-    uint32_t e[kFB];
-    float f[kFB];
-#pragma unroll
-    for (int u = 0; u < kFB; ++u) e[u] = row[u];
-#pragma unroll
-    for (int u = 0; u < kFB; ++u) f[u] = ph[e[u] & 0xFFFFu];
-#pragma unroll
-    for (int u = 0; u < kFB; ++u) s = __builtin_fmaf((float)(e[u] >> 16), f[u], s);
-    return s;
+// A fold-in step runs in word-major order, as the training sweep does: the
+// resident documents' rows are built (k_fi_rows), their tokens are bucketed
+// by word (k_fi_count, the three-pass scan, k_fi_scatter), and persistent
+// workgroups take one (word, segment) item at a time, stage the frozen phi
+// row and Q running sums in LDS and draw its tokens (k_fi_sample); documents
+// finishing their last sweep get their topic counts and topics packed for
+// one copy to the host (k_fi_final).  The order tokens are drawn in never
+// changes a draw, which is keyed by (seed, request id, position, sweep).
+// Slots are (request id, pool offset, tokens, sweep); pool offsets are
+// multiples of 8, and a document's row lives in the row pool at its offset
+// (at most min(n, K) entries, zero padded to a multiple of 8 <= round8(n)).
+
+// Copies admitted documents' words from the upload buffer into the pool:
+// records (source offset, pool offset, tokens, 0).
+__global__ void __launch_bounds__(256) k_fi_admit(const uint4 *__restrict__ adm, int nadm,
+                                                  const uint32_t *__restrict__ src, uint32_t *__restrict__ wpool) {
+    for (int a = blockIdx.x; a < nadm; a += gridDim.x) {
+        const uint4 r = adm[a];
+        for (uint32_t i = threadIdx.x; i < r.z; i += blockDim.x) wpool[r.y + i] = src[r.x + i];
+    }
 }
 
-// One fold-in sweep for every active request slot (rid, first token, n,
-// sweep): one workgroup per slot.  The doc's topic counts are histogrammed
-// in LDS from its current z and compacted into an ascending row; then one
-// lane per token draws from S (the row, FMA running sum, phi gathered from
-// the frozen table) and Q (the word's precomputed running sums).  The first
-// sweep draws the initial topics; the last one writes the doc's counts.
-__global__ void __launch_bounds__(256) k_foldin_step(
-    const uint4 *__restrict__ slots, int nslots, int K, int sweeps_total, float alpha,
-    const float *__restrict__ phi, const float *__restrict__ qs, const float *__restrict__ qacc,
-    const uint32_t *__restrict__ wpool, uint16_t *__restrict__ zpool, int32_t *__restrict__ counts_out,
-    uint64_t seed) {
-    extern __shared__ uint32_t shf[];
-    __shared__ uint32_t s_nz;
-    const int NS = (K + 63) >> 6;
+// Rows of the resident documents (one workgroup per slot): topics drawn from
+// the init stream on sweep 0, a 16-bit LDS histogram, a row of topic | count
+// << 16 in ascending topic order, zero padded to a multiple of kFB.
+__global__ void __launch_bounds__(256) k_fi_rows(const uint4 *__restrict__ slots, int nslots, int K, uint64_t seed,
+                                                 uint16_t *__restrict__ zpool, uint32_t *__restrict__ rows,
+                                                 uint32_t *__restrict__ row_nnz) {
+    extern __shared__ uint32_t shr[];
     const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6, bd = blockDim.x;
-    const int KH = (K + 1) >> 1, KR = (K + kFB - 1) / kFB * kFB;
-    uint32_t *h = shf;        // the doc's 16-bit topic counts
-    uint32_t *row = h + KH;   // its sorted row, zero padded to a block
+    const int KH = (K + 1) >> 1;
+    uint32_t *h = shr;
     for (int k = tid; k < KH; k += bd) h[k] = 0;
     __syncthreads();
-    for (int si = blockIdx.x; si < nslots; si += gridDim.x) {
-        const uint4 sl = slots[si];
+    for (int j = blockIdx.x; j < nslots; j += gridDim.x) {
+        const uint4 sl = slots[j];
         const uint32_t rid = sl.x, off = sl.y, n = sl.z, sweep = sl.w;
-        const uint64_t key = (uint64_t)rid << 32;
         uint16_t *z = zpool + off;
-        const uint32_t *wd = wpool + off;
         if (sweep == 0) {
             for (uint32_t i = tid; i < n; i += bd)
-                z[i] = (uint16_t)(ez_philox_x0(seed, key | i, 0u, EZ_TAG_FOLDIN_INIT) % (uint32_t)K);
+                z[i] = (uint16_t)(ez_philox_x0(seed, ((uint64_t)rid << 32) | i, 0u, EZ_TAG_FOLDIN_INIT) % (uint32_t)K);
             __syncthreads();
         }
         for (uint32_t i = tid; i < n; i += bd) h16_add(h, z[i]);
         __syncthreads();
         if (wv == 0) {  // one wavefront compacts the row (and clears the counts)
+            uint32_t *row = rows + off;
             uint32_t nz = 0;
-            for (int j = 0; j < K; j += 64) {
-                const int k = j + lane;
+            for (int c0 = 0; c0 < K; c0 += 64) {
+                const int k = c0 + lane;
                 const uint32_t c = k < K ? h16_get(h, (uint32_t)k) : 0u;
                 const uint64_t m = __ballot(c != 0);
                 if (c) row[nz + __popcll(m & lanemask_lt(lane))] = (uint32_t)k | (c << 16);
@@ -930,109 +932,400 @@ __global__ void __launch_bounds__(256) k_foldin_step(
             }
             // zero entries up to the next block: count 0 adds exactly nothing
             if ((uint32_t)lane < ((kFB - (nz % kFB)) % kFB)) row[nz + lane] = 0u;
-            if (lane == 0) s_nz = nz;
+            if (lane == 0) row_nnz[j] = nz;
         }
         __syncthreads();
-        const uint32_t nz = s_nz;
-        for (uint32_t base = 0; base < n; base += (uint32_t)bd) {
-            const uint32_t i = base + (uint32_t)tid;
-            if (i < n) {
-                const uint32_t w = wd[i];
-                const float *ph = phi + (size_t)w * K;
-                const float *qa = qacc + (size_t)w * NS;
-                const float u = ez_u24(ez_philox_x0(seed, key | i, sweep, EZ_TAG_FOLDIN));
-                // S over the row in blocks of kFB entries (8 independent phi
-                // gathers in flight per lane), zero padding adding nothing;
-                // cp[b] = running sum after block b, for the first kFCP blocks
-                const uint32_t nb = (nz + kFB - 1) / kFB;
-                float cp[kFCP];
-                float S = 0.0f;
+    }
+}
+
+// Tokens per word of the resident documents; the most frequent ranks are
+// summed in LDS first so they take one global atomic per workgroup.
+__global__ void __launch_bounds__(256) k_fi_count(const uint4 *__restrict__ slots, int nslots,
+                                                  const uint32_t *__restrict__ wpool, uint32_t *__restrict__ wcount) {
+    __shared__ uint32_t hc[kFoldHot];
+    const int tid = threadIdx.x, bd = blockDim.x;
+    for (uint32_t k = tid; k < kFoldHot; k += bd) hc[k] = 0;
+    __syncthreads();
+    for (int j = blockIdx.x; j < nslots; j += gridDim.x) {
+        const uint4 sl = slots[j];
+        for (uint32_t i = tid; i < sl.z; i += bd) {
+            const uint32_t w = wpool[sl.y + i];
+            if (w < kFoldHot) atomicAdd(&hc[w], 1u);
+            else atomicAdd(&wcount[w], 1u);
+        }
+    }
+    __syncthreads();
+    for (uint32_t k = tid; k < kFoldHot; k += bd)
+        if (hc[k]) atomicAdd(&wcount[k], hc[k]);
+}
+
+// Exclusive scan of M values per thread over a workgroup of up to 1024
+// threads; the totals go to tot.  sh holds 16 * M words.
+template <int M>
+__device__ __forceinline__ void block_scan(uint32_t (&v)[M], uint32_t *sh, uint32_t (&tot)[M]) {
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6, nw = blockDim.x >> 6;
+    uint32_t inc[M];
 #pragma unroll
-                for (int b = 0; b < kFCP; ++b) {
-                    if ((uint32_t)b < nb) S = foldin_block(row + kFB * b, ph, S);
-                    cp[b] = S;
-                }
-                for (uint32_t b = kFCP; b < nb; ++b) S = foldin_block(row + kFB * b, ph, S);
-                const float target = u * (S + qa[NS - 1]);
-                int zz = -1;
-                if (target < S) {
-                    // the first block whose running sum exceeds the target,
-                    // then its entries in order (a running sum above the
-                    // target exists: the last one is S)
-                    int bs = -1;
+    for (int m = 0; m < M; ++m) inc[m] = v[m];
+    for (int d = 1; d < 64; d <<= 1) {
 #pragma unroll
-                    for (int b = kFCP - 1; b >= 0; --b)
-                        if (cp[b] > target) bs = b;
-                    float run = 0.0f;
-                    uint32_t g0 = 0, g1 = nz;
-                    if (bs >= 0) {
+        for (int m = 0; m < M; ++m) {
+            const uint32_t x = __shfl_up(inc[m], d);
+            if (lane >= d) inc[m] += x;
+        }
+    }
+    if (lane == 63)
 #pragma unroll
-                        for (int b = 0; b < kFCP - 1; ++b)
-                            if (b + 1 == bs) run = cp[b];
-                        g0 = (uint32_t)bs * kFB;
-                        g1 = min(nz, g0 + kFB);
-                    } else {
-                        run = cp[kFCP - 1];
-                        g0 = kFCP * kFB;
-                    }
-                    for (uint32_t j = g0; j < g1; ++j) {
-                        const uint32_t e = row[j];
-                        run = __builtin_fmaf((float)(e >> 16), ph[e & 0xFFFFu], run);
-                        if (run > target) {
-                            zz = (int)(e & 0xFFFFu);
-                            break;
-                        }
-                    }
-                    if (zz < 0)  // not reached: kept as the canonical fallback
-                        for (uint32_t j = 0; j < nz; ++j)
-                            if (row[j] >> 16 && ph[row[j] & 0xFFFFu] > 0.0f) zz = (int)(row[j] & 0xFFFFu);
-                } else {
-                    const float t2 = target - S;
-                    int lo = 0, hi = NS;
-                    while (lo < hi) {
-                        const int mid = (lo + hi) >> 1;
-                        if (qa[mid] > t2) hi = mid; else lo = mid + 1;
-                    }
-                    int js = lo < NS ? lo : -1;
-                    if (js < 0)
-                        for (int j = NS - 1; j >= 0; --j)
-                            if (qa[j] > (j > 0 ? qa[j - 1] : 0.0f)) {
-                                js = j;
-                                break;
-                            }
-                    if (js >= 0) {
-                        const float t3 = t2 - (js > 0 ? qa[js - 1] : 0.0f);
-                        const float *qw = qs + (size_t)w * K;
-                        const int k0 = js * 64, ke = min(k0 + 64, K);
-                        lo = k0;
-                        hi = ke;
-                        while (lo < hi) {
-                            const int mid = (lo + hi) >> 1;
-                            if (qw[mid] > t3) hi = mid; else lo = mid + 1;
-                        }
-                        if (lo < ke) {
-                            zz = lo;
-                        } else {
-                            for (int k = ke - 1; k >= k0; --k)
-                                if (alpha * ph[k] > 0.0f) {
-                                    zz = k;
-                                    break;
-                                }
-                        }
-                    }
-                }
-                z[i] = (uint16_t)(zz < 0 ? 0 : zz);
+        for (int m = 0; m < M; ++m) sh[16 * m + wv] = inc[m];
+    __syncthreads();
+#pragma unroll
+    for (int m = 0; m < M; ++m) {
+        uint32_t o = 0, t = 0;
+        for (int w = 0; w < nw; ++w) {
+            const uint32_t x = sh[16 * m + w];
+            if (w < wv) o += x;
+            t += x;
+        }
+        tot[m] = t;
+        v[m] = o + inc[m] - v[m];
+    }
+    __syncthreads();
+}
+
+// A word's tokens in the step form one small item (fewer than kFoldStageQ
+// tokens: a wavefront draws them) or big items of at most kFoldSeg tokens
+// (a workgroup draws each).
+__device__ __forceinline__ uint32_t fold_big(uint32_t c) { return c >= kFoldStageQ ? (c + kFoldSeg - 1u) / kFoldSeg : 0u; }
+__device__ __forceinline__ uint32_t fold_small(uint32_t c) { return c && c < kFoldStageQ ? 1u : 0u; }
+
+// Scan pass 1: per 1024 words, their tokens, big items and small items.
+__global__ void __launch_bounds__(1024) k_fi_bsum(int V, const uint32_t *__restrict__ wcount,
+                                                  uint32_t *__restrict__ bsum) {
+    __shared__ uint32_t sh[48];
+    const uint32_t w = blockIdx.x * 1024u + threadIdx.x;
+    const uint32_t c = w < (uint32_t)V ? wcount[w] : 0u;
+    uint32_t v[3] = {c, fold_big(c), fold_small(c)}, t[3];
+    block_scan<3>(v, sh, t);
+    if (threadIdx.x == 0)
+        for (int m = 0; m < 3; ++m) bsum[3 * blockIdx.x + m] = t[m];
+}
+
+// Scan pass 2 (one workgroup): exclusive offsets of the per-block sums; the
+// item counts in ctr: [0] big items, [2] small items ([1] and [3] zero).
+__global__ void __launch_bounds__(1024) k_fi_btop(int nb, uint32_t *__restrict__ bsum, uint32_t *__restrict__ ctr) {
+    __shared__ uint32_t sh[48];
+    uint32_t carry[3] = {0u, 0u, 0u};
+    for (int c0 = 0; c0 < nb; c0 += 1024) {
+        const int i = c0 + (int)threadIdx.x;
+        uint32_t v[3], t[3];
+        for (int m = 0; m < 3; ++m) v[m] = i < nb ? bsum[3 * i + m] : 0u;
+        block_scan<3>(v, sh, t);
+        for (int m = 0; m < 3; ++m) {
+            if (i < nb) bsum[3 * i + m] = carry[m] + v[m];
+            carry[m] += t[m];
+        }
+    }
+    if (threadIdx.x == 0) {
+        ctr[0] = carry[1];
+        ctr[1] = 0u;
+        ctr[2] = carry[2];
+        ctr[3] = 0u;
+    }
+}
+
+// Scan pass 3: bucket starts (the scatter's cursors) and the items (word,
+// first, end, 0), big and small, in word order.
+__global__ void __launch_bounds__(1024) k_fi_bfin(int V, const uint32_t *__restrict__ wcount,
+                                                  const uint32_t *__restrict__ bsum, uint32_t *__restrict__ wcursor,
+                                                  uint4 *__restrict__ big, uint4 *__restrict__ small) {
+    __shared__ uint32_t sh[48];
+    const uint32_t w = blockIdx.x * 1024u + threadIdx.x;
+    const uint32_t c = w < (uint32_t)V ? wcount[w] : 0u;
+    uint32_t v[3] = {c, fold_big(c), fold_small(c)}, t[3];
+    block_scan<3>(v, sh, t);
+    if (w >= (uint32_t)V) return;
+    const uint32_t base = bsum[3 * blockIdx.x] + v[0];
+    wcursor[w] = base;
+    if (c >= kFoldStageQ) {
+        uint32_t ib = bsum[3 * blockIdx.x + 1] + v[1];
+        for (uint32_t x = 0; x < c; x += kFoldSeg) big[ib++] = make_uint4(w, base + x, base + min(c, x + kFoldSeg), 0u);
+    } else if (c) {
+        small[bsum[3 * blockIdx.x + 2] + v[2]] = make_uint4(w, base, base + c, 0u);
+    }
+}
+
+// Token references (slot << 16 | position) bucketed by word.  Ranks below
+// kFoldHot are counted in LDS, reserve one range per workgroup and fill it
+// through LDS cursors; the rest take a global cursor per token.
+__global__ void __launch_bounds__(256) k_fi_scatter(const uint4 *__restrict__ slots, int nslots,
+                                                    const uint32_t *__restrict__ wpool, uint32_t *__restrict__ wcursor,
+                                                    uint32_t *__restrict__ tokw) {
+    __shared__ uint32_t hc[kFoldHot];
+    const int tid = threadIdx.x, bd = blockDim.x;
+    for (uint32_t k = tid; k < kFoldHot; k += bd) hc[k] = 0;
+    __syncthreads();
+    for (int j = blockIdx.x; j < nslots; j += gridDim.x) {
+        const uint4 sl = slots[j];
+        for (uint32_t i = tid; i < sl.z; i += bd) {
+            const uint32_t w = wpool[sl.y + i];
+            if (w < kFoldHot) atomicAdd(&hc[w], 1u);
+            else tokw[atomicAdd(&wcursor[w], 1u)] = ((uint32_t)j << 16) | i;
+        }
+    }
+    __syncthreads();
+    for (uint32_t k = tid; k < kFoldHot; k += bd) {
+        const uint32_t c = hc[k];
+        if (c) hc[k] = atomicAdd(&wcursor[k], c);
+    }
+    __syncthreads();
+    for (int j = blockIdx.x; j < nslots; j += gridDim.x) {
+        const uint4 sl = slots[j];
+        for (uint32_t i = tid; i < sl.z; i += bd) {
+            const uint32_t w = wpool[sl.y + i];
+            if (w < kFoldHot) tokw[atomicAdd(&hc[w], 1u)] = ((uint32_t)j << 16) | i;
+        }
+    }
+}
+
+__device__ __forceinline__ float fi_sum8(const uint4 &a, const uint4 &b, const float *ph, float s) {
+    const uint32_t e[kFB] = {a.x, a.y, a.z, a.w, b.x, b.y, b.z, b.w};
+#pragma unroll
+    for (int u = 0; u < kFB; ++u) s = __builtin_fmaf((float)(e[u] >> 16), ph[e[u] & 0xFFFFu], s);
+    return s;
+}
+
+// The entry of a block (zero padded) at which the running sum from `run`
+// first exceeds the target; -1 if none does.  Zero entries add exactly
+// nothing, so they never cross the target first.
+__device__ __forceinline__ int fi_scan8(const uint4 &a, const uint4 &b, const float *ph, float run, float target) {
+    const uint32_t e[kFB] = {a.x, a.y, a.z, a.w, b.x, b.y, b.z, b.w};
+    int zz = -1;
+#pragma unroll
+    for (int q = 0; q < kFB; ++q) {
+        run = __builtin_fmaf((float)(e[q] >> 16), ph[e[q] & 0xFFFFu], run);
+        if (zz < 0 && run > target) zz = (int)(e[q] & 0xFFFFu);
+    }
+    return zz;
+}
+
+// First index in [k0, ke) (ke - k0 <= 64) whose running sum exceeds t, ke
+// if none: the running sums are nondecreasing, so this is the index a
+// bisection finds.  Two rounds of 8 independent loads: the last entry of
+// each group of 8, then the entries of the first group whose last one
+// exceeds t.
+__device__ __forceinline__ int q_first_above(const float *qw, int k0, int ke, float t) {
+    int grp = -1;
+#pragma unroll
+    for (int g = 7; g >= 0; --g) {
+        const int k = min(k0 + 8 * g + 7, ke - 1);
+        if (k0 + 8 * g < ke && qw[k] > t) grp = g;
+    }
+    if (grp < 0) return ke;
+    const int g0 = k0 + 8 * grp, g1 = min(g0 + 8, ke);
+    int lo = g1;
+#pragma unroll
+    for (int u = 7; u >= 0; --u)
+        if (g0 + u < g1 && qw[g0 + u] > t) lo = g0 + u;
+    return lo;
+}
+
+// One fold-in draw: token `ref` (slot << 16 | position) of a word whose phi
+// row (ph) and Q running sums (qa over the slab totals, qw within slabs) are
+// given.  S over the document's row in blocks of kFB entries, four blocks'
+// loads issued together (the first kFCP block sums kept), then the S entry
+// or the Q topic whose running sum first exceeds the target.
+__device__ __forceinline__ void fi_draw(uint32_t ref, const uint4 *__restrict__ slots,
+                                        const uint32_t *__restrict__ rows, const uint32_t *__restrict__ row_nnz,
+                                        const float *ph, const float *qa, const float *qw, int K, int NS, float alpha,
+                                        uint64_t seed, uint16_t *__restrict__ zpool) {
+    const uint32_t j = ref >> 16, i = ref & 0xFFFFu;
+    const uint4 sl = slots[j];
+    const uint32_t nz = row_nnz[j];
+    const uint32_t *row = rows + sl.y;
+    const float u = ez_u24(ez_philox_x0(seed, ((uint64_t)sl.x << 32) | i, sl.w, EZ_TAG_FOLDIN));
+    const uint32_t nb = (nz + kFB - 1) / kFB;
+    float cp[kFCP];
+    float S = 0.0f;
+#pragma unroll
+    for (int g = 0; g < kFCP; g += 4) {
+        uint4 e[8];
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            e[2 * q] = e[2 * q + 1] = make_uint4(0u, 0u, 0u, 0u);
+            if ((uint32_t)(g + q) < nb) {
+                e[2 * q] = *(const uint4 *)(row + kFB * (g + q));
+                e[2 * q + 1] = *(const uint4 *)(row + kFB * (g + q) + 4);
             }
         }
-        if (sweep + 1 == (uint32_t)sweeps_total) {  // final counts of the doc
-            __syncthreads();
-            for (uint32_t i = tid; i < n; i += bd) h16_add(h, z[i]);
-            __syncthreads();
-            int32_t *co = counts_out + (size_t)si * K;
-            for (int k = tid; k < K; k += bd) co[k] = (int32_t)h16_get(h, (uint32_t)k);
-            __syncthreads();
-            for (int k = tid; k < KH; k += bd) h[k] = 0;
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            if ((uint32_t)(g + q) < nb) S = fi_sum8(e[2 * q], e[2 * q + 1], ph, S);
+            cp[g + q] = S;
         }
+    }
+    for (uint32_t b = kFCP; b < nb; b += 4) {
+        uint4 e[8];
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            e[2 * q] = e[2 * q + 1] = make_uint4(0u, 0u, 0u, 0u);
+            if (b + q < nb) {
+                e[2 * q] = *(const uint4 *)(row + kFB * (b + q));
+                e[2 * q + 1] = *(const uint4 *)(row + kFB * (b + q) + 4);
+            }
+        }
+#pragma unroll
+        for (int q = 0; q < 4; ++q)
+            if (b + q < nb) S = fi_sum8(e[2 * q], e[2 * q + 1], ph, S);
+    }
+    const float target = u * (S + qa[NS - 1]);
+    int zz = -1;
+    if (target < S) {
+        // the first block whose running sum exceeds the target (one exists:
+        // the last running sum is S), then its entries in order
+        int bs = -1;
+#pragma unroll
+        for (int b = kFCP - 1; b >= 0; --b)
+            if (cp[b] > target) bs = b;
+        float run = 0.0f;
+        if (bs >= 0) {
+#pragma unroll
+            for (int b = 0; b < kFCP - 1; ++b)
+                if (b + 1 == bs) run = cp[b];
+            const uint32_t g0 = (uint32_t)bs * kFB;
+            zz = fi_scan8(*(const uint4 *)(row + g0), *(const uint4 *)(row + g0 + 4), ph, run, target);
+        } else {
+            // past the checkpoints: blocks four at a time until the block
+            // whose running sum exceeds the target, then its entries
+            run = cp[kFCP - 1];
+            for (uint32_t b = kFCP; b < nb && zz < 0; b += 4) {
+                uint4 e[8];
+#pragma unroll
+                for (int q = 0; q < 4; ++q) {
+                    e[2 * q] = e[2 * q + 1] = make_uint4(0u, 0u, 0u, 0u);
+                    if (b + q < nb) {
+                        e[2 * q] = *(const uint4 *)(row + kFB * (b + q));
+                        e[2 * q + 1] = *(const uint4 *)(row + kFB * (b + q) + 4);
+                    }
+                }
+#pragma unroll
+                for (int q = 0; q < 4; ++q) {
+                    if (zz < 0 && b + q < nb) {
+                        const float r2 = fi_sum8(e[2 * q], e[2 * q + 1], ph, run);
+                        if (r2 > target) zz = fi_scan8(e[2 * q], e[2 * q + 1], ph, run, target);
+                        else run = r2;
+                    }
+                }
+            }
+        }
+        if (zz < 0)  // not reached: kept as the canonical fallback
+            for (uint32_t g = 0; g < nz; ++g)
+                if ((row[g] >> 16) && ph[row[g] & 0xFFFFu] > 0.0f) zz = (int)(row[g] & 0xFFFFu);
+    } else {
+        const float t2 = target - S;
+        int lo = 0, hi = NS;
+        while (lo < hi) {
+            const int mid = (lo + hi) >> 1;
+            if (qa[mid] > t2) hi = mid; else lo = mid + 1;
+        }
+        int js = lo < NS ? lo : -1;
+        if (js < 0)
+            for (int jj = NS - 1; jj >= 0; --jj)
+                if (qa[jj] > (jj > 0 ? qa[jj - 1] : 0.0f)) {
+                    js = jj;
+                    break;
+                }
+        if (js >= 0) {
+            const float t3 = t2 - (js > 0 ? qa[js - 1] : 0.0f);
+            const int k0 = js * 64, ke = min(k0 + 64, K);
+            lo = q_first_above(qw, k0, ke, t3);
+            if (lo < ke) {
+                zz = lo;
+            } else {
+                for (int k = ke - 1; k >= k0; --k)
+                    if (alpha * ph[k] > 0.0f) {
+                        zz = k;
+                        break;
+                    }
+            }
+        }
+    }
+    zpool[sl.y + i] = (uint16_t)(zz < 0 ? 0 : zz);
+}
+
+// Workgroups of 4 wavefronts, items assigned by striding (every item is
+// one pass of similar cost, and a shared queue head would serialize on one
+// address).  First the big items, one per workgroup: phi_w, the slab
+// running sums and the running sums of the slab totals staged in LDS, one
+// lane per token.  Then the small items (fewer than 64 tokens), one per
+// wavefront for the first `ws` wavefronts: phi_w and the slab totals'
+// running sums in the wavefront's own LDS region, the slab running sums
+// read from the table.
+__global__ void __launch_bounds__(256) k_fi_sample(const uint4 *__restrict__ slots, int K, float alpha,
+                                                   const float *__restrict__ phi, const float *__restrict__ qs,
+                                                   const float *__restrict__ qacc, const uint32_t *__restrict__ rows,
+                                                   const uint32_t *__restrict__ row_nnz,
+                                                   const uint32_t *__restrict__ tokw, const uint4 *__restrict__ big,
+                                                   const uint4 *__restrict__ small, const uint32_t *__restrict__ ctr,
+                                                   uint16_t *__restrict__ zpool, uint64_t seed, int ws) {
+    extern __shared__ float shs[];
+    const int NS = (K + 63) >> 6, tid = threadIdx.x, bd = blockDim.x, lane = tid & 63, wv = tid >> 6;
+    const uint32_t n_big = ctr[0], n_small = ctr[2];
+    {
+        float *ph = shs;      // K: phi_w
+        float *qa = ph + K;   // NS: running sums of the slab totals
+        float *ql = qa + NS;  // K: slab running sums
+        for (uint32_t it = blockIdx.x; it < n_big; it += gridDim.x) {
+            __syncthreads();  // the previous item's reads are done
+            const uint4 item = big[it];
+            const size_t w = item.x;
+            for (int k = tid; k < K; k += bd) {
+                ph[k] = phi[w * K + k];
+                ql[k] = qs[w * K + k];
+            }
+            for (int j = tid; j < NS; j += bd) qa[j] = qacc[w * NS + j];
+            __syncthreads();
+            for (uint32_t t = item.y + tid; t < item.z; t += bd)
+                fi_draw(tokw[t], slots, rows, row_nnz, ph, qa, ql, K, NS, alpha, seed, zpool);
+        }
+    }
+    __syncthreads();
+    if (wv >= ws) return;
+    float *ph = shs + (size_t)wv * (K + NS);
+    float *qa = ph + K;
+    for (uint32_t it = blockIdx.x * (uint32_t)ws + wv; it < n_small; it += gridDim.x * (uint32_t)ws) {
+        const uint4 item = small[it];
+        const size_t w = item.x;
+        wave_lds_sync();  // the previous item's reads are done
+        for (int k = lane; k < K; k += 64) ph[k] = phi[w * K + k];
+        for (int j = lane; j < NS; j += 64) qa[j] = qacc[w * NS + j];
+        wave_lds_sync();
+        const uint32_t t = item.y + (uint32_t)lane;
+        if (t < item.z) fi_draw(tokw[t], slots, rows, row_nnz, ph, qa, qs + w * K, K, NS, alpha, seed, zpool);
+    }
+}
+
+// Documents that just drew their last sweep (fin[j] = (output index, topic
+// offset), output index ~0 for the others): topic counts into row `index`
+// of counts_out, topics into zout at the offset.
+__global__ void __launch_bounds__(256) k_fi_final(const uint4 *__restrict__ slots, const uint2 *__restrict__ fin,
+                                                  int nslots, int K, const uint16_t *__restrict__ zpool,
+                                                  int32_t *__restrict__ counts_out, uint16_t *__restrict__ zout) {
+    extern __shared__ uint32_t shf2[];
+    const int tid = threadIdx.x, bd = blockDim.x, KH = (K + 1) >> 1;
+    for (int j = blockIdx.x; j < nslots; j += gridDim.x) {
+        const uint2 f = fin[j];
+        if (f.x == 0xFFFFFFFFu) continue;  // uniform over the workgroup
+        const uint4 sl = slots[j];
+        for (int k = tid; k < KH; k += bd) shf2[k] = 0;
+        __syncthreads();
+        for (uint32_t i = tid; i < sl.z; i += bd) {
+            const uint16_t t = zpool[sl.y + i];
+            h16_add(shf2, t);
+            zout[f.y + i] = t;
+        }
+        __syncthreads();
+        int32_t *co = counts_out + (size_t)f.x * K;
+        for (int k = tid; k < K; k += bd) co[k] = (int32_t)h16_get(shf2, (uint32_t)k);
         __syncthreads();
     }
 }
@@ -1044,21 +1337,52 @@ hipError_t launch_foldin_tables(const DevState &s, float *phi, float *qs, float 
     return hipGetLastError();
 }
 
-static size_t foldin_wave_lds(int K) { return (size_t)(((K + 1) >> 1) + (K + kFB - 1) / kFB * kFB) * sizeof(uint32_t); }
-
-int foldin_waves_per_block(int K) {
-    (void)K;
-    return 4;  // one workgroup of 4 wavefronts per document
+// Small-item wavefronts per sampling workgroup: up to 4, within 64 KiB of LDS.
+static int foldin_small_waves(int K) {
+    const size_t per = ((size_t)K + (size_t)((K + 63) >> 6)) * sizeof(float);
+    return (int)std::max<size_t>(1, std::min<size_t>(4, 65536 / per));
 }
 
-hipError_t launch_foldin_step(const uint4 *slots, int nslots, int K, int sweeps_total, float alpha, const float *phi,
-                              const float *qs, const float *qacc, const uint32_t *wpool, uint16_t *zpool,
-                              int32_t *counts_out, uint64_t seed, hipStream_t st) {
+static size_t foldin_sample_lds(int K) {
+    const size_t NS = (size_t)((K + 63) >> 6);
+    return std::max((2 * (size_t)K + NS), (size_t)foldin_small_waves(K) * ((size_t)K + NS)) * sizeof(float);
+}
+
+size_t foldin_lds_bytes(int K) {
+    return std::max(foldin_sample_lds(K) + 16, (size_t)((K + 1) >> 1) * sizeof(uint32_t));
+}
+
+uint32_t foldin_max_items(int V, uint32_t max_tokens) { return (uint32_t)V + max_tokens / kFoldSeg + 1u; }
+
+hipError_t launch_foldin_admit(const uint4 *adm, int nadm, const uint32_t *src, uint32_t *wpool, hipStream_t st) {
+    if (nadm <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_fi_admit, dim3(std::min(nadm, 1024)), dim3(256), 0, st, adm, nadm, src, wpool);
+    return hipGetLastError();
+}
+
+hipError_t launch_foldin_step(const FoldinBatch &fb, const uint4 *slots, const uint2 *fin, int nslots, uint32_t ntok,
+                              int V, int K, float alpha, const float *phi, const float *qs, const float *qacc,
+                              const uint32_t *wpool, uint16_t *zpool, uint64_t seed, hipStream_t st) {
     if (nslots <= 0) return hipSuccess;
-    const int nwv = foldin_waves_per_block(K);
-    const size_t lds = foldin_wave_lds(K);  // one document per workgroup
-    hipLaunchKernelGGL(k_foldin_step, dim3(grid_for(nslots, 1, 1 << 16)), dim3(64 * nwv), lds, st, slots, nslots, K,
-                       sweeps_total, alpha, phi, qs, qacc, wpool, zpool, counts_out, seed);
+    const unsigned gs = (unsigned)std::min(nslots, 1 << 16), gc = (unsigned)std::min(nslots, 1024);
+    const unsigned nb = (unsigned)((V + 1023) / 1024);
+    const size_t lds_rows = (size_t)((K + 1) >> 1) * sizeof(uint32_t);
+    hipLaunchKernelGGL(k_fill_u32, dim3(grid_for((size_t)V, 256, 4096)), dim3(256), 0, st, fb.wcount, (size_t)V, 0u);
+    hipLaunchKernelGGL(k_fi_rows, dim3(gs), dim3(256), lds_rows, st, slots, nslots, K, seed, zpool, fb.rows,
+                       fb.row_nnz);
+    hipLaunchKernelGGL(k_fi_count, dim3(gc), dim3(256), 0, st, slots, nslots, wpool, fb.wcount);
+    hipLaunchKernelGGL(k_fi_bsum, dim3(nb), dim3(1024), 0, st, V, fb.wcount, fb.bsum);
+    hipLaunchKernelGGL(k_fi_btop, dim3(1), dim3(1024), 0, st, (int)nb, fb.bsum, fb.ctr);
+    hipLaunchKernelGGL(k_fi_bfin, dim3(nb), dim3(1024), 0, st, V, fb.wcount, fb.bsum, fb.wcursor, fb.items,
+                       fb.small);
+    hipLaunchKernelGGL(k_fi_scatter, dim3(gc), dim3(256), 0, st, slots, nslots, wpool, fb.wcursor, fb.tokw);
+    // no more workgroups than a wavefront per token could use
+    const unsigned gsmp = (unsigned)std::max<int64_t>(1, std::min<int64_t>(fb.grid, ((int64_t)ntok + 3) / 4));
+    hipLaunchKernelGGL(k_fi_sample, dim3(gsmp), dim3(256), foldin_sample_lds(K), st, slots, K, alpha,
+                       phi, qs, qacc, fb.rows, fb.row_nnz, fb.tokw, fb.items, fb.small, fb.ctr, zpool, seed,
+                       foldin_small_waves(K));
+    hipLaunchKernelGGL(k_fi_final, dim3(gs), dim3(256), lds_rows, st, slots, fin, nslots, K, zpool, fb.counts_out,
+                       fb.zout);
     return hipGetLastError();
 }
 
@@ -1172,7 +1496,7 @@ size_t max_lds_bytes(int K, int bd) {
     m = std::max(m, 4 * dt_wave_bytes(K));                           // WT tail, top-3, DT build
     m = std::max(m, (size_t)K * sizeof(double));                     // log-likelihood
     m = std::max(m, (size_t)K * sizeof(float) + 1024 * sizeof(float));  // fold-in tables
-    m = std::max(m, foldin_wave_lds(K) + 16);
+    m = std::max(m, foldin_lds_bytes(K));
     return m;
 }
 

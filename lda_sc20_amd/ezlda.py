@@ -16,6 +16,7 @@ _PKG = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_PKG, "libezlda.so")
 
 OK, EINVAL, EHIP, EIO, ESTATE, ENOMEM = 0, 1, 2, 3, 4, 5
+FOLDIN_TOKEN_BYTES = 16  # EZLDA_FOLDIN_TOKEN_BYTES: server device bytes per resident token
 STAGES = ("wt", "topk", "dt", "sample")
 
 # Every symbol include/ezlda.h declares (checked by the CPU test suite).

@@ -74,7 +74,7 @@ def test_alone_equals_in_mix_and_any_arrival_order():
     off, words = _docs(300, lens, seed=9)
     rids = rng.permutation(1 << 20)[:60].astype(np.uint32)
     zo, co = orc.foldin(off, words, rids, 20, 5)
-    srv = ezlda.Server(ctx, max_slots=16, sweeps=20, hbm_budget_bytes=6 * 3000, seed=5)
+    srv = ezlda.Server(ctx, max_slots=16, sweeps=20, hbm_budget_bytes=ezlda.FOLDIN_TOKEN_BYTES * 3000, seed=5)
     # alone
     srv.submit(int(rids[7]), words[off[7]: off[8]])
     z, c, _ = srv.wait(int(rids[7]), int(lens[7]), timeout_ms=30000)
@@ -104,7 +104,7 @@ def test_alone_equals_in_mix_and_any_arrival_order():
 
 def test_budget_rejects_oversized_and_bad_requests():
     ctx, _ = _model(32)
-    srv = ezlda.Server(ctx, max_slots=2, sweeps=3, hbm_budget_bytes=6 * 100, seed=1)
+    srv = ezlda.Server(ctx, max_slots=2, sweeps=3, hbm_budget_bytes=ezlda.FOLDIN_TOKEN_BYTES * 100, seed=1)
     with pytest.raises(ezlda.EzldaError) as e:
         srv.submit(1, np.zeros(101, np.int32))
     assert e.value.code == ezlda.ENOMEM
