@@ -76,11 +76,10 @@ struct FoldinBatch {
     uint32_t *row_nnz = nullptr;    // per slot
     uint32_t *wcount = nullptr;     // V: tokens per word in the step
     uint32_t *wcursor = nullptr;    // V: bucket cursors
-    uint32_t *bsum = nullptr;       // 3 x ceil(V / 1024): per-block tokens, big and small items
-    uint32_t *tokw = nullptr;       // resident tokens: slot << 16 | position, by word
+    uint32_t *bsum = nullptr;       // 3 x ceil(V / 1024): per-block big-word tokens, big items, other tokens
+    uint32_t *tokw = nullptr;       // resident tokens: slot << 16 | position, by word (big words first)
     uint4 *items = nullptr;         // foldin_max_items(V, tokens) big items: (word, first, end, 0)
-    uint4 *small = nullptr;         // V small items
-    uint32_t *ctr = nullptr;        // [0] big items, [2] small items
+    uint32_t *ctr = nullptr;        // big items, big-word tokens, other tokens, 0
     int32_t *counts_out = nullptr;  // per finishing document: K topic counts
     uint16_t *zout = nullptr;       // finishing documents' topics, packed
     int grid = 0;                   // persistent sampling workgroups

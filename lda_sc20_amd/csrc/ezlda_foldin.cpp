@@ -387,7 +387,7 @@ bool alloc_batch(ezlda_server *s, ez::FoldinBatch *b, uint32_t tokens, int nslot
            salloc(s, &b->wcount, s->V) == EZLDA_OK && salloc(s, &b->wcursor, s->V) == EZLDA_OK &&
            salloc(s, &b->bsum, 3 * nb) == EZLDA_OK && salloc(s, &b->tokw, tokens) == EZLDA_OK &&
            salloc(s, &b->items, ez::foldin_max_items(s->V, tokens)) == EZLDA_OK &&
-           salloc(s, &b->small, s->V) == EZLDA_OK && salloc(s, &b->ctr, 4) == EZLDA_OK && salloc(s, &b->counts_out, (size_t)nslots * s->K) == EZLDA_OK &&
+           salloc(s, &b->ctr, 4) == EZLDA_OK && salloc(s, &b->counts_out, (size_t)nslots * s->K) == EZLDA_OK &&
            salloc(s, &b->zout, tokens) == EZLDA_OK;
 }
 

@@ -992,26 +992,29 @@ __device__ __forceinline__ void block_scan(uint32_t (&v)[M], uint32_t *sh, uint3
     __syncthreads();
 }
 
-// A word's tokens in the step form one small item (fewer than kFoldStageQ
-// tokens: a wavefront draws them) or big items of at most kFoldSeg tokens
-// (a workgroup draws each).
+// A word with at least kFoldStageQ tokens in the step is big: its tokens
+// form items of at most kFoldSeg tokens, a workgroup each, with its phi
+// row staged in LDS.  The other words' tokens follow all big words' tokens
+// in the bucketed list and are drawn one lane per token straight from the
+// tables.
 __device__ __forceinline__ uint32_t fold_big(uint32_t c) { return c >= kFoldStageQ ? (c + kFoldSeg - 1u) / kFoldSeg : 0u; }
-__device__ __forceinline__ uint32_t fold_small(uint32_t c) { return c && c < kFoldStageQ ? 1u : 0u; }
 
-// Scan pass 1: per 1024 words, their tokens, big items and small items.
+// Scan pass 1: per 1024 words, their big-word tokens, big items and
+// small-word tokens.
 __global__ void __launch_bounds__(1024) k_fi_bsum(int V, const uint32_t *__restrict__ wcount,
                                                   uint32_t *__restrict__ bsum) {
     __shared__ uint32_t sh[48];
     const uint32_t w = blockIdx.x * 1024u + threadIdx.x;
     const uint32_t c = w < (uint32_t)V ? wcount[w] : 0u;
-    uint32_t v[3] = {c, fold_big(c), fold_small(c)}, t[3];
+    const bool bg = c >= kFoldStageQ;
+    uint32_t v[3] = {bg ? c : 0u, fold_big(c), bg ? 0u : c}, t[3];
     block_scan<3>(v, sh, t);
     if (threadIdx.x == 0)
         for (int m = 0; m < 3; ++m) bsum[3 * blockIdx.x + m] = t[m];
 }
 
-// Scan pass 2 (one workgroup): exclusive offsets of the per-block sums; the
-// item counts in ctr: [0] big items, [2] small items ([1] and [3] zero).
+// Scan pass 2 (one workgroup): exclusive offsets of the per-block sums; ctr
+// = [big items, big-word tokens, small-word tokens, 0].
 __global__ void __launch_bounds__(1024) k_fi_btop(int nb, uint32_t *__restrict__ bsum, uint32_t *__restrict__ ctr) {
     __shared__ uint32_t sh[48];
     uint32_t carry[3] = {0u, 0u, 0u};
@@ -1027,30 +1030,31 @@ __global__ void __launch_bounds__(1024) k_fi_btop(int nb, uint32_t *__restrict__
     }
     if (threadIdx.x == 0) {
         ctr[0] = carry[1];
-        ctr[1] = 0u;
+        ctr[1] = carry[0];
         ctr[2] = carry[2];
         ctr[3] = 0u;
     }
 }
 
-// Scan pass 3: bucket starts (the scatter's cursors) and the items (word,
-// first, end, 0), big and small, in word order.
+// Scan pass 3: bucket starts (the scatter's cursors: big words first, then
+// small words, each in word order) and the big items (word, first, end, 0).
 __global__ void __launch_bounds__(1024) k_fi_bfin(int V, const uint32_t *__restrict__ wcount,
-                                                  const uint32_t *__restrict__ bsum, uint32_t *__restrict__ wcursor,
-                                                  uint4 *__restrict__ big, uint4 *__restrict__ small) {
+                                                  const uint32_t *__restrict__ bsum, const uint32_t *__restrict__ ctr,
+                                                  uint32_t *__restrict__ wcursor, uint4 *__restrict__ big) {
     __shared__ uint32_t sh[48];
     const uint32_t w = blockIdx.x * 1024u + threadIdx.x;
     const uint32_t c = w < (uint32_t)V ? wcount[w] : 0u;
-    uint32_t v[3] = {c, fold_big(c), fold_small(c)}, t[3];
+    const bool bg = c >= kFoldStageQ;
+    uint32_t v[3] = {bg ? c : 0u, fold_big(c), bg ? 0u : c}, t[3];
     block_scan<3>(v, sh, t);
     if (w >= (uint32_t)V) return;
-    const uint32_t base = bsum[3 * blockIdx.x] + v[0];
-    wcursor[w] = base;
-    if (c >= kFoldStageQ) {
+    if (bg) {
+        const uint32_t base = bsum[3 * blockIdx.x] + v[0];
+        wcursor[w] = base;
         uint32_t ib = bsum[3 * blockIdx.x + 1] + v[1];
         for (uint32_t x = 0; x < c; x += kFoldSeg) big[ib++] = make_uint4(w, base + x, base + min(c, x + kFoldSeg), 0u);
-    } else if (c) {
-        small[bsum[3 * blockIdx.x + 2] + v[2]] = make_uint4(w, base, base + c, 0u);
+    } else {
+        wcursor[w] = ctr[1] + bsum[3 * blockIdx.x + 2] + v[2];
     }
 }
 
@@ -1108,39 +1112,40 @@ __device__ __forceinline__ int fi_scan8(const uint4 &a, const uint4 &b, const fl
     return zz;
 }
 
-// First index in [k0, ke) (ke - k0 <= 64) whose running sum exceeds t, ke
-// if none: the running sums are nondecreasing, so this is the index a
-// bisection finds.  Two rounds of 8 independent loads: the last entry of
-// each group of 8, then the entries of the first group whose last one
-// exceeds t.
-__device__ __forceinline__ int q_first_above(const float *qw, int k0, int ke, float t) {
-    int grp = -1;
+// First index in [k0, ke) whose running sum exceeds t, ke if none: the
+// running sums are nondecreasing, so this is the index a bisection finds.
+// Rounds of 8 independent loads: the last entries of up to 8 groups of 8,
+// then the entries of the first group whose last one exceeds t.
+__device__ __forceinline__ int first_above(const float *q, int k0, int ke, float t) {
+    for (int c0 = k0; c0 < ke; c0 += 64) {
+        const int c1 = min(c0 + 64, ke);
+        int grp = -1;
 #pragma unroll
-    for (int g = 7; g >= 0; --g) {
-        const int k = min(k0 + 8 * g + 7, ke - 1);
-        if (k0 + 8 * g < ke && qw[k] > t) grp = g;
+        for (int g = 7; g >= 0; --g) {
+            const int s0 = c0 + 8 * g;
+            if (s0 < c1 && q[min(s0 + 7, c1 - 1)] > t) grp = g;
+        }
+        if (grp >= 0) {
+            const int g0 = c0 + 8 * grp, g1 = min(g0 + 8, c1);
+            int lo = g1;
+#pragma unroll
+            for (int u = 7; u >= 0; --u)
+                if (g0 + u < g1 && q[g0 + u] > t) lo = g0 + u;
+            return lo;
+        }
     }
-    if (grp < 0) return ke;
-    const int g0 = k0 + 8 * grp, g1 = min(g0 + 8, ke);
-    int lo = g1;
-#pragma unroll
-    for (int u = 7; u >= 0; --u)
-        if (g0 + u < g1 && qw[g0 + u] > t) lo = g0 + u;
-    return lo;
+    return ke;
 }
 
-// One fold-in draw: token `ref` (slot << 16 | position) of a word whose phi
-// row (ph) and Q running sums (qa over the slab totals, qw within slabs) are
-// given.  S over the document's row in blocks of kFB entries, four blocks'
-// loads issued together (the first kFCP block sums kept), then the S entry
-// or the Q topic whose running sum first exceeds the target.
-__device__ __forceinline__ void fi_draw(uint32_t ref, const uint4 *__restrict__ slots,
-                                        const uint32_t *__restrict__ rows, const uint32_t *__restrict__ row_nnz,
+// One fold-in draw: token i of slot sl (its row of nz entries at pool
+// offset sl.y) for a word whose phi row (ph) and Q running sums (qa over
+// the slab totals, qw within slabs) are given, in LDS or in the tables.  S
+// over the row in blocks of kFB entries, four blocks' loads issued together
+// (the first kFCP block sums kept), then the S entry or the Q topic whose
+// running sum first exceeds the target.
+__device__ __forceinline__ void fi_draw(const uint4 sl, uint32_t i, uint32_t nz, const uint32_t *__restrict__ rows,
                                         const float *ph, const float *qa, const float *qw, int K, int NS, float alpha,
                                         uint64_t seed, uint16_t *__restrict__ zpool) {
-    const uint32_t j = ref >> 16, i = ref & 0xFFFFu;
-    const uint4 sl = slots[j];
-    const uint32_t nz = row_nnz[j];
     const uint32_t *row = rows + sl.y;
     const float u = ez_u24(ez_philox_x0(seed, ((uint64_t)sl.x << 32) | i, sl.w, EZ_TAG_FOLDIN));
     const uint32_t nb = (nz + kFB - 1) / kFB;
@@ -1222,11 +1227,7 @@ __device__ __forceinline__ void fi_draw(uint32_t ref, const uint4 *__restrict__ 
                 if ((row[g] >> 16) && ph[row[g] & 0xFFFFu] > 0.0f) zz = (int)(row[g] & 0xFFFFu);
     } else {
         const float t2 = target - S;
-        int lo = 0, hi = NS;
-        while (lo < hi) {
-            const int mid = (lo + hi) >> 1;
-            if (qa[mid] > t2) hi = mid; else lo = mid + 1;
-        }
+        const int lo = first_above(qa, 0, NS, t2);
         int js = lo < NS ? lo : -1;
         if (js < 0)
             for (int jj = NS - 1; jj >= 0; --jj)
@@ -1237,9 +1238,9 @@ __device__ __forceinline__ void fi_draw(uint32_t ref, const uint4 *__restrict__ 
         if (js >= 0) {
             const float t3 = t2 - (js > 0 ? qa[js - 1] : 0.0f);
             const int k0 = js * 64, ke = min(k0 + 64, K);
-            lo = q_first_above(qw, k0, ke, t3);
-            if (lo < ke) {
-                zz = lo;
+            const int kq = first_above(qw, k0, ke, t3);
+            if (kq < ke) {
+                zz = kq;
             } else {
                 for (int k = ke - 1; k >= k0; --k)
                     if (alpha * ph[k] > 0.0f) {
@@ -1252,55 +1253,48 @@ __device__ __forceinline__ void fi_draw(uint32_t ref, const uint4 *__restrict__ 
     zpool[sl.y + i] = (uint16_t)(zz < 0 ? 0 : zz);
 }
 
-// Workgroups of 4 wavefronts, items assigned by striding (every item is
-// one pass of similar cost, and a shared queue head would serialize on one
-// address).  First the big items, one per workgroup: phi_w, the slab
-// running sums and the running sums of the slab totals staged in LDS, one
-// lane per token.  Then the small items (fewer than 64 tokens), one per
-// wavefront for the first `ws` wavefronts: phi_w and the slab totals'
-// running sums in the wavefront's own LDS region, the slab running sums
-// read from the table.
+// Workgroups of 4 wavefronts.  First the big items, assigned by striding
+// (each is one pass of similar cost; a shared queue head would serialize on
+// one address): phi_w, the slab running sums and the running sums of the
+// slab totals staged in LDS, one lane per token.  Then the small words'
+// tokens, one lane each, the word read from the pool and its tables from
+// HBM (their phi rows are touched by few tokens each, and staging them
+// would leave most lanes idle).
 __global__ void __launch_bounds__(256) k_fi_sample(const uint4 *__restrict__ slots, int K, float alpha,
                                                    const float *__restrict__ phi, const float *__restrict__ qs,
-                                                   const float *__restrict__ qacc, const uint32_t *__restrict__ rows,
+                                                   const float *__restrict__ qacc, const uint32_t *__restrict__ wpool,
+                                                   const uint32_t *__restrict__ rows,
                                                    const uint32_t *__restrict__ row_nnz,
                                                    const uint32_t *__restrict__ tokw, const uint4 *__restrict__ big,
-                                                   const uint4 *__restrict__ small, const uint32_t *__restrict__ ctr,
-                                                   uint16_t *__restrict__ zpool, uint64_t seed, int ws) {
+                                                   const uint32_t *__restrict__ ctr, uint16_t *__restrict__ zpool,
+                                                   uint64_t seed) {
     extern __shared__ float shs[];
-    const int NS = (K + 63) >> 6, tid = threadIdx.x, bd = blockDim.x, lane = tid & 63, wv = tid >> 6;
-    const uint32_t n_big = ctr[0], n_small = ctr[2];
-    {
-        float *ph = shs;      // K: phi_w
-        float *qa = ph + K;   // NS: running sums of the slab totals
-        float *ql = qa + NS;  // K: slab running sums
-        for (uint32_t it = blockIdx.x; it < n_big; it += gridDim.x) {
-            __syncthreads();  // the previous item's reads are done
-            const uint4 item = big[it];
-            const size_t w = item.x;
-            for (int k = tid; k < K; k += bd) {
-                ph[k] = phi[w * K + k];
-                ql[k] = qs[w * K + k];
-            }
-            for (int j = tid; j < NS; j += bd) qa[j] = qacc[w * NS + j];
-            __syncthreads();
-            for (uint32_t t = item.y + tid; t < item.z; t += bd)
-                fi_draw(tokw[t], slots, rows, row_nnz, ph, qa, ql, K, NS, alpha, seed, zpool);
+    const int NS = (K + 63) >> 6, tid = threadIdx.x, bd = blockDim.x;
+    const uint32_t n_big = ctr[0], t_big = ctr[1], t_all = ctr[1] + ctr[2];
+    float *ph = shs;      // K: phi_w
+    float *qa = ph + K;   // NS: running sums of the slab totals
+    float *ql = qa + NS;  // K: slab running sums
+    for (uint32_t it = blockIdx.x; it < n_big; it += gridDim.x) {
+        __syncthreads();  // the previous item's reads are done
+        const uint4 item = big[it];
+        const size_t w = item.x;
+        for (int k = tid; k < K; k += bd) {
+            ph[k] = phi[w * K + k];
+            ql[k] = qs[w * K + k];
+        }
+        for (int j = tid; j < NS; j += bd) qa[j] = qacc[w * NS + j];
+        __syncthreads();
+        for (uint32_t t = item.y + tid; t < item.z; t += bd) {
+            const uint32_t ref = tokw[t], j = ref >> 16;
+            fi_draw(slots[j], ref & 0xFFFFu, row_nnz[j], rows, ph, qa, ql, K, NS, alpha, seed, zpool);
         }
     }
-    __syncthreads();
-    if (wv >= ws) return;
-    float *ph = shs + (size_t)wv * (K + NS);
-    float *qa = ph + K;
-    for (uint32_t it = blockIdx.x * (uint32_t)ws + wv; it < n_small; it += gridDim.x * (uint32_t)ws) {
-        const uint4 item = small[it];
-        const size_t w = item.x;
-        wave_lds_sync();  // the previous item's reads are done
-        for (int k = lane; k < K; k += 64) ph[k] = phi[w * K + k];
-        for (int j = lane; j < NS; j += 64) qa[j] = qacc[w * NS + j];
-        wave_lds_sync();
-        const uint32_t t = item.y + (uint32_t)lane;
-        if (t < item.z) fi_draw(tokw[t], slots, rows, row_nnz, ph, qa, qs + w * K, K, NS, alpha, seed, zpool);
+    for (uint32_t t = t_big + blockIdx.x * (uint32_t)bd + tid; t < t_all; t += gridDim.x * (uint32_t)bd) {
+        const uint32_t ref = tokw[t], j = ref >> 16, i = ref & 0xFFFFu;
+        const uint4 sl = slots[j];
+        const uint32_t nz = row_nnz[j];
+        const size_t w = wpool[sl.y + i];
+        fi_draw(sl, i, nz, rows, phi + w * K, qacc + w * NS, qs + w * K, K, NS, alpha, seed, zpool);
     }
 }
 
@@ -1337,16 +1331,7 @@ hipError_t launch_foldin_tables(const DevState &s, float *phi, float *qs, float 
     return hipGetLastError();
 }
 
-// Small-item wavefronts per sampling workgroup: up to 4, within 64 KiB of LDS.
-static int foldin_small_waves(int K) {
-    const size_t per = ((size_t)K + (size_t)((K + 63) >> 6)) * sizeof(float);
-    return (int)std::max<size_t>(1, std::min<size_t>(4, 65536 / per));
-}
-
-static size_t foldin_sample_lds(int K) {
-    const size_t NS = (size_t)((K + 63) >> 6);
-    return std::max((2 * (size_t)K + NS), (size_t)foldin_small_waves(K) * ((size_t)K + NS)) * sizeof(float);
-}
+static size_t foldin_sample_lds(int K) { return (2 * (size_t)K + (size_t)((K + 63) >> 6)) * sizeof(float); }
 
 size_t foldin_lds_bytes(int K) {
     return std::max(foldin_sample_lds(K) + 16, (size_t)((K + 1) >> 1) * sizeof(uint32_t));
@@ -1373,14 +1358,13 @@ hipError_t launch_foldin_step(const FoldinBatch &fb, const uint4 *slots, const u
     hipLaunchKernelGGL(k_fi_count, dim3(gc), dim3(256), 0, st, slots, nslots, wpool, fb.wcount);
     hipLaunchKernelGGL(k_fi_bsum, dim3(nb), dim3(1024), 0, st, V, fb.wcount, fb.bsum);
     hipLaunchKernelGGL(k_fi_btop, dim3(1), dim3(1024), 0, st, (int)nb, fb.bsum, fb.ctr);
-    hipLaunchKernelGGL(k_fi_bfin, dim3(nb), dim3(1024), 0, st, V, fb.wcount, fb.bsum, fb.wcursor, fb.items,
-                       fb.small);
+    hipLaunchKernelGGL(k_fi_bfin, dim3(nb), dim3(1024), 0, st, V, fb.wcount, fb.bsum, fb.ctr, fb.wcursor,
+                       fb.items);
     hipLaunchKernelGGL(k_fi_scatter, dim3(gc), dim3(256), 0, st, slots, nslots, wpool, fb.wcursor, fb.tokw);
-    // no more workgroups than a wavefront per token could use
-    const unsigned gsmp = (unsigned)std::max<int64_t>(1, std::min<int64_t>(fb.grid, ((int64_t)ntok + 3) / 4));
-    hipLaunchKernelGGL(k_fi_sample, dim3(gsmp), dim3(256), foldin_sample_lds(K), st, slots, K, alpha,
-                       phi, qs, qacc, fb.rows, fb.row_nnz, fb.tokw, fb.items, fb.small, fb.ctr, zpool, seed,
-                       foldin_small_waves(K));
+    // no more workgroups than a lane per token could use
+    const unsigned gsmp = (unsigned)std::max<int64_t>(1, std::min<int64_t>(fb.grid, ((int64_t)ntok + 255) / 256));
+    hipLaunchKernelGGL(k_fi_sample, dim3(gsmp), dim3(256), foldin_sample_lds(K), st, slots, K, alpha, phi, qs, qacc,
+                       wpool, fb.rows, fb.row_nnz, fb.tokw, fb.items, fb.ctr, zpool, seed);
     hipLaunchKernelGGL(k_fi_final, dim3(gs), dim3(256), lds_rows, st, slots, fin, nslots, K, zpool, fb.counts_out,
                        fb.zout);
     return hipGetLastError();
