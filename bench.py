@@ -325,10 +325,10 @@ def main() -> int:
     # token and by doc slot); per run (a word's tokens in one doc) its entry,
     # max-topic counts and doc info (16 + 4 + 8); phi staging once per word;
     # one read of every scanned DT entry (one scan per run with a full draw;
-    # 2-byte entries when K <= 1024, else 4)
+    # 2-byte entries when K <= 8192, else 4)
     VD = int(info.num_head_words)
     R = int(info.num_runs)
-    eb = 2 if K <= 1024 else 4
+    eb = 2 if K <= 8192 else 4
     s_scanned = n_scanned / K_steps
     b_sample = 4 * N + 28 * R + 4 * K * VD + 4 * tail_nnz + 4 * K + eb * s_scanned
     t_sample = stage_ms["sample"] / K_steps / 1e3
