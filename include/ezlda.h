@@ -49,8 +49,18 @@ typedef struct ezlda_args {
                                 from per-XCD queues (then from the others');
                                 0 = one workgroup per item; < 0 = 8 per CU.
                                 Topics do not depend on it */
-    int32_t num_chunks;      /* mainKernel.cu:50; recorded, corpus is resident */
-    int32_t num_streams;     /* Argument.cuh:7; recorded */
+    int32_t num_chunks;      /* mainKernel.cu:50: > 1 = out-of-core sweeps:
+                                the documents are cut into that many chunks
+                                (about equal token counts) whose token state
+                                and layout stay in pinned host memory and pass
+                                through two device buffers on two streams per
+                                sweep; topics equal the resident ones, and
+                                the counts (ezlda_get_nk, _get_wt_dense) are
+                                those of the current topics.  The per-stage
+                                calls, graphs, layout tuning and DT/aux
+                                getters then return EZLDA_ESTATE.
+                                <= 1 = the corpus is resident */
+    int32_t num_streams;     /* Argument.cuh:7; recorded (chunked sweeps use 2) */
     int32_t token_segment;   /* Doc.cuh:30: max tokens of one word per sampler
                                 work item (0 = 4096) */
     uint64_t seed;           /* Philox key */
@@ -94,6 +104,7 @@ typedef struct ezlda_info {
     int32_t block_dim;       /* sampler workgroup size in effect */
     int32_t token_segment;   /* in effect */
     int64_t num_runs;        /* sampler runs (a word's tokens in one doc) */
+    int32_t num_chunks;      /* chunks of the out-of-core path (1 = resident) */
 } ezlda_info;
 
 void ezlda_default_args(ezlda_args *args);

@@ -40,7 +40,7 @@ struct DevState {
     // work items
     uint4 *wt_segs = nullptr;       // head-word histogram segments (w, beg, end)
     int32_t n_wt_segs = 0;
-    uint4 *smp_segs = nullptr;      // sampler items (w, first run, end run, 0)
+    uint4 *smp_segs = nullptr;      // sampler items (w, first run, end run, draw-key offset)
     int32_t n_smp_segs = 0;
     uint4 *runs = nullptr;          // (first token, first slot, doc q, tokens): a word's tokens in one doc
     int64_t n_runs = 0;
@@ -55,6 +55,8 @@ struct DevState {
 
 hipError_t launch_zero_counts(const DevState &s, hipStream_t st);
 hipError_t launch_build_wt(const DevState &s, hipStream_t st);
+// Adds the counts of s.z (n_k, dense head rows, tail rows) without zeroing.
+hipError_t launch_accumulate_wt(const DevState &s, hipStream_t st);
 hipError_t launch_topk(const DevState &s, hipStream_t st);
 hipError_t launch_build_dt(const DevState &s, hipStream_t st);
 hipError_t launch_z_to_doc(const DevState &s, hipStream_t st);
@@ -69,7 +71,6 @@ hipError_t launch_copy(const void *src, void *dst, size_t bytes, hipStream_t st)
 // Fold-in against a frozen model: per-word phi rows, Q running sums inside
 // 64-topic slabs (qs, V x K) and over slab totals (qacc, V x ceil(K/64)).
 hipError_t launch_foldin_tables(const DevState &s, float *phi, float *qs, float *qacc, hipStream_t st);
-// One sweep of every slot (rid, first token, n, sweep) in the pools.
 // Buffers of the word-major fold-in step.
 struct FoldinBatch {
     uint32_t *rows = nullptr;       // row pool, parallel to the token pool: topic | count << 16

@@ -8,6 +8,22 @@
 #include "../../include/ezlda.h"
 #include "ezlda_device.h"
 
+// Out-of-core execution (num_chunks > 1): the documents, in the layout's doc
+// order, are cut into chunks of about equal token counts.  Each chunk keeps
+// its token state and word-major layout in one pinned host image; a sweep
+// passes every chunk through one of two device image buffers.
+struct EzChunk {
+    uint32_t q0 = 0, D = 0, N = 0;  // first doc (doc order), docs, tokens
+    uint32_t slot0 = 0;             // first doc slot of the corpus layout
+    uint32_t n_items = 0, n_wt_segs = 0, n_queues = 1;
+    uint64_t n_runs = 0, dt_entries = 0;
+    // image: z | z_doc | doc_info (copied back after each sweep), then
+    // soff, slot_wrd, slot_tok, items, runs, WT segments
+    size_t off_doc_info = 0, back_bytes = 0, off_soff = 0, off_slot_wrd = 0, off_slot_tok = 0, off_items = 0,
+           off_runs = 0, off_wt_segs = 0, bytes = 0;
+    uint8_t *img = nullptr;  // pinned
+};
+
 struct ezlda_ctx {
     ezlda_args a{};
     ez::DevState s{};
@@ -33,6 +49,18 @@ struct ezlda_ctx {
     hipGraphExec_t graph_exec = nullptr;
     int n_xcd = 1;      // XCDs of the device (round-robin workgroup dispatch)
     int doc_tiles = 0;  // doc tiles used for head-word items (0 = none)
+    // out-of-core execution
+    std::vector<EzChunk> chunks;       // empty: the corpus is resident
+    std::vector<uint32_t> chunk_slot0; // first slot of each chunk (lookup)
+    std::vector<uint32_t> tok_slot_g;  // token id -> corpus slot
+    uint8_t *cimg[2] = {nullptr, nullptr};  // device image buffers
+    uint32_t *cdt[2] = {nullptr, nullptr};  // DT rows of the chunk in flight
+    uint32_t *caux[2] = {nullptr, nullptr};
+    uint32_t *cctr[2] = {nullptr, nullptr};  // per-buffer sweep counter and item queues
+    double *cll[2] = {nullptr, nullptr};     // per-item log-likelihood partials
+    double *cllo[2] = {nullptr, nullptr};
+    int32_t *wt_next = nullptr, *nk_next = nullptr;  // counts of the sweep's new topics
+    hipStream_t st2 = nullptr;  // second chunk stream
 };
 
 namespace ezc {

@@ -410,7 +410,8 @@ __device__ __forceinline__ float sum_block(const RowBlock &r, float s, uint32_t 
 }
 
 // Sampler work item: a word, or a doc tile / token segment of a head word,
-// given as a range of runs.  A run is up to run_cap consecutive tokens of the
+// given as a range of runs, and the offset from the layout's token ids to
+// the corpus-wide ones that key the draws (0 unless the corpus is chunked).  A run is up to run_cap consecutive tokens of the
 // word in one document (tokens of a word are numbered by document, so the
 // repeats of a word in a document are adjacent); its tokens share n_d, the
 // max-topic counts and the DT row, hence the S sum and its checkpoints.  One
@@ -523,7 +524,7 @@ __device__ __forceinline__ void sample_item(const SampleArgs &A, uint32_t item, 
         for (uint32_t j = 0; j < len; ++j) {
             bool full = true;
             if (SHORTCUT) {
-                const float u = ez_u24(ez_philox_x0(A.seed, t0 + j, sweep, EZ_TAG_SAMPLE));
+                const float u = ez_u24(ez_philox_x0(A.seed, t0 + j + sg.w, sweep, EZ_TAG_SAMPLE));
                 const float maxS = (float)n2 * f2 + (float)(nd - n1 - n2) * f3;
                 const float B = (P1 + maxS) + Q;
                 full = !(u * B < P1 * kMargin);
@@ -572,7 +573,7 @@ __device__ __forceinline__ void sample_item(const SampleArgs &A, uint32_t item, 
             const uint32_t j = (uint32_t)__ffs(need) - 1u;
             need &= need - 1u;
             const uint32_t t = t0 + j;
-            const float u = ez_u24(ez_philox_x0(A.seed, t, sweep, EZ_TAG_SAMPLE));
+            const float u = ez_u24(ez_philox_x0(A.seed, t + sg.w, sweep, EZ_TAG_SAMPLE));
             const float target = u * total;
             uint32_t newz = k1;
             if (target < P1) {
@@ -1420,6 +1421,10 @@ hipError_t launch_zero_counts(const DevState &s, hipStream_t st) {
 hipError_t launch_build_wt(const DevState &s, hipStream_t st) {
     hipError_t e = launch_zero_counts(s, st);
     if (e != hipSuccess) return e;
+    return launch_accumulate_wt(s, st);
+}
+
+hipError_t launch_accumulate_wt(const DevState &s, hipStream_t st) {
     const size_t ldsK = (size_t)s.K * sizeof(uint32_t);
     if (s.N) hipLaunchKernelGGL(k_topic_hist, dim3(grid_for(s.N, 256 * 16, 2048)), dim3(256), ldsK, st, s.z, s.N, s.K, s.nk);
     if (s.n_wt_segs)

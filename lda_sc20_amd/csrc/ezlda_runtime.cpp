@@ -139,6 +139,17 @@ void free_device(ezlda_ctx *c) {
     c->dev_bytes = 0;
     c->layout_bytes = 0;
     c->s = ez::DevState{};
+    for (EzChunk &ch : c->chunks)
+        if (ch.img) (void)hipHostFree(ch.img);
+    c->chunks.clear();
+    c->chunk_slot0.clear();
+    c->tok_slot_g.clear();
+    for (int j = 0; j < 2; ++j) {
+        c->cimg[j] = nullptr;
+        c->cdt[j] = c->caux[j] = c->cctr[j] = nullptr;
+        c->cll[j] = c->cllo[j] = nullptr;
+    }
+    c->wt_next = c->nk_next = nullptr;
     c->loaded = false;
 }
 
@@ -296,6 +307,23 @@ std::vector<uint4> build_runs(std::vector<uint4> &items, const std::vector<uint3
     return runs;
 }
 
+// Dense head words: the first dense_words ranks, else the ranks with at
+// least dense_threshold (default K) tokens, and every word with 2^16 or more
+// tokens (tail counts are 16-bit).
+int head_words(const ezlda_ctx *c, int V, int K) {
+    auto count = [&](int r) { return (int64_t)(c->word_off[r + 1] - c->word_off[r]); };
+    int VD;
+    if (c->a.dense_words >= 0) {
+        VD = std::min(c->a.dense_words, V);
+    } else {
+        const int64_t thr = c->a.dense_threshold > 0 ? c->a.dense_threshold : K;
+        VD = 0;
+        while (VD < V && count(VD) >= thr) ++VD;
+    }
+    while (VD < V && count(VD) >= 65536) ++VD;
+    return VD;
+}
+
 // The part of the device layout that the tuning arguments decide (block_dim,
 // grid_dim, token_segment, dense_words / dense_threshold, doc_tiles,
 // run_cap): the head/tail split of WT, the WT head segments, the sampler
@@ -315,15 +343,7 @@ int build_layout(ezlda_ctx *c, const std::vector<uint32_t> &tok_doc, const std::
     c->layout_bytes = 0;
     // head / tail split (words by rank: count desc)
     auto count = [&](int r) { return (int64_t)(c->word_off[r + 1] - c->word_off[r]); };
-    int VD;
-    if (c->a.dense_words >= 0) {
-        VD = std::min(c->a.dense_words, V);
-    } else {
-        const int64_t thr = c->a.dense_threshold > 0 ? c->a.dense_threshold : K;
-        VD = 0;
-        while (VD < V && count(VD) >= thr) ++VD;
-    }
-    while (VD < V && count(VD) >= 65536) ++VD;  // tail counts are 16-bit
+    const int VD = head_words(c, V, K);
     std::vector<uint32_t> tail_off(V - VD + 1, 0);
     for (int r = VD; r < V; ++r)
         tail_off[r - VD + 1] = tail_off[r - VD] + std::min<uint32_t>((uint32_t)count(r), (uint32_t)K);
@@ -363,11 +383,298 @@ int build_layout(ezlda_ctx *c, const std::vector<uint32_t> &tok_doc, const std::
     return EZLDA_OK;
 }
 
+// ------------------------------------------------------------ out-of-core
+// num_chunks > 1 (mainKernel.cu:133-310 streams chunks through two CUDA
+// streams): every word keeps a dense count row, so counts accumulate over
+// chunks with the same kernels; each chunk has its own word-major layout
+// (the corpus order restricted to its documents), and its items carry the
+// offset from its token ids to the corpus ones, so every draw is keyed as in
+// the resident layout and the topics are the same.
+
+bool is_chunked(const ezlda_ctx *c) { return !c->chunks.empty(); }
+
+int refuse_chunked(ezlda_ctx *c, const char *what) {
+    return fail(c, EZLDA_ESTATE, std::string(what) + " is not available when the corpus is chunked (num_chunks > 1)");
+}
+
+size_t al16(size_t x) { return (x + 15u) & ~(size_t)15u; }
+
+int load_chunked(ezlda_ctx *c, const std::vector<uint32_t> &tok_doc, const std::vector<uint32_t> &tok_slot,
+                 const std::vector<uint32_t> &slot_wrd, const std::vector<uint16_t> &z,
+                 const std::vector<uint16_t> &z_doc) {
+    ez::DevState &s = c->s;
+    const int V = s.V, K = s.K, D = s.D;
+    const uint32_t N = s.N;
+    const int C = std::min<int>(c->a.num_chunks, D);
+    // boundaries in doc order: about N / C tokens each, at least one doc each
+    std::vector<uint32_t> qb(C + 1, 0);
+    for (int k = 1; k < C; ++k) {
+        const uint32_t q = (uint32_t)(std::lower_bound(c->soff.begin(), c->soff.end(),
+                                                       (uint32_t)((uint64_t)N * (uint64_t)k / (uint64_t)C)) -
+                                      c->soff.begin());
+        qb[k] = std::min<uint32_t>(std::max<uint32_t>(q, qb[k - 1] + 1), (uint32_t)(D - (C - k)));
+    }
+    qb[C] = (uint32_t)D;
+    std::vector<int32_t> chunk_of_q(D);
+    for (int k = 0; k < C; ++k)
+        for (uint32_t q = qb[k]; q < qb[k + 1]; ++q) chunk_of_q[q] = k;
+    // chunk token ids: the corpus token order restricted to the chunk
+    std::vector<uint32_t> loc(N), cnt(C, 0);
+    for (uint32_t t = 0; t < N; ++t) loc[t] = cnt[chunk_of_q[tok_doc[t]]]++;
+    const int VDi = head_words(c, V, K);  // item tiling as in the resident layout
+    const bool compact = s.compact;
+    c->chunks.assign(C, EzChunk{});
+    c->chunk_slot0.assign(C, 0);
+    size_t max_bytes = 0;
+    uint64_t max_dt = 0, dt_total = 0;
+    uint32_t max_n = 1, max_items = 1;
+    int64_t items_total = 0, runs_total = 0;
+    for (int k = 0; k < C; ++k) {
+        EzChunk &ch = c->chunks[k];
+        ch.q0 = qb[k];
+        ch.D = qb[k + 1] - qb[k];
+        ch.slot0 = c->soff[ch.q0];
+        ch.N = c->soff[qb[k + 1]] - ch.slot0;
+        c->chunk_slot0[k] = ch.slot0;
+        const uint32_t n = ch.N;
+        std::vector<uint32_t> woff(V + 1, 0), glob(n), tdoc(n), tslot(n), soffc(ch.D + 1), swrd(n), stok(n);
+        std::vector<uint16_t> zc(n), zd(n);
+        for (uint32_t pp = 0; pp < n; ++pp) {
+            const uint32_t p = ch.slot0 + pp, t = c->slot_tok[p], l = loc[t];
+            glob[l] = t;
+            tdoc[l] = tok_doc[t] - ch.q0;
+            tslot[l] = tok_slot[t] - ch.slot0;
+            zc[l] = z[t];
+            zd[pp] = z_doc[p];
+            swrd[pp] = slot_wrd[p];
+            stok[pp] = l;
+            woff[slot_wrd[p] + 1]++;
+        }
+        for (int r = 0; r < V; ++r) woff[r + 1] += woff[r];
+        for (uint32_t q = 0; q <= ch.D; ++q) soffc[q] = c->soff[ch.q0 + q] - ch.slot0;
+        std::vector<uint2> di(ch.D);
+        uint64_t dtc = 0;
+        for (uint32_t q = 0; q < ch.D; ++q) {
+            const uint32_t nq = soffc[q + 1] - soffc[q];
+            di[q] = make_uint2((uint32_t)dtc, nq << 16);
+            dtc += compact ? (dt_cap(nq, K) + 63u) & ~63u : (dt_cap(nq, K) + 31u) & ~31u;
+        }
+        if (dtc >= 0xFFFFFFF0ull) return fail(c, EZLDA_EINVAL, "DT capacity of a chunk exceeds 2^32 entries");
+        ch.dt_entries = dtc;
+        int tiles = 0;
+        std::vector<uint4> items = build_sample_items(c->a, woff, tdoc, soffc, swrd, VDi, K, c->n_xcd, &tiles);
+        const std::vector<uint4> runs = build_runs(items, tdoc, tslot, (uint32_t)c->a.run_cap);
+        for (uint4 &it : items)
+            if (it.x != 0xFFFFFFFFu) it.w = glob[woff[it.x]] - woff[it.x];
+        ch.n_queues = tiles > 0 ? (uint32_t)std::min(c->n_xcd, 8) : 1u;
+        std::vector<uint4> wsegs;
+        for (int r = 0; r < V; ++r)
+            for (uint32_t b = woff[r]; b < woff[r + 1]; b += 32768u)
+                wsegs.push_back(make_uint4((uint32_t)r, b, std::min(b + 32768u, woff[r + 1]), 0u));
+        // image
+        size_t o = 4 * (size_t)n;  // z, z_doc
+        o = al16(o);
+        ch.off_doc_info = o;
+        o += sizeof(uint2) * ch.D;
+        ch.back_bytes = o;
+        o = al16(o);
+        ch.off_soff = o;
+        o += 4 * ((size_t)ch.D + 1);
+        o = al16(o);
+        ch.off_slot_wrd = o;
+        o += 4 * (size_t)n;
+        o = al16(o);
+        ch.off_slot_tok = o;
+        o += 4 * (size_t)n;
+        o = al16(o);
+        ch.off_items = o;
+        o += sizeof(uint4) * items.size();
+        ch.off_runs = o;
+        o += sizeof(uint4) * runs.size();
+        ch.off_wt_segs = o;
+        o += sizeof(uint4) * wsegs.size();
+        ch.bytes = al16(o);
+        EZ_TRY_HIP(c, hipHostMalloc((void **)&ch.img, ch.bytes, hipHostMallocDefault));
+        std::memset(ch.img, 0, ch.bytes);
+        auto put = [&](size_t off, const void *src, size_t bytes) {
+            if (bytes) std::memcpy(ch.img + off, src, bytes);
+        };
+        put(0, zc.data(), 2 * (size_t)n);
+        put(2 * (size_t)n, zd.data(), 2 * (size_t)n);
+        put(ch.off_doc_info, di.data(), sizeof(uint2) * ch.D);
+        put(ch.off_soff, soffc.data(), 4 * ((size_t)ch.D + 1));
+        put(ch.off_slot_wrd, swrd.data(), 4 * (size_t)n);
+        put(ch.off_slot_tok, stok.data(), 4 * (size_t)n);
+        put(ch.off_items, items.data(), sizeof(uint4) * items.size());
+        put(ch.off_runs, runs.data(), sizeof(uint4) * runs.size());
+        put(ch.off_wt_segs, wsegs.data(), sizeof(uint4) * wsegs.size());
+        ch.n_items = (uint32_t)items.size();
+        ch.n_runs = runs.size();
+        ch.n_wt_segs = (uint32_t)wsegs.size();
+        max_bytes = std::max(max_bytes, ch.bytes);
+        max_dt = std::max(max_dt, dtc);
+        max_n = std::max(max_n, n);
+        max_items = std::max(max_items, ch.n_items);
+        dt_total += dtc;
+        items_total += ch.n_items;
+        runs_total += (int64_t)ch.n_runs;
+    }
+    c->tok_slot_g = tok_slot;
+    // device: two image buffers, the model and the next sweep's counts
+    int r = 0;
+    const size_t dt_words = compact ? (max_dt + 256 + 1) / 2 : max_dt + 256;
+    for (int j = 0; j < 2; ++j) {
+        if ((r = dalloc(c, &c->cimg[j], max_bytes))) return r;
+        if ((r = dalloc(c, &c->cdt[j], dt_words))) return r;
+        EZ_TRY_HIP(c, hipMemset(c->cdt[j], 0, dt_words * sizeof(uint32_t)));
+        if ((r = dalloc(c, &c->caux[j], max_n))) return r;
+        if ((r = dalloc(c, &c->cctr[j], 16))) return r;
+        EZ_TRY_HIP(c, hipMemset(c->cctr[j], 0, 16 * sizeof(uint32_t)));
+        if ((r = dalloc(c, &c->cll[j], max_items))) return r;
+        if ((r = dalloc(c, &c->cllo[j], 1))) return r;
+    }
+    if ((r = dalloc(c, &s.wt_head, (size_t)V * K))) return r;
+    if ((r = dalloc(c, &c->wt_next, (size_t)V * K))) return r;
+    if ((r = dalloc(c, &s.nk, K))) return r;
+    if ((r = dalloc(c, &c->nk_next, K))) return r;
+    if ((r = dalloc(c, &s.inv, K))) return r;
+    if ((r = dalloc(c, &s.wtop, V))) return r;
+    if ((r = dalloc(c, &s.tail_off, 1))) return r;
+    EZ_TRY_HIP(c, hipMemset(s.tail_off, 0, sizeof(uint32_t)));
+    if ((r = dalloc(c, &s.counters, 16))) return r;
+    if ((r = dalloc(c, &s.stats, 2))) return r;
+    if ((r = dalloc(c, &s.ll_out, 1))) return r;
+    EZ_TRY_HIP(c, hipMemset(s.counters, 0, 16 * sizeof(uint32_t)));
+    EZ_TRY_HIP(c, hipMemset(s.stats, 0, 2 * sizeof(unsigned long long)));
+    if (!c->st2) EZ_TRY_HIP(c, hipStreamCreateWithFlags(&c->st2, hipStreamNonBlocking));
+    s.VD = V;
+    s.n_smp_segs = (int32_t)std::min<int64_t>(items_total, INT32_MAX);
+    s.n_runs = runs_total;
+    s.block_dim = c->a.block_dim;
+    s.grid_dim = c->a.grid_dim;
+    c->dt_cap = (int64_t)dt_total;
+    c->tail_cap = 0;
+    c->doc_tiles = 0;
+    EZ_TRY_HIP(c, hipDeviceSynchronize());
+    return EZLDA_OK;
+}
+
+// Device view of chunk k in image buffer j: the chunk's arrays, the shared
+// model (every word dense), the buffer's own sweep counter and item queues.
+ez::DevState chunk_state(const ezlda_ctx *c, int k, int j) {
+    const EzChunk &ch = c->chunks[k];
+    ez::DevState cs = c->s;
+    uint8_t *img = c->cimg[j];
+    cs.D = (int32_t)ch.D;
+    cs.N = ch.N;
+    cs.VD = c->s.V;
+    cs.word_off = nullptr;
+    cs.z = reinterpret_cast<uint16_t *>(img);
+    cs.z_doc = reinterpret_cast<uint16_t *>(img + 2 * (size_t)ch.N);
+    cs.doc_info = reinterpret_cast<uint2 *>(img + ch.off_doc_info);
+    cs.soff = reinterpret_cast<uint32_t *>(img + ch.off_soff);
+    cs.slot_wrd = reinterpret_cast<uint32_t *>(img + ch.off_slot_wrd);
+    cs.slot_tok = reinterpret_cast<uint32_t *>(img + ch.off_slot_tok);
+    cs.aux_doc = c->caux[j];
+    cs.dt = c->cdt[j];
+    cs.smp_segs = reinterpret_cast<uint4 *>(img + ch.off_items);
+    cs.n_smp_segs = (int32_t)ch.n_items;
+    cs.runs = reinterpret_cast<uint4 *>(img + ch.off_runs);
+    cs.n_runs = (int64_t)ch.n_runs;
+    cs.wt_segs = reinterpret_cast<uint4 *>(img + ch.off_wt_segs);
+    cs.n_wt_segs = (int32_t)ch.n_wt_segs;
+    cs.n_queues = (int32_t)ch.n_queues;
+    cs.counters = c->cctr[j];
+    cs.ll_part = c->cll[j];
+    cs.ll_out = c->cllo[j];
+    return cs;
+}
+
+// Counts of the current topics: every chunk's histogram added up.
+int chunk_prime(ezlda_ctx *c) {
+    EZ_TRY_HIP(c, ez::launch_zero_counts(c->s, c->st));
+    for (int k = 0; k < (int)c->chunks.size(); ++k) {
+        const EzChunk &ch = c->chunks[k];
+        EZ_TRY_HIP(c, hipMemcpyAsync(c->cimg[0], ch.img, ch.bytes, hipMemcpyHostToDevice, c->st));
+        EZ_TRY_HIP(c, ez::launch_accumulate_wt(chunk_state(c, k, 0), c->st));
+    }
+    EZ_TRY_HIP(c, hipStreamSynchronize(c->st));
+    c->counts_fresh = true;
+    return EZLDA_OK;
+}
+
+// n sweeps over the chunks; chunk k goes through image buffer k % 2 on its
+// own stream, so one chunk's copies overlap the other's kernels.  Each chunk
+// rebuilds its DT rows, samples against the sweep's model, adds its new
+// topics to the next counts and copies z, z_doc and its row lengths back.
+int chunk_sweeps(ezlda_ctx *c, int n, hipEvent_t ev_begin, hipEvent_t ev_end) {
+    if (!c->counts_fresh)
+        if (int r = chunk_prime(c)) return r;
+    hipStream_t sts[2] = {c->st, c->st2};
+    hipEvent_t e_start = nullptr, e_done = nullptr;
+    int rc = EZLDA_OK;
+    if (hipEventCreateWithFlags(&e_start, hipEventDisableTiming) != hipSuccess ||
+        hipEventCreateWithFlags(&e_done, hipEventDisableTiming) != hipSuccess)
+        rc = fail(c, EZLDA_EHIP, "hipEventCreate failed");
+    auto chk = [&](hipError_t e, const char *what) {
+        if (rc == EZLDA_OK && e != hipSuccess) rc = fail(c, EZLDA_EHIP, std::string(what) + ": " + hipGetErrorString(e));
+    };
+    if (rc == EZLDA_OK && ev_begin) chk(hipEventRecord(ev_begin, c->st), "hipEventRecord");
+    const size_t vk = (size_t)c->s.V * c->s.K;
+    for (int i = 0; i < n && rc == EZLDA_OK; ++i) {
+        chk(ez::launch_topk(c->s, c->st), "top-3");
+        ez::DevState nx = c->s;  // zero the next counts
+        nx.wt_head = c->wt_next;
+        nx.nk = c->nk_next;
+        nx.VD = c->s.V;
+        chk(ez::launch_zero_counts(nx, c->st), "zero counts");
+        chk(hipEventRecord(e_start, c->st), "hipEventRecord");
+        chk(hipStreamWaitEvent(c->st2, e_start, 0), "hipStreamWaitEvent");
+        for (int k = 0; k < (int)c->chunks.size() && rc == EZLDA_OK; ++k) {
+            const int j = k & 1;
+            hipStream_t st = sts[j];
+            const EzChunk &ch = c->chunks[k];
+            const ez::DevState cs = chunk_state(c, k, j);
+            chk(hipMemcpyAsync(c->cimg[j], ch.img, ch.bytes, hipMemcpyHostToDevice, st), "chunk upload");
+            chk(ez::launch_set_u32(cs.counters, c->sweep, st), "sweep counter");
+            chk(ez::launch_build_dt(cs, st), "DT rebuild");
+            chk(ez::launch_sample(cs, c->a.shortcut != 0, st), "sample");
+            ez::DevState acc = cs;
+            acc.wt_head = c->wt_next;
+            acc.nk = c->nk_next;
+            chk(ez::launch_accumulate_wt(acc, st), "count accumulation");
+            chk(hipMemcpyAsync(ch.img, c->cimg[j], ch.back_bytes, hipMemcpyDeviceToHost, st), "chunk download");
+        }
+        chk(hipEventRecord(e_done, c->st2), "hipEventRecord");
+        chk(hipStreamWaitEvent(c->st, e_done, 0), "hipStreamWaitEvent");
+        std::swap(c->s.wt_head, c->wt_next);
+        std::swap(c->s.nk, c->nk_next);
+        c->sweep++;
+        (void)vk;
+    }
+    if (rc == EZLDA_OK && ev_end) chk(hipEventRecord(ev_end, c->st), "hipEventRecord");
+    chk(hipStreamSynchronize(c->st), "chunked sweep");
+    if (e_start) (void)hipEventDestroy(e_start);
+    if (e_done) (void)hipEventDestroy(e_done);
+    c->counts_fresh = rc == EZLDA_OK;
+    return rc;
+}
+
+// Chunk holding corpus slot p.
+int chunk_of_slot(const ezlda_ctx *c, uint32_t p) {
+    return (int)(std::upper_bound(c->chunk_slot0.begin(), c->chunk_slot0.end(), p) - c->chunk_slot0.begin()) - 1;
+}
+
 }  // namespace
 
 int ezc::refresh_counts(ezlda_ctx *c) {
     if (int r = need_loaded(c)) return r;
-    if (!c->counts_fresh) EZ_TRY_HIP(c, ez::launch_build_wt(c->s, c->st));
+    if (is_chunked(c)) {
+        if (!c->counts_fresh)
+            if (int r = chunk_prime(c)) return r;
+    } else if (!c->counts_fresh) {
+        EZ_TRY_HIP(c, ez::launch_build_wt(c->s, c->st));
+    }
     EZ_TRY_HIP(c, ez::launch_topk(c->s, c->st));
     EZ_TRY_HIP(c, hipStreamSynchronize(c->st));
     c->counts_fresh = true;
@@ -463,8 +770,10 @@ ezlda_ctx *ezlda_create(const ezlda_args *args) {
 void ezlda_destroy(ezlda_ctx *c) {
     if (!c) return;
     if (c->st) (void)hipStreamSynchronize(c->st);
+    if (c->st2) (void)hipStreamSynchronize(c->st2);
     free_device(c);
     if (c->st) (void)hipStreamDestroy(c->st);
+    if (c->st2) (void)hipStreamDestroy(c->st2);
     delete c;
 }
 
@@ -588,6 +897,22 @@ int ezlda_load_docs(ezlda_ctx *c, int32_t V, int32_t D, const int64_t *doc_off, 
     parallel_for(N, [&](int64_t b, int64_t e, int) {
         for (int64_t p = b; p < e; ++p) z_doc[p] = z[c->slot_tok[p]];
     });
+    if (c->a.num_chunks > 1 && D > 1) {  // out of core: chunk images in pinned host memory
+        ez::DevState &s = c->s;
+        s.compact = K <= (int)kCompactK;
+        s.topic_bits = s.compact ? topic_bits(K) : 16u;
+        s.K = K; s.V = V; s.D = D; s.N = N;
+        s.alpha = c->a.alpha; s.beta = c->a.beta;
+        s.vbeta = (float)((double)V * (double)c->a.beta);
+        s.seed = c->a.seed;
+        c->dt_cap_canon = 0;
+        for (int d = 0; d < D; ++d) c->dt_cap_canon += dt_cap(c->doc_off[d + 1] - c->doc_off[d], K);
+        if (int r = load_chunked(c, tok_doc, tok_slot, slot_wrd, z, z_doc)) return r;
+        c->sweep = 0;
+        c->counts_fresh = false;
+        c->loaded = true;
+        return EZLDA_OK;
+    }
     // DT rows in doc order q, each 128-byte aligned (the build zero-fills
     // the rest): K <= 8192 keeps 16-bit entries with counts in pieces
     // (capacity dt_cap(n, K) rounded up to 64 entries), larger K 32-bit
@@ -670,11 +995,13 @@ int ezlda_get_info(ezlda_ctx *c, ezlda_info *info) {
     info->block_dim = c->a.block_dim;
     info->token_segment = c->a.token_segment;
     info->num_runs = c->s.n_runs;
+    info->num_chunks = is_chunked(c) ? (int32_t)c->chunks.size() : 1;
     return EZLDA_OK;
 }
 
 int ezlda_build_wt(ezlda_ctx *c) {
     if (int r = need_loaded(c)) return r;
+    if (is_chunked(c)) return refuse_chunked(c, "ezlda_build_wt");
     EZ_TRY_HIP(c, ez::launch_build_wt(c->s, c->st));
     EZ_TRY_HIP(c, hipStreamSynchronize(c->st));
     c->counts_fresh = true;
@@ -683,6 +1010,7 @@ int ezlda_build_wt(ezlda_ctx *c) {
 
 int ezlda_topk(ezlda_ctx *c) {
     if (int r = need_loaded(c)) return r;
+    if (is_chunked(c)) return refuse_chunked(c, "ezlda_topk");
     EZ_TRY_HIP(c, ez::launch_topk(c->s, c->st));
     EZ_TRY_HIP(c, hipStreamSynchronize(c->st));
     return EZLDA_OK;
@@ -690,6 +1018,7 @@ int ezlda_topk(ezlda_ctx *c) {
 
 int ezlda_build_dt(ezlda_ctx *c) {
     if (int r = need_loaded(c)) return r;
+    if (is_chunked(c)) return refuse_chunked(c, "ezlda_build_dt");
     EZ_TRY_HIP(c, ez::launch_build_dt(c->s, c->st));
     EZ_TRY_HIP(c, hipStreamSynchronize(c->st));
     return EZLDA_OK;
@@ -697,6 +1026,7 @@ int ezlda_build_dt(ezlda_ctx *c) {
 
 int ezlda_sample(ezlda_ctx *c) {
     if (int r = need_loaded(c)) return r;
+    if (is_chunked(c)) return refuse_chunked(c, "ezlda_sample");
     EZ_TRY_HIP(c, ez::launch_sample(c->s, c->a.shortcut != 0, c->st));
     EZ_TRY_HIP(c, ez::launch_advance_sweep(c->s, c->st));
     EZ_TRY_HIP(c, hipStreamSynchronize(c->st));
@@ -723,6 +1053,7 @@ static int enqueue_sweep(ezlda_ctx *c, hipEvent_t *ev) {
 
 int ezlda_sweep(ezlda_ctx *c, int32_t n) {
     if (int r = need_loaded(c)) return r;
+    if (is_chunked(c)) return chunk_sweeps(c, n, nullptr, nullptr);
     for (int i = 0; i < n; ++i) {
         if (int r = enqueue_sweep(c, nullptr)) return r;
         c->sweep++;
@@ -750,6 +1081,7 @@ static int ensure_graph(ezlda_ctx *c) {
 
 int ezlda_sweep_graph(ezlda_ctx *c, int32_t n) {
     if (int r = need_loaded(c)) return r;
+    if (is_chunked(c)) return refuse_chunked(c, "ezlda_sweep_graph");
     if (int r = ensure_graph(c)) return r;
     for (int i = 0; i < n; ++i) {
         EZ_TRY_HIP(c, hipGraphLaunch(c->graph_exec, c->st));
@@ -793,6 +1125,7 @@ void layout_inputs(ezlda_ctx *c, std::vector<uint32_t> &tok_doc, std::vector<uin
 
 int ezlda_set_layout(ezlda_ctx *c, const ezlda_args *args) {
     if (int r = need_loaded(c)) return r;
+    if (is_chunked(c)) return refuse_chunked(c, "ezlda_set_layout");
     if (!args) return fail(c, EZLDA_EINVAL, "null args");
     ezlda_args a = c->a;
     a.block_dim = args->block_dim;
@@ -813,6 +1146,7 @@ int ezlda_set_layout(ezlda_ctx *c, const ezlda_args *args) {
 int ezlda_autotune(ezlda_ctx *c, int32_t sweeps_per_trial, ezlda_args *chosen, double *ms_per_sweep,
                    ezlda_tune_trial *trials, int32_t max_trials, int32_t *num_trials) {
     if (int r = need_loaded(c)) return r;
+    if (is_chunked(c)) return refuse_chunked(c, "ezlda_autotune");
     if (sweeps_per_trial < 1) return fail(c, EZLDA_EINVAL, "sweeps_per_trial must be >= 1");
     std::vector<uint32_t> tok_doc, tok_slot, slot_wrd;
     layout_inputs(c, tok_doc, tok_slot, slot_wrd);
@@ -892,6 +1226,25 @@ int ezlda_autotune(ezlda_ctx *c, int32_t sweeps_per_trial, ezlda_args *chosen, d
 int ezlda_time_sweeps(ezlda_ctx *c, int32_t n, int32_t use_graph, double *total_ms, double *stage_ms) {
     if (int r = need_loaded(c)) return r;
     if (n < 1) return fail(c, EZLDA_EINVAL, "num_sweeps must be >= 1");
+    if (is_chunked(c)) {  // whole sweeps only: stages overlap across the chunk streams
+        if (use_graph) return refuse_chunked(c, "a captured sweep");
+        hipEvent_t e0 = nullptr, e1 = nullptr;
+        if (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess) {
+            if (e0) (void)hipEventDestroy(e0);
+            return fail(c, EZLDA_EHIP, "hipEventCreate failed");
+        }
+        int rc = chunk_sweeps(c, n, e0, e1);
+        float ms = 0.f;
+        if (rc == EZLDA_OK && hipEventElapsedTime(&ms, e0, e1) != hipSuccess)
+            rc = fail(c, EZLDA_EHIP, "hipEventElapsedTime failed");
+        (void)hipEventDestroy(e0);
+        (void)hipEventDestroy(e1);
+        if (rc) return rc;
+        if (total_ms) *total_ms = ms;
+        if (stage_ms)
+            for (int k = 0; k < EZLDA_NUM_STAGES; ++k) stage_ms[k] = -1.0;
+        return EZLDA_OK;
+    }
     double st[EZLDA_NUM_STAGES] = {0, 0, 0, 0};
     double tot = 0.0;
     if (use_graph) {
@@ -954,6 +1307,16 @@ int ezlda_get_stats(ezlda_ctx *c, uint64_t *stats) {
     EZ_TRY_HIP(c, hipMemcpy(v, c->s.stats, sizeof v, hipMemcpyDeviceToHost));
     stats[0] = v[0];
     stats[1] = v[1];
+    if (is_chunked(c)) {  // every row is dense; DT row lengths come back with each chunk
+        uint64_t b = 0;
+        for (const EzChunk &ch : c->chunks) {
+            const uint2 *di = reinterpret_cast<const uint2 *>(ch.img + ch.off_doc_info);
+            for (uint32_t q = 0; q < ch.D; ++q) b += di[q].y & 0xFFFFu;
+        }
+        stats[2] = 0;
+        stats[3] = b;
+        return EZLDA_OK;
+    }
     std::vector<uint32_t> tn(c->s.V - c->s.VD);
     std::vector<uint2> di(c->s.D);
     if (!tn.empty()) EZ_TRY_HIP(c, hipMemcpy(tn.data(), c->s.tail_nnz, tn.size() * 4, hipMemcpyDeviceToHost));
@@ -975,6 +1338,23 @@ int ezlda_reset_stats(ezlda_ctx *c) {
 
 int ezlda_loglik(ezlda_ctx *c, double *ll) {
     if (int r = need_loaded(c)) return r;
+    if (is_chunked(c)) {  // per chunk, summed in chunk order
+        if (int r = ezc::refresh_counts(c)) return r;
+        double tot = 0.0;
+        for (int k = 0; k < (int)c->chunks.size(); ++k) {
+            const EzChunk &ch = c->chunks[k];
+            const ez::DevState cs = chunk_state(c, k, 0);
+            double v = 0.0;
+            EZ_TRY_HIP(c, hipMemcpyAsync(c->cimg[0], ch.img, ch.bytes, hipMemcpyHostToDevice, c->st));
+            EZ_TRY_HIP(c, ez::launch_build_dt(cs, c->st));
+            EZ_TRY_HIP(c, ez::launch_loglik(cs, c->st));
+            EZ_TRY_HIP(c, hipMemcpyAsync(&v, cs.ll_out, sizeof(double), hipMemcpyDeviceToHost, c->st));
+            EZ_TRY_HIP(c, hipStreamSynchronize(c->st));
+            tot += v;
+        }
+        *ll = tot;
+        return EZLDA_OK;
+    }
     if (!c->counts_fresh) EZ_TRY_HIP(c, ez::launch_build_wt(c->s, c->st));
     c->counts_fresh = true;
     EZ_TRY_HIP(c, ez::launch_topk(c->s, c->st));
@@ -987,6 +1367,14 @@ int ezlda_loglik(ezlda_ctx *c, double *ll) {
 
 int ezlda_get_topics(ezlda_ctx *c, uint16_t *zout) {
     if (int r = need_loaded(c)) return r;
+    if (is_chunked(c)) {  // from the chunk images (z by doc slot)
+        for (size_t i = 0; i < c->doc_tok.size(); ++i) {
+            const uint32_t p = c->tok_slot_g[c->doc_tok[i]];
+            const EzChunk &ch = c->chunks[chunk_of_slot(c, p)];
+            zout[i] = reinterpret_cast<const uint16_t *>(ch.img + 2 * (size_t)ch.N)[p - ch.slot0];
+        }
+        return EZLDA_OK;
+    }
     std::vector<uint16_t> z(c->s.N);
     EZ_TRY_HIP(c, hipMemcpy(z.data(), c->s.z, z.size() * sizeof(uint16_t), hipMemcpyDeviceToHost));
     for (size_t i = 0; i < z.size(); ++i) zout[i] = z[c->doc_tok[i]];
@@ -995,6 +1383,20 @@ int ezlda_get_topics(ezlda_ctx *c, uint16_t *zout) {
 
 int ezlda_set_topics(ezlda_ctx *c, const uint16_t *zin) {
     if (int r = need_loaded(c)) return r;
+    if (is_chunked(c)) {
+        for (size_t i = 0; i < c->doc_tok.size(); ++i)
+            if (zin[i] >= c->s.K) return fail(c, EZLDA_EINVAL, "topic out of range");
+        for (size_t i = 0; i < c->doc_tok.size(); ++i) {
+            const uint32_t p = c->tok_slot_g[c->doc_tok[i]];
+            EzChunk &ch = c->chunks[chunk_of_slot(c, p)];
+            const uint32_t pp = p - ch.slot0;
+            const uint32_t l = reinterpret_cast<const uint32_t *>(ch.img + ch.off_slot_tok)[pp];
+            reinterpret_cast<uint16_t *>(ch.img)[l] = zin[i];
+            reinterpret_cast<uint16_t *>(ch.img + 2 * (size_t)ch.N)[pp] = zin[i];
+        }
+        c->counts_fresh = false;
+        return EZLDA_OK;
+    }
     std::vector<uint16_t> z(c->s.N), z_doc(c->s.N);
     for (size_t i = 0; i < z.size(); ++i) {
         if (zin[i] >= c->s.K) return fail(c, EZLDA_EINVAL, "topic out of range");
@@ -1055,6 +1457,7 @@ int ezlda_get_wt_dense(ezlda_ctx *c, int32_t *wt) {
 
 int ezlda_export_model(ezlda_ctx *c, const char *dir) {
     if (int r = need_loaded(c)) return r;
+    if (is_chunked(c)) return refuse_chunked(c, "ezlda_export_model");
     if (!dir) return fail(c, EZLDA_EINVAL, "null directory");
     if (int r = ezc::refresh_counts(c)) return r;
     EZ_TRY_HIP(c, ez::launch_build_dt(c->s, c->st));
@@ -1110,6 +1513,7 @@ int ezlda_export_model(ezlda_ctx *c, const char *dir) {
 
 int ezlda_get_dt_raw(ezlda_ctx *c, uint32_t *entries, int32_t *nnz) {
     if (int r = need_loaded(c)) return r;
+    if (is_chunked(c)) return refuse_chunked(c, "ezlda_get_dt_raw");
     const ez::DevState &s = c->s;
     std::vector<uint2> di(s.D);
     std::vector<uint32_t> dt(s.compact ? (c->dt_cap + 1) / 2 : c->dt_cap);
@@ -1151,6 +1555,7 @@ int ezlda_get_top(ezlda_ctx *c, uint16_t *top) {
 
 int ezlda_get_aux(ezlda_ctx *c, uint32_t *aux) {
     if (int r = need_loaded(c)) return r;
+    if (is_chunked(c)) return refuse_chunked(c, "ezlda_get_aux");
     std::vector<uint32_t> a(c->s.N), at(c->s.N);
     EZ_TRY_HIP(c, hipMemcpy(a.data(), c->s.aux_doc, a.size() * 4, hipMemcpyDeviceToHost));
     for (size_t p = 0; p < a.size(); ++p) at[c->slot_tok[p]] = a[p];

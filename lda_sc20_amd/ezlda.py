@@ -57,7 +57,7 @@ class Info(C.Structure):
                 ("wt_tail_capacity", C.c_int64), ("num_sample_items", C.c_int32), ("sweep", C.c_uint32),
                 ("device_bytes", C.c_int64), ("doc_tiles", C.c_int32), ("num_xcd", C.c_int32),
                 ("run_cap", C.c_int32), ("grid_dim", C.c_int32), ("block_dim", C.c_int32),
-                ("token_segment", C.c_int32), ("num_runs", C.c_int64)]
+                ("token_segment", C.c_int32), ("num_runs", C.c_int64), ("num_chunks", C.c_int32)]
 
 
 class TuneTrial(C.Structure):
@@ -250,7 +250,8 @@ def read_chunked(prefix: str, num_chunks: int = 0) -> Corpus:
 
 # ------------------------------------------------------------------ context
 class Context:
-    """One ezlda_ctx: a corpus resident on the current HIP device."""
+    """One ezlda_ctx: a corpus on the current HIP device (resident, or chunked
+    through pinned host memory when num_chunks > 1)."""
 
     def __init__(self, args: Optional[Args] = None, **kw):
         self._L = lib()
