@@ -249,6 +249,9 @@ def main() -> int:
                     help="Poisson arrival rates as fractions of the burst's docs/s")
     ap.add_argument("--serve-seconds", type=float, default=2.0)
     ap.add_argument("--serve-slots", type=int, default=2048)
+    ap.add_argument("--chunks", type=int, default=1,
+                    help="> 1: out-of-core sweeps over that many chunks in pinned host memory (times whole sweeps, "
+                         "PCIe transfers included)")
     ap.add_argument("--serve-budget", type=int, default=1 << 30, help="HBM bytes for resident requests")
     ap.add_argument("--serve-plain-docs", type=int, default=32)
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
@@ -275,7 +278,10 @@ def main() -> int:
     dense = -1 if wl["head_frac"] is None else int(math.ceil(wl["head_frac"] * corpus.V))
     ctx = ezlda.Context(K=K, alpha=50.0 / K, beta=0.01, seed=wl["seed"], dense_words=dense, shortcut=1,
                         doc_tiles=args.doc_tiles, run_cap=args.run_cap, grid_dim=args.grid_dim,
-                        block_dim=args.block_dim)
+                        block_dim=args.block_dim, num_chunks=args.chunks)
+    chunked = args.chunks > 1
+    if chunked:  # whole sweeps only (no stage times, graphs or layout tuning)
+        args.no_graph, args.autotune = True, False
     t = time.perf_counter()
     ctx.load(corpus)
     info = ctx.info()
@@ -331,8 +337,8 @@ def main() -> int:
     eb = 2 if K <= 8192 else 4
     s_scanned = n_scanned / K_steps
     b_sample = 4 * N + 28 * R + 4 * K * VD + 4 * tail_nnz + 4 * K + eb * s_scanned
-    t_sample = stage_ms["sample"] / K_steps / 1e3
-    achieved = b_sample / t_sample / 1e9
+    t_sample = stage_ms["sample"] / K_steps / 1e3 if not chunked else None
+    achieved = b_sample / t_sample / 1e9 if not chunked else None
     # whole-sweep algorithmic bytes (same accounting for every stage)
     b_sweep = (b_sample                                                      # sample
                + 2 * N + 2 * N + 4 * K * VD + 4 * tail_nnz                  # n_k + WT rebuild
@@ -362,8 +368,9 @@ def main() -> int:
                    "num_head_words": VD, "shortcut": True, "sweeps_before_timing": sweeps_before,
                    "doc_tiles": int(info.doc_tiles), "run_cap": int(info.run_cap), "num_runs": int(info.num_runs),
                    "grid_dim": int(info.grid_dim), "block_dim": int(info.block_dim),
-                   "token_segment": int(info.token_segment)},
-        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                   "token_segment": int(info.token_segment), "num_chunks": int(info.num_chunks)},
+        "roofline": None if chunked else
+                    {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": measured_traffic(args.workload),
                      "kernel": "k_sample", "bytes_per_launch": b_sample, "avg_launch_ms": t_sample * 1e3,
                      "traffic_source": "profiles/hbm_traffic_cfg2.md (FETCH_SIZE, WRITE_SIZE; copy-calibrated)"},
