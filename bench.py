@@ -8,7 +8,9 @@ Zipf(1.07) vocabulary, dense WT head = top 1% of words), on one MI355X.
 A step is one whole sweep: WT/n_k rebuild, top-3, DT rebuild, sampling.  The
 corpus and model are resident in HBM before the timed region; the K timed
 sweeps run back to back on the library's stream with HIP events around every
-stage (no host synchronisation inside the region).  Prints one JSON line.
+stage (no host synchronisation inside the region); they give the value, the
+stage split and the sampler's roofline.  K more sweeps replayed from one
+captured hipGraph are reported beside them.  Prints one JSON line.
 """
 from __future__ import annotations
 
@@ -290,7 +292,11 @@ def main() -> int:
         f"device={info.device_bytes / 2**30:.2f} GiB")
 
     t = time.perf_counter()
-    ctx.sweep(args.warmup)
+    if args.no_graph or args.warmup < 1:
+        ctx.sweep(args.warmup)
+    else:  # the last warm-up sweep captures and first launches the graph
+        ctx.sweep(args.warmup - 1)
+        ctx.sweep_graph(1)
     log(f"{args.warmup} warm-up sweeps in {time.perf_counter() - t:.2f}s")
     tuning = None
     if args.autotune:
@@ -315,17 +321,22 @@ def main() -> int:
 
     device_sync()
     sweeps_before = int(ctx.info().sweep)
+    K_steps = args.steps
+    N = int(info.num_tokens)
     ctx.reset_stats()
     t0 = time.perf_counter()
-    total_ms, stage_ms = ctx.time_sweeps(args.steps, use_graph=False)
+    total_ms, stage_ms = ctx.time_sweeps(K_steps, use_graph=False)
     device_sync()
     wall = time.perf_counter() - t0
     n_full, n_scanned, tail_nnz, dt_nnz = ctx.stats()
-    K_steps = args.steps
-    N = int(info.num_tokens)
-    ms_per_step = total_ms / K_steps
+    eager_ms = ms_per_step = total_ms / K_steps
     value = N * K_steps / (total_ms / 1e3)
     log(f"timed {K_steps} sweeps: {total_ms:.2f} ms (wall {wall * 1e3:.1f} ms), stages {stage_ms}")
+    graph_ms = None
+    if not args.no_graph:  # the next K sweeps as replays of the captured sweep
+        gtot, _ = ctx.time_sweeps(K_steps, use_graph=True)
+        graph_ms = gtot / K_steps
+        log(f"{K_steps} more sweeps as graph replays: {graph_ms:.3f} ms/sweep")
 
     # algorithmic bytes of one sampler launch: the two z writes per token (by
     # token and by doc slot); per run (a word's tokens in one doc) its entry,
@@ -344,11 +355,6 @@ def main() -> int:
                + 2 * N + 2 * N + 4 * K * VD + 4 * tail_nnz                  # n_k + WT rebuild
                + 4 * K * VD + 4 * tail_nnz + 8 * info.num_words             # top-3
                + 2 * N + 4 * N + 4 * N + eb * dt_nnz + 8 * info.num_docs)   # DT + aux (z_doc, word, aux)
-    graph_ms = None
-    if not args.no_graph:
-        gms, _ = ctx.time_sweeps(K_steps, use_graph=True)
-        graph_ms = gms / K_steps
-        log(f"graph replay: {graph_ms:.3f} ms/sweep")
     ll = ctx.loglik()
     out = {
         "metric": "sampled tokens/s per sweep at K=1024 (+% HBM peak)" if K == 1024 else
