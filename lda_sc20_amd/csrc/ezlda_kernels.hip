@@ -896,6 +896,27 @@ __global__ void __launch_bounds__(256) k_fi_admit(const uint4 *__restrict__ adm,
     }
 }
 
+// f(value, i) for every i < n of a document's pool range, eight loads per
+// thread issued before their uses (long documents would otherwise chain one
+// load latency per 256 tokens).
+template <class T, class F>
+__device__ __forceinline__ void for_doc_tokens(const T *__restrict__ p, uint32_t n, F f) {
+    constexpr int U = 8;
+    for (uint32_t i0 = threadIdx.x; i0 < n; i0 += U * blockDim.x) {
+        T v[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const uint32_t i = i0 + (uint32_t)u * blockDim.x;
+            v[u] = i < n ? p[i] : T(0);
+        }
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const uint32_t i = i0 + (uint32_t)u * blockDim.x;
+            if (i < n) f(v[u], i);
+        }
+    }
+}
+
 // Rows of the resident documents (one workgroup per slot): topics drawn from
 // the init stream on sweep 0, a 16-bit LDS histogram, a row of topic | count
 // << 16 in ascending topic order, zero padded to a multiple of kFB.
@@ -917,7 +938,7 @@ __global__ void __launch_bounds__(256) k_fi_rows(const uint4 *__restrict__ slots
                 z[i] = (uint16_t)(ez_philox_x0(seed, ((uint64_t)rid << 32) | i, 0u, EZ_TAG_FOLDIN_INIT) % (uint32_t)K);
             __syncthreads();
         }
-        for (uint32_t i = tid; i < n; i += bd) h16_add(h, z[i]);
+        for_doc_tokens(z, n, [&](uint16_t t, uint32_t) { h16_add(h, t); });
         __syncthreads();
         if (wv == 0) {  // one wavefront compacts the row (and clears the counts)
             uint32_t *row = rows + off;
@@ -949,11 +970,10 @@ __global__ void __launch_bounds__(256) k_fi_count(const uint4 *__restrict__ slot
     __syncthreads();
     for (int j = blockIdx.x; j < nslots; j += gridDim.x) {
         const uint4 sl = slots[j];
-        for (uint32_t i = tid; i < sl.z; i += bd) {
-            const uint32_t w = wpool[sl.y + i];
+        for_doc_tokens(wpool + sl.y, sl.z, [&](uint32_t w, uint32_t) {
             if (w < kFoldHot) atomicAdd(&hc[w], 1u);
             else atomicAdd(&wcount[w], 1u);
-        }
+        });
     }
     __syncthreads();
     for (uint32_t k = tid; k < kFoldHot; k += bd)
@@ -1071,24 +1091,33 @@ __global__ void __launch_bounds__(256) k_fi_scatter(const uint4 *__restrict__ sl
     __syncthreads();
     for (int j = blockIdx.x; j < nslots; j += gridDim.x) {
         const uint4 sl = slots[j];
-        for (uint32_t i = tid; i < sl.z; i += bd) {
-            const uint32_t w = wpool[sl.y + i];
+        for_doc_tokens(wpool + sl.y, sl.z, [&](uint32_t w, uint32_t i) {
             if (w < kFoldHot) atomicAdd(&hc[w], 1u);
             else tokw[atomicAdd(&wcursor[w], 1u)] = ((uint32_t)j << 16) | i;
-        }
+        });
     }
     __syncthreads();
-    for (uint32_t k = tid; k < kFoldHot; k += bd) {
-        const uint32_t c = hc[k];
-        if (c) hc[k] = atomicAdd(&wcursor[k], c);
+    {  // one range per hot word of the workgroup; the atomics issued together
+        constexpr uint32_t U = kFoldHot / 256;
+        uint32_t c[U], b[U];
+#pragma unroll
+        for (uint32_t u = 0; u < U; ++u) c[u] = tid + u * bd < kFoldHot ? hc[tid + u * bd] : 0u;
+#pragma unroll
+        for (uint32_t u = 0; u < U; ++u) b[u] = c[u] ? atomicAdd(&wcursor[tid + u * bd], c[u]) : 0u;
+#pragma unroll
+        for (uint32_t u = 0; u < U; ++u)
+            if (c[u]) hc[tid + u * bd] = b[u];
+        for (uint32_t k = tid + U * bd; k < kFoldHot; k += bd) {  // block sizes below 256
+            const uint32_t cc = hc[k];
+            if (cc) hc[k] = atomicAdd(&wcursor[k], cc);
+        }
     }
     __syncthreads();
     for (int j = blockIdx.x; j < nslots; j += gridDim.x) {
         const uint4 sl = slots[j];
-        for (uint32_t i = tid; i < sl.z; i += bd) {
-            const uint32_t w = wpool[sl.y + i];
+        for_doc_tokens(wpool + sl.y, sl.z, [&](uint32_t w, uint32_t i) {
             if (w < kFoldHot) tokw[atomicAdd(&hc[w], 1u)] = ((uint32_t)j << 16) | i;
-        }
+        });
     }
 }
 
@@ -1313,11 +1342,10 @@ __global__ void __launch_bounds__(256) k_fi_final(const uint4 *__restrict__ slot
         const uint4 sl = slots[j];
         for (int k = tid; k < KH; k += bd) shf2[k] = 0;
         __syncthreads();
-        for (uint32_t i = tid; i < sl.z; i += bd) {
-            const uint16_t t = zpool[sl.y + i];
+        for_doc_tokens(zpool + sl.y, sl.z, [&](uint16_t t, uint32_t i) {
             h16_add(shf2, t);
             zout[f.y + i] = t;
-        }
+        });
         __syncthreads();
         int32_t *co = counts_out + (size_t)f.x * K;
         for (int k = tid; k < K; k += bd) co[k] = (int32_t)h16_get(shf2, (uint32_t)k);
