@@ -58,7 +58,8 @@ typedef struct ezlda_args {
                                 the counts (ezlda_get_nk, _get_wt_dense) are
                                 those of the current topics.  The per-stage
                                 calls, graphs, layout tuning and DT/aux
-                                getters then return EZLDA_ESTATE.
+                                getters then return EZLDA_ESTATE; model
+                                export works in both modes.
                                 <= 1 = the corpus is resident */
     int32_t num_streams;     /* Argument.cuh:7; recorded (chunked sweeps use 2) */
     int32_t token_segment;   /* Doc.cuh:30: max tokens of one word per sampler

@@ -1457,11 +1457,8 @@ int ezlda_get_wt_dense(ezlda_ctx *c, int32_t *wt) {
 
 int ezlda_export_model(ezlda_ctx *c, const char *dir) {
     if (int r = need_loaded(c)) return r;
-    if (is_chunked(c)) return refuse_chunked(c, "ezlda_export_model");
     if (!dir) return fail(c, EZLDA_EINVAL, "null directory");
     if (int r = ezc::refresh_counts(c)) return r;
-    EZ_TRY_HIP(c, ez::launch_build_dt(c->s, c->st));
-    EZ_TRY_HIP(c, hipStreamSynchronize(c->st));
     const ez::DevState &s = c->s;
     const std::string d(dir);
     std::vector<int32_t> wt((size_t)s.V * s.K), nk(s.K);
@@ -1480,25 +1477,26 @@ int ezlda_export_model(ezlda_ctx *c, const char *dir) {
         ok = (std::fclose(f) == 0) && ok;
         if (!ok) return fail(c, EZLDA_EIO, "short write to " + d + "/phi.f32");
     }
-    {
-        std::vector<uint32_t> ent((size_t)std::max<int64_t>(c->dt_cap_canon, 1));
-        std::vector<int32_t> nnz(std::max(s.D, 1));
-        if (int r = ezlda_get_dt_raw(c, ent.data(), nnz.data())) return r;
+    {  // per-document topic counts of the current topics, ascending topic
+        std::vector<uint16_t> z(s.N);
+        if (int r = ezlda_get_topics(c, z.data())) return r;
         std::FILE *f = std::fopen((d + "/theta.txt").c_str(), "w");
         if (!f) return fail(c, EZLDA_EIO, "cannot write " + d + "/theta.txt");
-        uint64_t off = 0;
+        std::vector<uint32_t> cnt(s.K, 0);
+        std::vector<uint16_t> seen;
         bool ok = true;
         for (int doc = 0; doc < s.D && ok; ++doc) {  // input doc order
-            const uint32_t n = c->doc_off[doc + 1] - c->doc_off[doc];
+            const uint32_t b = c->doc_off[doc], n = c->doc_off[doc + 1] - b;
             ok = std::fprintf(f, "%u", n) > 0;
-            for (int32_t i = 0; i < nnz[doc];) {  // merge a topic's pieces
-                const uint32_t k = ent[off + i] & 0xFFFFu;
-                uint32_t cnt = 0;
-                for (; i < nnz[doc] && (ent[off + i] & 0xFFFFu) == k; ++i) cnt += ent[off + i] >> 16;
-                ok = ok && std::fprintf(f, " %u:%u", k, cnt) > 0;
+            seen.clear();
+            for (uint32_t i = 0; i < n; ++i)
+                if (cnt[z[b + i]]++ == 0) seen.push_back(z[b + i]);
+            std::sort(seen.begin(), seen.end());
+            for (const uint16_t k : seen) {
+                ok = ok && std::fprintf(f, " %u:%u", (unsigned)k, cnt[k]) > 0;
+                cnt[k] = 0;
             }
             ok = ok && std::fputc('\n', f) != EOF;
-            off += dt_cap(n, s.K);
         }
         ok = (std::fclose(f) == 0) && ok;
         if (!ok) return fail(c, EZLDA_EIO, "short write to " + d + "/theta.txt");

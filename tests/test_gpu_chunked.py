@@ -69,6 +69,11 @@ def test_chunked_topics_checkpoint_and_refusals(tmp_path):
     chk.sweep(2)
     res.sweep(2)
     assert np.array_equal(chk.get_topics(), res.get_topics())
+    for ctx, name in ((chk, "chunked"), (res, "resident")):
+        os.makedirs(tmp_path / name)
+        ctx.export_model(str(tmp_path / name))
+    for f in ("phi.f32", "theta.txt", "model.txt"):
+        assert open(tmp_path / "chunked" / f, "rb").read() == open(tmp_path / "resident" / f, "rb").read(), f
     path = str(tmp_path / "ck.bin")
     chk.save_checkpoint(path)
     chk.sweep(3)
