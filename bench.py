@@ -158,11 +158,16 @@ def serving_bench(ctx, ezlda, wl: dict, args) -> dict:
         gaps = rng.exponential(1.0 / qps, n) if qps else np.zeros(n)
         t0 = time.perf_counter()
         at = t0 + np.cumsum(gaps)
-        for j in range(n):
-            if qps:
+        if qps:  # Poisson arrivals, one document per call
+            for j in range(n):
                 _wait_until(at[j])
-            srv.submit(rid, docs[rid % len(docs)])
-            rid += 1
+                srv.submit(rid, docs[rid % len(docs)])
+                rid += 1
+        else:  # the burst: one batch call
+            batch = [docs[(rid + j) % len(docs)] for j in range(n)]
+            boff = np.concatenate([[0], np.cumsum([len(b) for b in batch])]).astype(np.int64)
+            srv.submit_many(np.arange(rid, rid + n, dtype=np.uint32), boff, np.concatenate(batch))
+            rid += n
         submit_s = time.perf_counter() - t0
         srv.drain()
         wall = time.perf_counter() - t0

@@ -272,6 +272,12 @@ void ezlda_server_destroy(ezlda_server *srv);
 const char *ezlda_server_last_error(const ezlda_server *srv);
 /* Queues one document (input word ids). */
 int ezlda_server_submit(ezlda_server *srv, uint32_t request_id, const int32_t *words, int32_t n);
+/* Queues n_docs documents at once: document d has id request_ids[d] and the
+ * words [doc_off[d], doc_off[d+1]) (doc_off[0] = 0).  Every document is
+ * checked first; if any is rejected (as ezlda_server_submit would reject it,
+ * or an id repeats within the batch), none is queued. */
+int ezlda_server_submit_many(ezlda_server *srv, int32_t n_docs, const uint32_t *request_ids, const int64_t *doc_off,
+                             const int32_t *words);
 /* Blocks until the request is done (timeout_ms < 0: no limit) and copies its
  * final topics (n values) and topic counts (K values); either may be NULL.
  * The result is then released.  An id that is neither queued, resident nor

@@ -273,8 +273,12 @@ void worker_main(ezlda_server *s) {
         UpLayout L{};
         {
             std::unique_lock<std::mutex> lk(s->mu);
-            // admission: a free slot, pool room and upload room, in arrival order
-            while (!s->pending.empty() && !s->free_slots.empty()) {
+            // admission: a free slot, pool room and upload room, in arrival
+            // order, at most 2 * ceil(slots / sweeps) documents per step: a
+            // burst then enters as a steady flow instead of a cohort that
+            // finishes (and is replaced) in one step
+            const size_t pace = 2 * (((size_t)s->max_slots + (size_t)s->sweeps - 1) / (size_t)s->sweeps);
+            while (!s->pending.empty() && !s->free_slots.empty() && adm.size() < pace) {
                 const uint32_t n = (uint32_t)s->pending.front().words.size();
                 if (up_words + n > s->up_cap) break;
                 uint32_t off = 0;
@@ -499,6 +503,48 @@ int ezlda_server_submit(ezlda_server *s, uint32_t request_id, const int32_t *wor
     if (s->t_first < 0) s->t_first = rq.t_submit;
     s->submitted++;
     s->pending.push_back(std::move(rq));
+    s->cv_work.notify_one();
+    return EZLDA_OK;
+}
+
+int ezlda_server_submit_many(ezlda_server *s, int32_t n_docs, const uint32_t *request_ids, const int64_t *doc_off,
+                             const int32_t *words) {
+    if (!s || n_docs < 0 || (n_docs > 0 && (!request_ids || !doc_off)) || (n_docs > 0 && doc_off[0] != 0))
+        return sfail(s, EZLDA_EINVAL, "bad fold-in batch");
+    std::vector<Request> rq((size_t)n_docs);
+    for (int32_t d = 0; d < n_docs; ++d) {  // validate and map everything before queuing anything
+        const int64_t n = doc_off[d + 1] - doc_off[d];
+        if (n < 0) return sfail(s, EZLDA_EINVAL, "doc_off must be non-decreasing");
+        if (n > 65535) return sfail(s, EZLDA_EINVAL, "a fold-in document has more than 65535 tokens");
+        if (n > 0 && !words) return sfail(s, EZLDA_EINVAL, "bad fold-in batch");
+        rq[d].rid = request_ids[d];
+        rq[d].words.resize((size_t)n);
+        const int32_t *w = words + doc_off[d];
+        for (int64_t i = 0; i < n; ++i) {
+            if (w[i] < 0 || w[i] >= s->V) return sfail(s, EZLDA_EINVAL, "word id out of range in fold-in request");
+            rq[d].words[i] = (uint32_t)s->rank_of[w[i]];
+        }
+    }
+    std::lock_guard<std::mutex> lk(s->mu);
+    if (s->worker_rc != EZLDA_OK) return sfail(s, s->worker_rc, s->err);
+    std::unordered_set<uint32_t> ids;
+    for (const Request &r : rq) {
+        if (r.words.size() > s->budget_tokens) {
+            s->rejected++;
+            return sfail(s, EZLDA_ENOMEM, "document larger than the server's HBM token budget");
+        }
+        if (s->done.count(r.rid)) return sfail(s, EZLDA_EINVAL, "request id already has an unclaimed result");
+        if (s->inflight.count(r.rid) || !ids.insert(r.rid).second)
+            return sfail(s, EZLDA_EINVAL, "request id already in flight");
+    }
+    const double t = now_ms();
+    if (s->t_first < 0 && n_docs > 0) s->t_first = t;
+    for (Request &r : rq) {
+        s->inflight.insert(r.rid);
+        r.t_submit = t;
+        s->pending.push_back(std::move(r));
+    }
+    s->submitted += n_docs;
     s->cv_work.notify_one();
     return EZLDA_OK;
 }

@@ -31,7 +31,7 @@ EXPORTS = (
     "ezlda_corpus_free", "ezlda_corpus_from_arrays", "ezlda_read_docword", "ezlda_write_chunked",
     "ezlda_read_chunked", "ezlda_selftest_philox", "ezlda_selftest_copy", "ezlda_get_stats",
     "ezlda_reset_stats", "ezlda_server_create", "ezlda_server_destroy", "ezlda_server_last_error",
-    "ezlda_server_submit", "ezlda_server_wait", "ezlda_server_drain", "ezlda_server_get_stats",
+    "ezlda_server_submit", "ezlda_server_submit_many", "ezlda_server_wait", "ezlda_server_drain", "ezlda_server_get_stats",
     "ezlda_server_reset_stats", "ezlda_foldin_plain", "ezlda_set_layout", "ezlda_autotune",
     "ezlda_export_model",
 )
@@ -147,6 +147,7 @@ def lib() -> C.CDLL:
         "ezlda_server_destroy": (None, [P]),
         "ezlda_server_last_error": (C.c_char_p, [P]),
         "ezlda_server_submit": (C.c_int, [P, C.c_uint32, P, C.c_int32]),
+        "ezlda_server_submit_many": (C.c_int, [P, C.c_int32, P, P, P]),
         "ezlda_server_wait": (C.c_int, [P, C.c_uint32, P, P, C.POINTER(C.c_double), C.c_int32]),
         "ezlda_server_drain": (C.c_int, [P]),
         "ezlda_server_get_stats": (C.c_int, [P, C.POINTER(ServerStats)]),
@@ -450,6 +451,16 @@ class Server:
     def submit(self, rid: int, words: np.ndarray):
         w = np.ascontiguousarray(words, dtype=np.int32)
         self._chk(self._L.ezlda_server_submit(self._h, rid, _ptr(w), len(w)))
+
+    def submit_many(self, rids, doc_off, words):
+        """Queues len(rids) documents (words[doc_off[d]:doc_off[d+1]]) in one
+        call; all or none."""
+        ids = np.ascontiguousarray(rids, dtype=np.uint32)
+        off = np.ascontiguousarray(doc_off, dtype=np.int64)
+        w = np.ascontiguousarray(words, dtype=np.int32)
+        if len(off) != len(ids) + 1:
+            raise EzldaError(EINVAL, "doc_off needs len(rids) + 1 entries")
+        self._chk(self._L.ezlda_server_submit_many(self._h, len(ids), _ptr(ids), _ptr(off), _ptr(w)))
 
     def wait(self, rid: int, n: int, timeout_ms: int = -1):
         """(z, topic counts, latency ms) of a finished request."""

@@ -149,6 +149,12 @@ public:
     void submit(uint32_t rid, const std::vector<int32_t> &words) {
         chk(ezlda_server_submit(h_, rid, words.data(), (int32_t)words.size()));
     }
+    // Documents d = 0..ids.size()-1: words [doc_off[d], doc_off[d+1]); all or none.
+    void submit_many(const std::vector<uint32_t> &ids, const std::vector<int64_t> &doc_off,
+                     const std::vector<int32_t> &words) {
+        if (doc_off.size() != ids.size() + 1) throw Error(EZLDA_EINVAL, "doc_off needs ids.size() + 1 entries");
+        chk(ezlda_server_submit_many(h_, (int32_t)ids.size(), ids.data(), doc_off.data(), words.data()));
+    }
     // Topics and topic counts of a finished request; latency in ms.
     double wait(uint32_t rid, std::vector<uint16_t> *z, std::vector<int32_t> *counts, int32_t timeout_ms = -1) {
         double lat = 0.0;

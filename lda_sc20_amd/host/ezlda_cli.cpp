@@ -189,7 +189,11 @@ int cmd_foldin(const Opts &o) {
     const int D = req.docs();
     const int64_t *off = req.doc_off();
     const int32_t *w = req.word_ids();
-    for (int d = 0; d < D; ++d) srv.submit((uint32_t)d, std::vector<int32_t>(w + off[d], w + off[d + 1]));
+    {  // one batch: every document or none
+        std::vector<uint32_t> ids((size_t)D);
+        for (int d = 0; d < D; ++d) ids[d] = (uint32_t)d;
+        srv.submit_many(ids, std::vector<int64_t>(off, off + D + 1), std::vector<int32_t>(w, w + off[D]));
+    }
     srv.drain();
     std::FILE *f = o.has("out") ? std::fopen(o.str("out").c_str(), "w") : stdout;
     if (!f) throw ezlda::Error(EZLDA_EIO, "cannot open " + o.str("out"));

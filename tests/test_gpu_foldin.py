@@ -138,3 +138,27 @@ def test_server_survives_model_training_afterwards():
         z, c, _ = srv.wait(r + 1, int(off[r + 1] - off[r]), timeout_ms=30000)
         assert np.array_equal(z, zo[off[r]: off[r + 1]]) and np.array_equal(c, co[r])
     srv.close()
+
+
+def test_submit_many_all_or_none():
+    ctx, orc = _model(64)
+    lens = np.array([5, 0, 300, 64, 1, 77])
+    off, words = _docs(300, lens, seed=12)
+    rids = np.array([9, 3, 77, 1000, 5, 6], np.uint32)
+    zo, co = orc.foldin(off, words, rids, 6, 31)
+    srv = ezlda.Server(ctx, max_slots=4, sweeps=6, hbm_budget_bytes=1 << 20, seed=31)
+    bad = words.copy()
+    bad[off[3] + 2] = 300  # word id out of range in the fourth document
+    with pytest.raises(ezlda.EzldaError):
+        srv.submit_many(rids, off, bad)
+    with pytest.raises(ezlda.EzldaError):  # nothing was queued
+        srv.wait(int(rids[0]), int(lens[0]), timeout_ms=1000)
+    with pytest.raises(ezlda.EzldaError):  # an id twice in one batch
+        srv.submit_many(np.array([1, 2, 1], np.uint32), off[:4], words)
+    srv.submit_many(rids, off, words)
+    srv.drain()
+    for d in range(len(lens)):
+        z, c, _ = srv.wait(int(rids[d]), int(lens[d]), timeout_ms=30000)
+        assert np.array_equal(z, zo[off[d]: off[d + 1]]) and np.array_equal(c, co[d]), d
+    assert srv.stats().completed == len(lens)
+    srv.close()
