@@ -1378,7 +1378,10 @@ hipError_t launch_foldin_step(const FoldinBatch &fb, const uint4 *slots, const u
                               int V, int K, float alpha, const float *phi, const float *qs, const float *qacc,
                               const uint32_t *wpool, uint16_t *zpool, uint64_t seed, hipStream_t st) {
     if (nslots <= 0) return hipSuccess;
-    const unsigned gs = (unsigned)std::min(nslots, 1 << 16), gc = (unsigned)std::min(nslots, 1024);
+    // count and scatter: one workgroup per CU (fb.grid is 8 per CU), each
+    // flushing its LDS table of the most frequent words once for many slots
+    const unsigned gs = (unsigned)std::min(nslots, 1 << 16);
+    const unsigned gc = (unsigned)std::max(1, std::min(nslots, std::max(1, fb.grid / 8)));
     const unsigned nb = (unsigned)((V + 1023) / 1024);
     const size_t lds_rows = (size_t)((K + 1) >> 1) * sizeof(uint32_t);
     hipLaunchKernelGGL(k_fill_u32, dim3(grid_for((size_t)V, 256, 4096)), dim3(256), 0, st, fb.wcount, (size_t)V, 0u);
