@@ -81,7 +81,6 @@ struct FoldinBatch {
     uint32_t *tokw = nullptr;       // resident tokens: slot << 16 | position, by word (big words first)
     uint4 *items = nullptr;         // foldin_max_items(V, tokens) big items: (word, first, end, 0)
     uint32_t *ctr = nullptr;        // big items, big-word tokens, other tokens, 0
-    int32_t *counts_out = nullptr;  // per finishing document: K topic counts
     uint16_t *zout = nullptr;       // finishing documents' topics, packed
     int grid = 0;                   // persistent sampling workgroups
 };

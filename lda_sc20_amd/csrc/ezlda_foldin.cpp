@@ -57,8 +57,8 @@ struct Result {
     double latency_ms = 0.0;
 };
 
-// A document finishing with a step: where its counts and topics land in the
-// step's result copy.
+// A document finishing with a step: where its topics land in the step's result
+// copy (the host counts them).
 struct Finishing {
     uint32_t rid = 0, n = 0, fidx = 0, zoff = 0;
     double t_submit = 0.0;
@@ -69,9 +69,8 @@ struct Finishing {
 // event after its result copies, and the documents it finishes.
 struct StepRec {
     uint8_t *up_h = nullptr;
-    int32_t *cnt_h = nullptr;
     uint16_t *z_h = nullptr;
-    size_t cnt_cap = 0, z_cap = 0;
+    size_t z_cap = 0;
     hipEvent_t ev = nullptr;
     std::vector<Finishing> fins;
     bool busy = false;
@@ -230,7 +229,8 @@ bool publish(ezlda_server *s, StepRec &R) {
     for (size_t f = 0; f < R.fins.size(); ++f) {
         const Finishing &x = R.fins[f];
         res[f].z.assign(R.z_h + x.zoff, R.z_h + x.zoff + x.n);
-        res[f].counts.assign(R.cnt_h + (size_t)x.fidx * s->K, R.cnt_h + (size_t)(x.fidx + 1) * s->K);
+        res[f].counts.assign(s->K, 0);
+        for (const uint16_t t : res[f].z) res[f].counts[t]++;
     }
     std::lock_guard<std::mutex> lk(s->mu);
     const double t = now_ms();
@@ -349,8 +349,7 @@ void worker_main(ezlda_server *s) {
                     std::memcpy(R.up_h + L.words + (size_t)adm_rec[k].x * sizeof(uint32_t), adm[k].words.data(),
                                 adm[k].words.size() * sizeof(uint32_t));
             hipError_t e = hipSuccess;
-            if (!ensure_pinned(&R.cnt_h, &R.cnt_cap, (size_t)nfin * s->K) ||
-                !ensure_pinned(&R.z_h, &R.z_cap, (size_t)zsum + 1))
+            if (!ensure_pinned(&R.z_h, &R.z_cap, (size_t)zsum + 1))
                 e = hipErrorOutOfMemory;
             if (e == hipSuccess) e = hipMemcpyAsync(s->up_d, R.up_h, L.end, hipMemcpyHostToDevice, s->st);
             if (e == hipSuccess)
@@ -360,9 +359,6 @@ void worker_main(ezlda_server *s) {
                 e = ez::launch_foldin_step(s->fb, (const uint4 *)s->up_d, (const uint2 *)(s->up_d + L.fin), (int)ns,
                                            ntok, s->V, s->K, s->alpha, s->phi, s->qs, s->qacc, s->wpool, s->zpool,
                                            s->seed, s->st);
-            if (e == hipSuccess && nfin)
-                e = hipMemcpyAsync(R.cnt_h, s->fb.counts_out, (size_t)nfin * s->K * sizeof(int32_t),
-                                   hipMemcpyDeviceToHost, s->st);
             if (e == hipSuccess && zsum)
                 e = hipMemcpyAsync(R.z_h, s->fb.zout, (size_t)zsum * sizeof(uint16_t), hipMemcpyDeviceToHost, s->st);
             if (e == hipSuccess) e = hipEventRecord(R.ev, s->st);
@@ -391,8 +387,7 @@ bool alloc_batch(ezlda_server *s, ez::FoldinBatch *b, uint32_t tokens, int nslot
            salloc(s, &b->wcount, s->V) == EZLDA_OK && salloc(s, &b->wcursor, s->V) == EZLDA_OK &&
            salloc(s, &b->bsum, 3 * nb) == EZLDA_OK && salloc(s, &b->tokw, tokens) == EZLDA_OK &&
            salloc(s, &b->items, ez::foldin_max_items(s->V, tokens)) == EZLDA_OK &&
-           salloc(s, &b->ctr, 4) == EZLDA_OK && salloc(s, &b->counts_out, (size_t)nslots * s->K) == EZLDA_OK &&
-           salloc(s, &b->zout, tokens) == EZLDA_OK;
+           salloc(s, &b->ctr, 4) == EZLDA_OK && salloc(s, &b->zout, tokens) == EZLDA_OK;
 }
 
 }  // namespace
@@ -469,7 +464,6 @@ void ezlda_server_destroy(ezlda_server *s) {
     for (void *p : s->allocs) (void)hipFree(p);
     for (StepRec &R : s->rec) {
         if (R.up_h) (void)hipHostFree(R.up_h);
-        if (R.cnt_h) (void)hipHostFree(R.cnt_h);
         if (R.z_h) (void)hipHostFree(R.z_h);
         if (R.ev) (void)hipEventDestroy(R.ev);
     }
@@ -644,9 +638,13 @@ int ezlda_foldin_plain(ezlda_server *s, uint32_t request_id, const int32_t *word
                                          (uint32_t)n, s->V, s->K, s->alpha, s->phi, s->qs, s->qacc, s->pw, s->pz, s->seed, s->pst));
         SV_HIP(s, hipStreamSynchronize(s->pst));
     }
-    if (n && z_out) SV_HIP(s, hipMemcpy(z_out, s->pfb.zout, n * sizeof(uint16_t), hipMemcpyDeviceToHost));
-    if (counts_out)
-        SV_HIP(s, hipMemcpy(counts_out, s->pfb.counts_out, s->K * sizeof(int32_t), hipMemcpyDeviceToHost));
+    std::vector<uint16_t> z((size_t)n);
+    if (n) SV_HIP(s, hipMemcpy(z.data(), s->pfb.zout, n * sizeof(uint16_t), hipMemcpyDeviceToHost));
+    if (n && z_out) std::memcpy(z_out, z.data(), n * sizeof(uint16_t));
+    if (counts_out) {
+        std::memset(counts_out, 0, sizeof(int32_t) * (size_t)s->K);
+        for (const uint16_t t : z) counts_out[t]++;
+    }
     return EZLDA_OK;
 }
 

@@ -1329,27 +1329,16 @@ __global__ void __launch_bounds__(256) k_fi_sample(const uint4 *__restrict__ slo
 }
 
 // Documents that just drew their last sweep (fin[j] = (output index, topic
-// offset), output index ~0 for the others): topic counts into row `index`
-// of counts_out, topics into zout at the offset.
+// offset), output index ~0 for the others): their topics packed into zout at
+// the offset, for one copy to the host (which counts them).
 __global__ void __launch_bounds__(256) k_fi_final(const uint4 *__restrict__ slots, const uint2 *__restrict__ fin,
-                                                  int nslots, int K, const uint16_t *__restrict__ zpool,
-                                                  int32_t *__restrict__ counts_out, uint16_t *__restrict__ zout) {
-    extern __shared__ uint32_t shf2[];
-    const int tid = threadIdx.x, bd = blockDim.x, KH = (K + 1) >> 1;
+                                                  int nslots, const uint16_t *__restrict__ zpool,
+                                                  uint16_t *__restrict__ zout) {
     for (int j = blockIdx.x; j < nslots; j += gridDim.x) {
         const uint2 f = fin[j];
         if (f.x == 0xFFFFFFFFu) continue;  // uniform over the workgroup
         const uint4 sl = slots[j];
-        for (int k = tid; k < KH; k += bd) shf2[k] = 0;
-        __syncthreads();
-        for_doc_tokens(zpool + sl.y, sl.z, [&](uint16_t t, uint32_t i) {
-            h16_add(shf2, t);
-            zout[f.y + i] = t;
-        });
-        __syncthreads();
-        int32_t *co = counts_out + (size_t)f.x * K;
-        for (int k = tid; k < K; k += bd) co[k] = (int32_t)h16_get(shf2, (uint32_t)k);
-        __syncthreads();
+        for_doc_tokens(zpool + sl.y, sl.z, [&](uint16_t t, uint32_t i) { zout[f.y + i] = t; });
     }
 }
 
@@ -1397,8 +1386,7 @@ hipError_t launch_foldin_step(const FoldinBatch &fb, const uint4 *slots, const u
     const unsigned gsmp = (unsigned)std::max<int64_t>(1, std::min<int64_t>(fb.grid, ((int64_t)ntok + 255) / 256));
     hipLaunchKernelGGL(k_fi_sample, dim3(gsmp), dim3(256), foldin_sample_lds(K), st, slots, K, alpha, phi, qs, qacc,
                        wpool, fb.rows, fb.row_nnz, fb.tokw, fb.items, fb.ctr, zpool, seed);
-    hipLaunchKernelGGL(k_fi_final, dim3(gs), dim3(256), lds_rows, st, slots, fin, nslots, K, zpool, fb.counts_out,
-                       fb.zout);
+    hipLaunchKernelGGL(k_fi_final, dim3(gs), dim3(256), 0, st, slots, fin, nslots, zpool, fb.zout);
     return hipGetLastError();
 }
 
