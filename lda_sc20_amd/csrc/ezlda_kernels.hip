@@ -281,6 +281,8 @@ hipError_t launch_z_to_doc(const DevState &s, hipStream_t st) {
 // word's top-2 topics (the re-pack of :125-161), and compacts the row in
 // ascending topic order (:183-238).  A row starts on a 128-byte boundary and
 // is zero-padded to a multiple of 32 entries.
+static constexpr int kDtU = 8;  // slots per lane per chunk in k_dt_build
+
 template <bool COMPACT>
 __global__ void k_dt_build(int D, int K, int VD, uint32_t tb, const uint32_t *__restrict__ soff,
                            const uint32_t *__restrict__ slot_wrd, const uint32_t *__restrict__ slot_tok,
@@ -297,17 +299,71 @@ __global__ void k_dt_build(int D, int K, int VD, uint32_t tb, const uint32_t *__
     wave_lds_sync();
     for (int q = blockIdx.x * nwv + wv; q < D; q += gridDim.x * nwv) {
         const uint32_t b = soff[q], e = soff[q + 1];
-        for (uint32_t i = b + lane; i < e; i += 64) {
-            const uint32_t t = slot_wrd[i] < (uint32_t)VD ? z_doc[i] : z[slot_tok[i]];
-            h16_add(h, t);
-            atomicOr(&bm[t >> 5], 1u << (t & 31u));
+        // The doc in chunks of kDtU slots per lane, every load of a chunk
+        // issued before the first is used.  A doc of one chunk keeps its
+        // words' top topics in registers for the aux pass.
+        const bool one = e - b <= 64u * kDtU;
+        uint32_t tp[kDtU];  // top-1 | top-2 << 16 of each slot's word (one-chunk docs)
+        for (uint32_t c = b; c < e; c += 64u * kDtU) {
+            uint32_t w[kDtU], t[kDtU];
+#pragma unroll
+            for (int u = 0; u < kDtU; ++u) {
+                const uint32_t i = c + lane + 64u * u;
+                w[u] = i < e ? slot_wrd[i] : 0u;
+                t[u] = i < e ? z_doc[i] : 0u;
+            }
+#pragma unroll
+            for (int u = 0; u < kDtU; ++u) {
+                const uint32_t i = c + lane + 64u * u;
+                if (i < e && w[u] >= (uint32_t)VD) t[u] = z[slot_tok[i]];
+            }
+            if (one) {
+#pragma unroll
+                for (int u = 0; u < kDtU; ++u) {
+                    const ushort4 x = wtop[w[u]];
+                    tp[u] = x.x | ((uint32_t)x.y << 16);
+                }
+            }
+#pragma unroll
+            for (int u = 0; u < kDtU; ++u) {
+                if (c + lane + 64u * u < e) {
+                    h16_add(h, t[u]);
+                    atomicOr(&bm[t[u] >> 5], 1u << (t[u] & 31u));
+                }
+            }
         }
         wave_lds_sync();
-        for (uint32_t i = b + lane; i < e; i += 64) {
-            const ushort4 tp = wtop[slot_wrd[i]];
-            const uint32_t n1 = h16_get(h, tp.x);
-            const uint32_t n2 = tp.y != kNone ? h16_get(h, tp.y) : 0u;
-            aux_doc[i] = n1 | (n2 << 16);
+        if (one) {
+#pragma unroll
+            for (int u = 0; u < kDtU; ++u) {
+                const uint32_t i = b + lane + 64u * u;
+                if (i < e) {
+                    const uint32_t k2 = tp[u] >> 16;
+                    aux_doc[i] = h16_get(h, tp[u] & 0xFFFFu) | ((k2 != kNone ? h16_get(h, k2) : 0u) << 16);
+                }
+            }
+        } else {
+            for (uint32_t c = b; c < e; c += 64u * kDtU) {
+                uint32_t x[kDtU];
+#pragma unroll
+                for (int u = 0; u < kDtU; ++u) {
+                    const uint32_t i = c + lane + 64u * u;
+                    x[u] = i < e ? slot_wrd[i] : 0u;
+                }
+#pragma unroll
+                for (int u = 0; u < kDtU; ++u) {
+                    const ushort4 y = wtop[x[u]];
+                    x[u] = y.x | ((uint32_t)y.y << 16);
+                }
+#pragma unroll
+                for (int u = 0; u < kDtU; ++u) {
+                    const uint32_t i = c + lane + 64u * u;
+                    if (i < e) {
+                        const uint32_t k2 = x[u] >> 16;
+                        aux_doc[i] = h16_get(h, x[u] & 0xFFFFu) | ((k2 != kNone ? h16_get(h, k2) : 0u) << 16);
+                    }
+                }
+            }
         }
         wave_lds_sync();
         // Compaction over the bitmap: lane j owns topics [32j, 32j + 32) of
