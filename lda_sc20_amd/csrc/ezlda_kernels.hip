@@ -655,35 +655,40 @@ __device__ __forceinline__ void sample_item(const SampleArgs &A, uint32_t item, 
                     b1 = 0;  // no running sum exceeds t2
                 }
                 bool found = false;
-                for (uint32_t g = b0 * kBlkU4; g < b1 * kBlkU4 && !found; ++g) {
-                    const uint4 e = rp[g];
-                    if (COMPACT) {
-                        const uint32_t w[4] = {e.x, e.y, e.z, e.w};
+                // the block(s) from the checkpoint, all four loads at once; the
+                // first running sum above t2 (running sums never decrease)
+                for (uint32_t gb = b0; gb < b1 && !found; ++gb) {
+                    const RowBlock blk = load_block(rp + kBlkU4 * gb);
 #pragma unroll
-                        for (int q = 0; q < 4; ++q) {
-                            if (found) break;
-                            const float ra = __builtin_fmaf(c16_count(w[q], tb), phi_c16(w[q], tb), run);
-                            const float rb =
-                                __builtin_fmaf((float)(w[q] >> (16u + tb)), phi_c16(w[q] >> 16, tb), ra);
-                            if (ra > t2) {
-                                newz = c16_topic(w[q], tb);
-                                found = true;
-                            } else if (rb > t2) {
-                                newz = c16_topic(w[q] >> 16, tb);
+                    for (int gi = 0; gi < kBlkU4; ++gi) {
+                        const uint4 e = blk.e[gi];
+                        if (COMPACT) {
+                            const uint32_t w[4] = {e.x, e.y, e.z, e.w};
+#pragma unroll
+                            for (int q = 0; q < 4; ++q) {
+                                const float ra = __builtin_fmaf(c16_count(w[q], tb), phi_c16(w[q], tb), run);
+                                const float rb =
+                                    __builtin_fmaf((float)(w[q] >> (16u + tb)), phi_c16(w[q] >> 16, tb), ra);
+                                if (!found && ra > t2) {
+                                    newz = c16_topic(w[q], tb);
+                                    found = true;
+                                } else if (!found && rb > t2) {
+                                    newz = c16_topic(w[q] >> 16, tb);
+                                    found = true;
+                                }
+                                run = rb;
+                            }
+                        } else if (!found) {
+                            const float r0 = __builtin_fmaf(dt_count(e.x), phi_at0(e.x), run);
+                            const float r1 = __builtin_fmaf(dt_count(e.y), phi_at0(e.y), r0);
+                            const float r2 = __builtin_fmaf(dt_count(e.z), phi_at0(e.z), r1);
+                            const float r3 = __builtin_fmaf(dt_count(e.w), phi_at0(e.w), r2);
+                            if (r3 > t2) {
+                                newz = dt_topic(r0 > t2 ? e.x : r1 > t2 ? e.y : r2 > t2 ? e.z : e.w);
                                 found = true;
                             }
-                            run = rb;
+                            run = r3;
                         }
-                    } else {
-                        const float r0 = __builtin_fmaf(dt_count(e.x), phi_at0(e.x), run);
-                        const float r1 = __builtin_fmaf(dt_count(e.y), phi_at0(e.y), r0);
-                        const float r2 = __builtin_fmaf(dt_count(e.z), phi_at0(e.z), r1);
-                        const float r3 = __builtin_fmaf(dt_count(e.w), phi_at0(e.w), r2);
-                        if (r3 > t2) {
-                            newz = dt_topic(r0 > t2 ? e.x : r1 > t2 ? e.y : r2 > t2 ? e.z : e.w);
-                            found = true;
-                        }
-                        run = r3;
                     }
                 }
                 if (!found) {  // rounding left every running sum at or below t2
