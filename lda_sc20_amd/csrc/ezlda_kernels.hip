@@ -539,21 +539,45 @@ __device__ __forceinline__ void sample_item(const SampleArgs &A, uint32_t item, 
         phi[tp.x] = 0.0f;
     }
     __syncthreads();
+    // Q running sums inside each 64-topic slab, left to right (phi is 0 past
+    // K, which leaves the sum unchanged): the slab's 64 values are loaded at
+    // once, then summed in order
     for (int j = tid; j < NS; j += blockDim.x) {
-        const int k0 = j * 64, ke = min(k0 + 64, K);
+        const float4 *p4 = reinterpret_cast<const float4 *>(phi + j * 64);
+        float4 *q4 = reinterpret_cast<float4 *>(qs + j * 64);
+        float4 v[16];
+#pragma unroll
+        for (int i = 0; i < 16; ++i) v[i] = p4[i];
         float run = 0.0f;
-        for (int k = k0; k < ke; ++k) {
-            run = run + alpha * phi[k];
-            qs[k] = run;
+#pragma unroll
+        for (int i = 0; i < 16; ++i) {
+            run = run + alpha * v[i].x;
+            v[i].x = run;
+            run = run + alpha * v[i].y;
+            v[i].y = run;
+            run = run + alpha * v[i].z;
+            v[i].z = run;
+            run = run + alpha * v[i].w;
+            v[i].w = run;
         }
+#pragma unroll
+        for (int i = 0; i < 16; ++i) q4[i] = v[i];
         qt[j] = run;
     }
     __syncthreads();
     if (tid == 0) {
         float acc = 0.0f;
-        for (int j = 0; j < NS; ++j) {
-            acc = acc + qt[j];
-            qacc[j] = acc;
+        for (int j0 = 0; j0 < NS; j0 += 8) {
+            float v[8];
+#pragma unroll
+            for (int i = 0; i < 8; ++i) v[i] = j0 + i < NS ? qt[j0 + i] : 0.0f;
+#pragma unroll
+            for (int i = 0; i < 8; ++i) {
+                if (j0 + i < NS) {
+                    acc = acc + v[i];
+                    qacc[j0 + i] = acc;
+                }
+            }
         }
     }
     __syncthreads();
