@@ -51,6 +51,8 @@ struct DevState {
     double *ll_out = nullptr;       // 1
     int32_t block_dim = 128;
     int32_t grid_dim = 0;           // 0 = one workgroup per sampler item
+    int32_t n_cu = 256;             // compute units of the device
+    int32_t lds_cu = 160 * 1024;    // LDS bytes per compute unit
 };
 
 hipError_t launch_zero_counts(const DevState &s, hipStream_t st);

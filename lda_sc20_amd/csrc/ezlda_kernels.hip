@@ -57,6 +57,21 @@ static size_t wave_hist_bytes(int K) { return (size_t)((K + 1) >> 1) * sizeof(ui
 // DT rebuild: 16-bit counts plus a topic bitmap per wavefront
 static size_t dt_wave_bytes(int K) { return wave_hist_bytes(K) + (size_t)((K + 31) >> 5) * sizeof(uint32_t); }
 
+// Launch of a kernel whose wavefronts each own wb bytes of LDS (cleared
+// when the wavefront starts) and stride over n units (words, docs).  When
+// at least 16 wavefronts' LDS fits on a CU: workgroups of four wavefronts,
+// one unit each (up to cap workgroups), which the dispatcher balances.
+// Otherwise (large K, where clearing the LDS costs more than a unit):
+// persistent one-wavefront workgroups, as many as the CUs' LDS holds.
+struct WaveGrid {
+    int grid, nwv;
+};
+static WaveGrid wave_grid(const DevState &s, size_t n, size_t wb, int cap) {
+    const size_t fit = wb ? (size_t)s.lds_cu / wb : 32;
+    if (fit >= 16) return {grid_for(n, 4, cap), 4};
+    return {grid_for(n, 1, s.n_cu * (int)std::max<size_t>(1, fit)), 1};
+}
+
 // A DT entry is (topic << 2) | (count << 16): its low half is the byte
 // offset of phi[topic] in a float array, so a gather needs one AND.
 __device__ __forceinline__ uint32_t dt_topic(uint32_t e) { return (e & 0xFFFFu) >> 2; }
@@ -1535,8 +1550,8 @@ hipError_t launch_accumulate_wt(const DevState &s, hipStream_t st) {
         hipLaunchKernelGGL(k_wt_head, dim3(s.n_wt_segs), dim3(256), ldsK, st, s.z, s.wt_segs, s.K, s.wt_head);
     const int ntail = s.V - s.VD;
     if (ntail > 0) {
-        const int nwv = 4;
-        hipLaunchKernelGGL(k_wt_tail, dim3(grid_for(ntail, nwv, 1 << 16)), dim3(64 * nwv), nwv * wave_hist_bytes(s.K), st,
+        const WaveGrid g = wave_grid(s, (size_t)ntail, wave_hist_bytes(s.K), 1 << 16);
+        hipLaunchKernelGGL(k_wt_tail, dim3(g.grid), dim3(64 * g.nwv), g.nwv * wave_hist_bytes(s.K), st,
                            s.z, s.word_off, s.VD, s.V, s.K, s.tail_off, s.wt_tail, s.tail_nnz);
     }
     return hipGetLastError();
@@ -1544,22 +1559,20 @@ hipError_t launch_accumulate_wt(const DevState &s, hipStream_t st) {
 
 hipError_t launch_topk(const DevState &s, hipStream_t st) {
     hipLaunchKernelGGL(k_inv, dim3(grid_for(s.K, 256, 1 << 16)), dim3(256), 0, st, s.nk, s.K, s.vbeta, s.inv);
-    const int nwv = 4;
-    hipLaunchKernelGGL(k_word_top, dim3(grid_for(s.V, nwv, 1 << 16)), dim3(64 * nwv), nwv * wave_hist_bytes(s.K), st,
+    const WaveGrid g = wave_grid(s, (size_t)s.V, wave_hist_bytes(s.K), 1 << 16);
+    hipLaunchKernelGGL(k_word_top, dim3(g.grid), dim3(64 * g.nwv), g.nwv * wave_hist_bytes(s.K), st,
                        s.V, s.VD, s.K, s.beta, s.wt_head, s.tail_off, s.wt_tail, s.tail_nnz, s.inv, s.wtop);
     return hipGetLastError();
 }
 
 hipError_t launch_build_dt(const DevState &s, hipStream_t st) {
     if (s.D == 0) return hipSuccess;
-    const int nwv = 4;
+    const WaveGrid g = wave_grid(s, (size_t)s.D, dt_wave_bytes(s.K), 1 << 18);
     if (s.compact)
-        hipLaunchKernelGGL(k_dt_build<true>, dim3(grid_for(s.D, nwv, 1 << 18)), dim3(64 * nwv),
-                           nwv * dt_wave_bytes(s.K), st, s.D, s.K, s.VD, s.topic_bits, s.soff, s.slot_wrd, s.slot_tok, s.z,
+        hipLaunchKernelGGL(k_dt_build<true>, dim3(g.grid), dim3(64 * g.nwv), g.nwv * dt_wave_bytes(s.K), st, s.D, s.K, s.VD, s.topic_bits, s.soff, s.slot_wrd, s.slot_tok, s.z,
                            s.z_doc, s.wtop, s.aux_doc, s.doc_info, s.dt);
     else
-        hipLaunchKernelGGL(k_dt_build<false>, dim3(grid_for(s.D, nwv, 1 << 18)), dim3(64 * nwv),
-                           nwv * dt_wave_bytes(s.K), st, s.D, s.K, s.VD, s.topic_bits, s.soff, s.slot_wrd, s.slot_tok, s.z,
+        hipLaunchKernelGGL(k_dt_build<false>, dim3(g.grid), dim3(64 * g.nwv), g.nwv * dt_wave_bytes(s.K), st, s.D, s.K, s.VD, s.topic_bits, s.soff, s.slot_wrd, s.slot_tok, s.z,
                            s.z_doc, s.wtop, s.aux_doc, s.doc_info, s.dt);
     return hipGetLastError();
 }

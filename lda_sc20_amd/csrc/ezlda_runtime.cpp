@@ -74,6 +74,15 @@ int device_cus() {
     return n;
 }
 
+// LDS bytes per compute unit of the current device (160 KiB on MI355X).
+int device_lds_per_cu() {
+    int dev = 0, n = 0;
+    if (hipGetDevice(&dev) != hipSuccess ||
+        hipDeviceGetAttribute(&n, hipDeviceAttributeMaxSharedMemoryPerMultiprocessor, dev) != hipSuccess || n < 1)
+        n = 160 * 1024;
+    return n;
+}
+
 // Validates the layout arguments of `a` (those build_layout reads) and
 // fills their defaults; returns an error message or "".
 std::string check_layout_args(ezlda_args &a) {
@@ -373,6 +382,8 @@ int build_layout(ezlda_ctx *c, const std::vector<uint32_t> &tok_doc, const std::
     s.n_queues = c->doc_tiles > 0 ? std::min(c->n_xcd, 8) : 1;
     s.block_dim = c->a.block_dim;
     s.grid_dim = c->a.grid_dim;
+    s.n_cu = device_cus();
+    s.lds_cu = device_lds_per_cu();
     c->tail_cap = tail_off.back();
     c->counts_fresh = false;
     // the sampler keeps z_doc only for head words: bring every slot up to date
@@ -552,6 +563,8 @@ int load_chunked(ezlda_ctx *c, const std::vector<uint32_t> &tok_doc, const std::
     s.n_runs = runs_total;
     s.block_dim = c->a.block_dim;
     s.grid_dim = c->a.grid_dim;
+    s.n_cu = device_cus();
+    s.lds_cu = device_lds_per_cu();
     c->dt_cap = (int64_t)dt_total;
     c->tail_cap = 0;
     c->doc_tiles = 0;
