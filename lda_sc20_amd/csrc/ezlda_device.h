@@ -37,6 +37,7 @@ struct DevState {
     int32_t *nk = nullptr;          // K
     float *inv = nullptr;           // K: 1/(n_k + V beta)
     ushort4 *wtop = nullptr;        // V: k1, k2, k3 (0xFFFF = none)
+    uint32_t *zero_order = nullptr; // K: topics by beta * inv, largest first (tail words' zero counts)
     // work items
     uint4 *wt_segs = nullptr;       // head-word histogram segments (w, beg, end)
     int32_t n_wt_segs = 0;

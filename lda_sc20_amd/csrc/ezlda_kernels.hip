@@ -114,21 +114,38 @@ __global__ void k_inc_u32(uint32_t *p) { *p = *p + 1u; }
 // ------------------------------------------------------- n_k histogram
 // n_k = column sums of WT (WTDen_Sum_Update_Kernel utility.cu:1104-1126 plus
 // the tail's atomicAdd in sparseMatrixAdd :1132-1204): one LDS histogram of
-// z per workgroup, flushed with one atomic per nonzero topic.
+// z per workgroup, flushed with one atomic per nonzero topic.  Each lane
+// reads eight tokens at once and adds each run of equal topics among them
+// with one LDS atomic (word-major neighbours often share a topic, and equal
+// topics in one instruction contend for one LDS address).
 __global__ void k_topic_hist(const uint16_t *__restrict__ z, uint32_t N, int K, int32_t *__restrict__ nk) {
     extern __shared__ uint32_t sh[];
     for (int k = threadIdx.x; k < K; k += blockDim.x) sh[k] = 0;
     __syncthreads();
-    const size_t stride = (size_t)gridDim.x * blockDim.x * 2;
-    const uint32_t *z2 = reinterpret_cast<const uint32_t *>(z);
-    for (size_t i = ((size_t)blockIdx.x * blockDim.x + threadIdx.x) * 2; i < N; i += stride) {
-        if (i + 1 < N) {
-            uint32_t p = z2[i >> 1];
-            atomicAdd(&sh[p & 0xFFFFu], 1u);
-            atomicAdd(&sh[p >> 16], 1u);
-        } else {
-            atomicAdd(&sh[z[i]], 1u);
+    // tokens before the first 16-byte boundary, then whole groups of eight
+    const size_t mis = ((16u - ((uintptr_t)z & 15u)) & 15u) >> 1;
+    const size_t head = mis < N ? mis : N;
+    const size_t n8 = (N - head) >> 3;
+    const uint4 *z8 = reinterpret_cast<const uint4 *>(z + head);
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n8; i += (size_t)gridDim.x * blockDim.x) {
+        const uint4 v = z8[i];
+        const uint32_t t[8] = {v.x & 0xFFFFu, v.x >> 16, v.y & 0xFFFFu, v.y >> 16,
+                               v.z & 0xFFFFu, v.z >> 16, v.w & 0xFFFFu, v.w >> 16};
+        uint32_t c = 1;
+#pragma unroll
+        for (int j = 0; j < 7; ++j) {
+            if (t[j + 1] != t[j]) {
+                atomicAdd(&sh[t[j]], c);
+                c = 1;
+            } else {
+                ++c;
+            }
         }
+        atomicAdd(&sh[t[7]], c);
+    }
+    if (blockIdx.x == 0) {
+        for (size_t i = threadIdx.x; i < head; i += blockDim.x) atomicAdd(&sh[z[i]], 1u);
+        for (size_t i = head + (n8 << 3) + threadIdx.x; i < N; i += blockDim.x) atomicAdd(&sh[z[i]], 1u);
     }
     __syncthreads();
     for (int k = threadIdx.x; k < K; k += blockDim.x) {
@@ -225,6 +242,30 @@ __device__ __forceinline__ void top3_insert(Top3 &t, float v, uint32_t k) {
     }
 }
 
+// Topics by beta * inv[k] (the value of a topic a word has no count of),
+// largest first, ties to the lower topic.  A workgroup of four wavefronts
+// ranks 64 topics: lane i's topic is compared with a quarter of all topics
+// by each wavefront, and the four counts are added in LDS.
+__global__ void __launch_bounds__(256) k_zero_order(const float *__restrict__ inv, int K, float beta,
+                                                    uint32_t *__restrict__ zero_order) {
+    extern __shared__ float sv[];  // K values, then 64 ranks
+    uint32_t *rank = reinterpret_cast<uint32_t *>(sv + K);
+    for (int k = threadIdx.x; k < K; k += blockDim.x) sv[k] = ((float)0u + beta) * inv[k];
+    if (threadIdx.x < 64) rank[threadIdx.x] = 0;
+    __syncthreads();
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    const int k = blockIdx.x * 64 + lane;
+    const int q = (K + 3) >> 2, j0 = wv * q, j1 = min(K, j0 + q);
+    if (k < K) {
+        const float v = sv[k];
+        uint32_t r = 0;
+        for (int j = j0; j < j1; ++j) r += better(sv[j], (uint32_t)j, v, (uint32_t)k) ? 1u : 0u;
+        atomicAdd(&rank[lane], r);
+    }
+    __syncthreads();
+    if (wv == 0 && k < K) zero_order[rank[lane]] = (uint32_t)k;
+}
+
 // Per word the three largest phi_wk, ties to the lower topic
 // (MaxTopicDense/Sparse_Update_Kernel, utility.cu:305-678, :679-1062).  One
 // wavefront per word; a tail word's counts are scattered into a 16-bit LDS
@@ -233,7 +274,7 @@ __device__ __forceinline__ void top3_insert(Top3 &t, float v, uint32_t k) {
 __global__ void k_word_top(int V, int VD, int K, float beta, const int32_t *__restrict__ wt_head,
                            const uint32_t *__restrict__ tail_off, const uint32_t *__restrict__ wt_tail,
                            const uint32_t *__restrict__ tail_nnz, const float *__restrict__ inv,
-                           ushort4 *__restrict__ wtop) {
+                           const uint32_t *__restrict__ zero_order, ushort4 *__restrict__ wtop) {
     extern __shared__ uint32_t sh[];
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6, nwv = blockDim.x >> 6;
     const int KH = (K + 1) >> 1;
@@ -246,15 +287,28 @@ __global__ void k_word_top(int V, int VD, int K, float beta, const int32_t *__re
             const int32_t *row = wt_head + (size_t)w * K;
             for (int k = lane; k < K; k += 64) top3_insert(t, ((float)row[k] + beta) * inv[k], (uint32_t)k);
         } else {
+            // the word's topics with a count, then those without one: their
+            // value is beta * inv[k], so the first three of zero_order that
+            // have no count are the best of them
             const int tw = w - VD;
             const uint32_t b = tail_off[tw], n = tail_nnz[tw];
             for (uint32_t i = lane; i < n; i += 64) {
                 const uint32_t e = wt_tail[b + i];
-                atomicAdd(&h[(e & 0xFFFFu) >> 1], (e >> 16) << ((e & 1u) << 4));
+                const uint32_t k = e & 0xFFFFu;
+                atomicAdd(&h[k >> 1], (e >> 16) << ((k & 1u) << 4));
+                top3_insert(t, ((float)(e >> 16) + beta) * inv[k], k);
             }
             wave_lds_sync();
-            for (int k = lane; k < K; k += 64)
-                top3_insert(t, ((float)h16_get(h, (uint32_t)k) + beta) * inv[k], (uint32_t)k);
+            uint32_t got = 0;  // wavefront-uniform
+            for (int j0 = 0; j0 < K && got < 3u; j0 += 64) {
+                const int j = j0 + lane;
+                const uint32_t k = j < K ? zero_order[j] : 0u;
+                const bool free_k = j < K && h16_get(h, k) == 0u;
+                const uint64_t m = __ballot(free_k);
+                if (free_k && got + (uint32_t)__popcll(m & lanemask_lt(lane)) < 3u)
+                    top3_insert(t, ((float)0u + beta) * inv[k], k);
+                got += (uint32_t)__popcll(m);
+            }
             wave_lds_sync();
             for (uint32_t i = lane; i < n; i += 64) h[(wt_tail[b + i] & 0xFFFFu) >> 1] = 0;
             wave_lds_sync();
@@ -1559,9 +1613,13 @@ hipError_t launch_accumulate_wt(const DevState &s, hipStream_t st) {
 
 hipError_t launch_topk(const DevState &s, hipStream_t st) {
     hipLaunchKernelGGL(k_inv, dim3(grid_for(s.K, 256, 1 << 16)), dim3(256), 0, st, s.nk, s.K, s.vbeta, s.inv);
+    if (s.V > s.VD)
+        hipLaunchKernelGGL(k_zero_order, dim3(grid_for(s.K, 64, 1 << 16)), dim3(256),
+                           (size_t)s.K * sizeof(float) + 64 * sizeof(uint32_t), st, s.inv, s.K, s.beta, s.zero_order);
     const WaveGrid g = wave_grid(s, (size_t)s.V, wave_hist_bytes(s.K), 1 << 16);
     hipLaunchKernelGGL(k_word_top, dim3(g.grid), dim3(64 * g.nwv), g.nwv * wave_hist_bytes(s.K), st,
-                       s.V, s.VD, s.K, s.beta, s.wt_head, s.tail_off, s.wt_tail, s.tail_nnz, s.inv, s.wtop);
+                       s.V, s.VD, s.K, s.beta, s.wt_head, s.tail_off, s.wt_tail, s.tail_nnz, s.inv, s.zero_order,
+                       s.wtop);
     return hipGetLastError();
 }
 

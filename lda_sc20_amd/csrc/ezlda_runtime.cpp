@@ -550,6 +550,7 @@ int load_chunked(ezlda_ctx *c, const std::vector<uint32_t> &tok_doc, const std::
     if ((r = dalloc(c, &c->nk_next, K))) return r;
     if ((r = dalloc(c, &s.inv, K))) return r;
     if ((r = dalloc(c, &s.wtop, V))) return r;
+    if ((r = dalloc(c, &s.zero_order, K))) return r;
     if ((r = dalloc(c, &s.tail_off, 1))) return r;
     EZ_TRY_HIP(c, hipMemset(s.tail_off, 0, sizeof(uint32_t)));
     if ((r = dalloc(c, &s.counters, 16))) return r;
@@ -965,6 +966,7 @@ int ezlda_load_docs(ezlda_ctx *c, int32_t V, int32_t D, const int64_t *doc_off, 
     if ((r = dalloc(c, &s.nk, K))) return r;
     if ((r = dalloc(c, &s.inv, K))) return r;
     if ((r = dalloc(c, &s.wtop, V))) return r;
+    if ((r = dalloc(c, &s.zero_order, K))) return r;
     if ((r = dalloc(c, &s.counters, 16))) return r;
     if ((r = dalloc(c, &s.stats, 2))) return r;
     if ((r = dalloc(c, &s.ll_out, 1))) return r;
