@@ -112,22 +112,19 @@ __global__ void k_set_u32(uint32_t *p, uint32_t v) { *p = v; }
 __global__ void k_inc_u32(uint32_t *p) { *p = *p + 1u; }
 
 // ------------------------------------------------------- n_k histogram
-// n_k = column sums of WT (WTDen_Sum_Update_Kernel utility.cu:1104-1126 plus
-// the tail's atomicAdd in sparseMatrixAdd :1132-1204): one LDS histogram of
-// z per workgroup, flushed with one atomic per nonzero topic.  Each lane
-// reads eight tokens at once and adds each run of equal topics among them
-// with one LDS atomic (word-major neighbours often share a topic, and equal
-// topics in one instruction contend for one LDS address).
-__global__ void k_topic_hist(const uint16_t *__restrict__ z, uint32_t N, int K, int32_t *__restrict__ nk) {
-    extern __shared__ uint32_t sh[];
-    for (int k = threadIdx.x; k < K; k += blockDim.x) sh[k] = 0;
-    __syncthreads();
-    // tokens before the first 16-byte boundary, then whole groups of eight
-    const size_t mis = ((16u - ((uintptr_t)z & 15u)) & 15u) >> 1;
-    const size_t head = mis < N ? mis : N;
-    const size_t n8 = (N - head) >> 3;
-    const uint4 *z8 = reinterpret_cast<const uint4 *>(z + head);
-    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n8; i += (size_t)gridDim.x * blockDim.x) {
+// Adds the topics of tokens [0, n) of zs to the LDS histogram sh: lanes take
+// groups of eight tokens (one 16-byte load) i0, i0 + stride, ..., and add
+// each run of equal neighbouring topics among them with one atomic (word-
+// major neighbours often share a topic, and equal topics in one LDS
+// instruction contend for one address).  The lane with i0 == 0 also adds the
+// tokens before the first 16-byte boundary and after the last whole group.
+__device__ __forceinline__ void hist_add_tokens(uint32_t *sh, const uint16_t *__restrict__ zs, size_t n, size_t i0,
+                                                size_t stride) {
+    const size_t mis = ((16u - ((uintptr_t)zs & 15u)) & 15u) >> 1;
+    const size_t head = mis < n ? mis : n;
+    const size_t n8 = (n - head) >> 3;
+    const uint4 *z8 = reinterpret_cast<const uint4 *>(zs + head);
+    for (size_t i = i0; i < n8; i += stride) {
         const uint4 v = z8[i];
         const uint32_t t[8] = {v.x & 0xFFFFu, v.x >> 16, v.y & 0xFFFFu, v.y >> 16,
                                v.z & 0xFFFFu, v.z >> 16, v.w & 0xFFFFu, v.w >> 16};
@@ -143,10 +140,20 @@ __global__ void k_topic_hist(const uint16_t *__restrict__ z, uint32_t N, int K, 
         }
         atomicAdd(&sh[t[7]], c);
     }
-    if (blockIdx.x == 0) {
-        for (size_t i = threadIdx.x; i < head; i += blockDim.x) atomicAdd(&sh[z[i]], 1u);
-        for (size_t i = head + (n8 << 3) + threadIdx.x; i < N; i += blockDim.x) atomicAdd(&sh[z[i]], 1u);
+    if (i0 == 0) {
+        for (size_t i = 0; i < head; ++i) atomicAdd(&sh[zs[i]], 1u);
+        for (size_t i = head + (n8 << 3); i < n; ++i) atomicAdd(&sh[zs[i]], 1u);
     }
+}
+
+// n_k = column sums of WT (WTDen_Sum_Update_Kernel utility.cu:1104-1126 plus
+// the tail's atomicAdd in sparseMatrixAdd :1132-1204): one LDS histogram of
+// z per workgroup, flushed with one atomic per nonzero topic.
+__global__ void k_topic_hist(const uint16_t *__restrict__ z, uint32_t N, int K, int32_t *__restrict__ nk) {
+    extern __shared__ uint32_t sh[];
+    for (int k = threadIdx.x; k < K; k += blockDim.x) sh[k] = 0;
+    __syncthreads();
+    hist_add_tokens(sh, z, N, (size_t)blockIdx.x * blockDim.x + threadIdx.x, (size_t)gridDim.x * blockDim.x);
     __syncthreads();
     for (int k = threadIdx.x; k < K; k += blockDim.x) {
         uint32_t c = sh[k];
@@ -164,7 +171,7 @@ __global__ void k_wt_head(const uint16_t *__restrict__ z, const uint4 *__restric
     const uint4 sg = segs[blockIdx.x];
     for (int k = threadIdx.x; k < K; k += blockDim.x) sh[k] = 0;
     __syncthreads();
-    for (uint32_t i = sg.y + threadIdx.x; i < sg.z; i += blockDim.x) atomicAdd(&sh[z[i]], 1u);
+    hist_add_tokens(sh, z + sg.y, sg.z - sg.y, threadIdx.x, blockDim.x);
     __syncthreads();
     int32_t *row = wt_head + (size_t)sg.x * K;
     for (int k = threadIdx.x; k < K; k += blockDim.x) {
@@ -175,16 +182,19 @@ __global__ void k_wt_head(const uint16_t *__restrict__ z, const uint4 *__restric
 
 // ------------------------------------------------------------- tail WT
 // Sorted sparse tail rows (WT_Update_Kernel, utility.cu:10-76): one
-// wavefront per tail word; 16-bit LDS histogram, then a ballot/popcount
-// compaction in ascending topic order, clearing the histogram as it goes.
+// wavefront per tail word; a 16-bit LDS histogram and a bitmap of the
+// word's topics, then a compaction over the bitmap in ascending topic order
+// (lane j owns topics [32j, 32j + 32) of each 64-word chunk and takes its
+// place from a wave prefix sum), clearing its counts and bitmap as it goes.
 __global__ void k_wt_tail(const uint16_t *__restrict__ z, const uint32_t *__restrict__ word_off, int VD, int V,
                           int K, const uint32_t *__restrict__ tail_off, uint32_t *__restrict__ wt_tail,
                           uint32_t *__restrict__ tail_nnz) {
     extern __shared__ uint32_t sh[];
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6, nwv = blockDim.x >> 6;
-    const int KH = (K + 1) >> 1;
-    uint32_t *h = sh + wv * KH;
-    for (int k = lane; k < KH; k += 64) h[k] = 0;
+    const int KH = (K + 1) >> 1, NW = (K + 31) >> 5;
+    uint32_t *h = sh + wv * (KH + NW);  // 16-bit counts
+    uint32_t *bm = h + KH;              // bitmap of the word's topics
+    for (int k = lane; k < KH + NW; k += 64) h[k] = 0;
     wave_lds_sync();
     const int ntail = V - VD;
     for (int tw = blockIdx.x * nwv + wv; tw < ntail; tw += gridDim.x * nwv) {
@@ -194,18 +204,34 @@ __global__ void k_wt_tail(const uint16_t *__restrict__ z, const uint32_t *__rest
             if (lane == 0) tail_nnz[tw] = 0;
             continue;
         }
-        for (uint32_t i = b + lane; i < e; i += 64) h16_add(h, z[i]);
+        for (uint32_t i = b + lane; i < e; i += 64) {
+            const uint32_t t = z[i];
+            h16_add(h, t);
+            atomicOr(&bm[t >> 5], 1u << (t & 31u));
+        }
         wave_lds_sync();
         const uint32_t base = tail_off[tw];
         uint32_t nz = 0;
-        for (int j = 0; j < K; j += 64) {
-            const int k = j + lane;
-            const uint32_t c = k < K ? h16_get(h, (uint32_t)k) : 0u;
-            const uint64_t m = __ballot(c != 0);
-            if (c) wt_tail[base + nz + __popcll(m & lanemask_lt(lane))] = (uint32_t)k | (c << 16);
-            nz += (uint32_t)__popcll(m);
-            wave_lds_sync();
-            if (k < K && !(k & 1)) h[k >> 1] = 0;
+        for (int j = 0; j < NW; j += 64) {
+            const int wi = j + lane;
+            const uint32_t word = wi < NW ? bm[wi] : 0u;
+            const uint32_t m = (uint32_t)__popc(word);
+            uint32_t incl = m;  // inclusive scan over lanes
+            for (int d = 1; d < 64; d <<= 1) {
+                const uint32_t v = __shfl_up(incl, d);
+                if (lane >= d) incl += v;
+            }
+            uint32_t pos = nz + incl - m;
+            for (uint32_t x = word; x; x &= x - 1u) {
+                const uint32_t k = ((uint32_t)wi << 5) + (uint32_t)__ffs(x) - 1u;
+                wt_tail[base + pos++] = k | (h16_get(h, k) << 16);
+            }
+            for (uint32_t x = word; x; x &= x - 1u) {  // the lane's own 16 count words
+                const uint32_t k = ((uint32_t)wi << 5) + (uint32_t)__ffs(x) - 1u;
+                h[k >> 1] = 0;
+            }
+            if (wi < NW) bm[wi] = 0;
+            nz += __shfl(incl, 63);
         }
         wave_lds_sync();
         if (lane == 0) tail_nnz[tw] = nz;
@@ -1604,8 +1630,8 @@ hipError_t launch_accumulate_wt(const DevState &s, hipStream_t st) {
         hipLaunchKernelGGL(k_wt_head, dim3(s.n_wt_segs), dim3(256), ldsK, st, s.z, s.wt_segs, s.K, s.wt_head);
     const int ntail = s.V - s.VD;
     if (ntail > 0) {
-        const WaveGrid g = wave_grid(s, (size_t)ntail, wave_hist_bytes(s.K), 1 << 16);
-        hipLaunchKernelGGL(k_wt_tail, dim3(g.grid), dim3(64 * g.nwv), g.nwv * wave_hist_bytes(s.K), st,
+        const WaveGrid g = wave_grid(s, (size_t)ntail, dt_wave_bytes(s.K), 1 << 16);
+        hipLaunchKernelGGL(k_wt_tail, dim3(g.grid), dim3(64 * g.nwv), g.nwv * dt_wave_bytes(s.K), st,
                            s.z, s.word_off, s.VD, s.V, s.K, s.tail_off, s.wt_tail, s.tail_nnz);
     }
     return hipGetLastError();
