@@ -7,6 +7,12 @@
 
 namespace ez {
 
+// Sweep counter and persistent-sampler queue heads: each head on its own
+// 128-byte line, so the workgroups claiming items from different queues do
+// not serialize on one line.
+static constexpr int kQueueStride = 32;
+static constexpr int kCounterWords = kQueueStride * 9;
+
 // Device image of one corpus + model.  Token ids are word-major positions;
 // N < 2^32.  Sizes: V words, VD head words, D docs, K topics.
 struct DevState {
@@ -46,7 +52,7 @@ struct DevState {
     uint4 *runs = nullptr;          // (first token, first slot, doc q, tokens): a word's tokens in one doc
     int64_t n_runs = 0;
     int32_t n_queues = 1;           // persistent sampler: item queues (<= 8)
-    uint32_t *counters = nullptr;   // [0] sweep, [8 .. 8 + n_queues) item queues
+    uint32_t *counters = nullptr;   // kCounterWords: [0] sweep, queue q's head at [kQueueStride * (q + 1)]
     unsigned long long *stats = nullptr;  // [0] full draws, [1] DT entries scanned
     double *ll_part = nullptr;      // n_smp_segs
     double *ll_out = nullptr;       // 1

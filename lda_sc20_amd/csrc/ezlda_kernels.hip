@@ -590,7 +590,7 @@ struct SampleArgs {
     const uint32_t *dt;
     uint16_t *z, *z_doc;      // by token (word-major) and by doc slot
     const uint32_t *counters;
-    uint32_t *queue;  // persistent mode: one item counter per queue
+    uint32_t *queue;  // persistent mode: queue q's item counter at queue[q * kQueueStride]
     uint32_t n_queues;
     uint32_t tb;  // compact rows: topic bits
     unsigned long long *stats;
@@ -884,7 +884,7 @@ __global__ void __launch_bounds__(1024) k_sample(const SampleArgs A) {
             const uint32_t q = (x0 + h) % nq;
             while (true) {
                 __syncthreads();  // the previous item is done with s_item and LDS
-                if (threadIdx.x == 0) s_item = atomicAdd(&A.queue[q], 1u);
+                if (threadIdx.x == 0) s_item = atomicAdd(&A.queue[q * kQueueStride], 1u);
                 __syncthreads();
                 const uint64_t item = (uint64_t)q + (uint64_t)s_item * nq;
                 if (item >= A.n_items) break;
@@ -1714,7 +1714,7 @@ hipError_t launch_sample(const DevState &s, bool shortcut, hipStream_t st) {
     A.z = s.z;
     A.z_doc = s.z_doc;
     A.counters = s.counters;
-    A.queue = s.counters + 8;  // counters[8 .. 8 + n_queues)
+    A.queue = s.counters + kQueueStride;
     A.n_queues = s.grid_dim > 0 ? (uint32_t)s.n_queues : 0u;
     A.stats = s.stats;
     A.seed = s.seed;
@@ -1723,7 +1723,8 @@ hipError_t launch_sample(const DevState &s, bool shortcut, hipStream_t st) {
     if (A.n_queues) {
         // queue heads start at zero (a fill kernel, so a captured sweep
         // replays it)
-        hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(64), 0, st, s.counters + 8, (size_t)A.n_queues, 0u);
+        hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(256), 0, st, s.counters + kQueueStride,
+                           (size_t)kQueueStride * A.n_queues, 0u);
         grid = (unsigned)s.grid_dim;
     }
     if (shortcut && s.compact)

@@ -539,8 +539,8 @@ int load_chunked(ezlda_ctx *c, const std::vector<uint32_t> &tok_doc, const std::
         if ((r = dalloc(c, &c->cdt[j], dt_words))) return r;
         EZ_TRY_HIP(c, hipMemset(c->cdt[j], 0, dt_words * sizeof(uint32_t)));
         if ((r = dalloc(c, &c->caux[j], max_n))) return r;
-        if ((r = dalloc(c, &c->cctr[j], 16))) return r;
-        EZ_TRY_HIP(c, hipMemset(c->cctr[j], 0, 16 * sizeof(uint32_t)));
+        if ((r = dalloc(c, &c->cctr[j], ez::kCounterWords))) return r;
+        EZ_TRY_HIP(c, hipMemset(c->cctr[j], 0, ez::kCounterWords * sizeof(uint32_t)));
         if ((r = dalloc(c, &c->cll[j], max_items))) return r;
         if ((r = dalloc(c, &c->cllo[j], 1))) return r;
     }
@@ -553,10 +553,10 @@ int load_chunked(ezlda_ctx *c, const std::vector<uint32_t> &tok_doc, const std::
     if ((r = dalloc(c, &s.zero_order, K))) return r;
     if ((r = dalloc(c, &s.tail_off, 1))) return r;
     EZ_TRY_HIP(c, hipMemset(s.tail_off, 0, sizeof(uint32_t)));
-    if ((r = dalloc(c, &s.counters, 16))) return r;
+    if ((r = dalloc(c, &s.counters, ez::kCounterWords))) return r;
     if ((r = dalloc(c, &s.stats, 2))) return r;
     if ((r = dalloc(c, &s.ll_out, 1))) return r;
-    EZ_TRY_HIP(c, hipMemset(s.counters, 0, 16 * sizeof(uint32_t)));
+    EZ_TRY_HIP(c, hipMemset(s.counters, 0, ez::kCounterWords * sizeof(uint32_t)));
     EZ_TRY_HIP(c, hipMemset(s.stats, 0, 2 * sizeof(unsigned long long)));
     if (!c->st2) EZ_TRY_HIP(c, hipStreamCreateWithFlags(&c->st2, hipStreamNonBlocking));
     s.VD = V;
@@ -967,11 +967,11 @@ int ezlda_load_docs(ezlda_ctx *c, int32_t V, int32_t D, const int64_t *doc_off, 
     if ((r = dalloc(c, &s.inv, K))) return r;
     if ((r = dalloc(c, &s.wtop, V))) return r;
     if ((r = dalloc(c, &s.zero_order, K))) return r;
-    if ((r = dalloc(c, &s.counters, 16))) return r;
+    if ((r = dalloc(c, &s.counters, ez::kCounterWords))) return r;
     if ((r = dalloc(c, &s.stats, 2))) return r;
     if ((r = dalloc(c, &s.ll_out, 1))) return r;
     if ((r = build_layout(c, tok_doc, tok_slot, slot_wrd))) return r;
-    EZ_TRY_HIP(c, hipMemsetAsync(s.counters, 0, 16 * sizeof(uint32_t), c->st));
+    EZ_TRY_HIP(c, hipMemsetAsync(s.counters, 0, ez::kCounterWords * sizeof(uint32_t), c->st));
     EZ_TRY_HIP(c, hipMemsetAsync(s.stats, 0, 2 * sizeof(unsigned long long), c->st));
     EZ_TRY_HIP(c, hipStreamSynchronize(c->st));
     EZ_TRY_HIP(c, hipDeviceSynchronize());
