@@ -79,7 +79,7 @@ typedef struct ezlda_args {
                                 not depend on it */
     int32_t run_cap;         /* the sampler shares one DT row scan among the
                                 repeats of a word in a document, at most
-                                run_cap tokens (1..32, 0 = 16); 1 = one scan
+                                run_cap tokens (1..32, 0 = 4); 1 = one scan
                                 per token.  Topics do not depend on it */
 } ezlda_args;
 

@@ -90,7 +90,7 @@ std::string check_layout_args(ezlda_args &a) {
     if (a.block_dim % 64 != 0 || a.block_dim > 1024) return "block_dim must be a multiple of 64 and <= 1024";
     if (a.token_segment <= 0) a.token_segment = 4096;
     if (a.grid_dim < 0) a.grid_dim = 8 * device_cus();  // auto: 8 persistent workgroups per CU
-    if (a.run_cap == 0) a.run_cap = 16;
+    if (a.run_cap == 0) a.run_cap = 4;
     if (a.run_cap < 1 || a.run_cap > 32) return "run_cap must be in [1, 32]";
     return "";
 }
@@ -712,7 +712,7 @@ void ezlda_default_args(ezlda_args *a) {
     a->dense_threshold = 0;  // K, as preprocess/streamTestBalance.cpp:291
     a->shortcut = 1;
     a->topic_bits = 16;
-    a->run_cap = 16;
+    a->run_cap = 4;
 }
 
 const char *ezlda_last_error(const ezlda_ctx *ctx) { return ctx ? ctx->err.c_str() : g_err.c_str(); }

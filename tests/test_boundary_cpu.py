@@ -54,7 +54,7 @@ def test_default_args_mirror_reference_argument_surface():
     # workgroups per CU (grid_dim < 0), runs of up to 16 tokens
     assert a.K == 1024 and a.beta == pytest.approx(0.01)
     assert a.alpha == pytest.approx(50.0 / 1024)
-    assert a.block_dim == 128 and a.grid_dim == -1 and a.run_cap == 16 and a.token_segment == 4096
+    assert a.block_dim == 128 and a.grid_dim == -1 and a.run_cap == 4 and a.token_segment == 4096
 
 
 # -------------------------------------------------------------- generator

@@ -85,6 +85,7 @@ CASES = [
     (64, 400, 300, 60, {"block_dim": 512}),
     # one DT scan per token; long runs; persistent workgroups on item queues
     (64, 400, 300, 60, {"run_cap": 1}),
+    (64, 400, 300, 60, {"run_cap": 16}),
     (64, 400, 300, 60, {"run_cap": 32, "grid_dim": 37}),
     (1024, 600, 150, 300, {"dense_threshold": 200, "run_cap": 3, "grid_dim": 300, "doc_tiles": 16}),
     (100, 500, 200, 80, {"grid_dim": 1, "doc_tiles": -1}),
