@@ -43,11 +43,14 @@ typedef struct ezlda_args {
     float beta;              /* Argument.cuh:8; <= 0 means 0.01 */
     int32_t block_dim;       /* Argument.cuh:4 BlockDim: threads per workgroup
                                 of the sampler; multiple of 64, <= 1024
-                                (<= 0 = 128) */
+                                (<= 0 = auto: 64 where sixteen such
+                                workgroups' LDS fits in a CU at this K,
+                                else 128) */
     int32_t grid_dim;        /* Argument.cuh:5 GridDim: > 0 = that many
                                 persistent sampler workgroups pulling items
                                 from per-XCD queues (then from the others');
-                                0 = one workgroup per item; < 0 = 8 per CU.
+                                0 = one workgroup per item; < 0 = 16 per CU
+                                at 64 threads, else 8 per CU.
                                 Topics do not depend on it */
     int32_t num_chunks;      /* mainKernel.cu:50: > 1 = out-of-core sweeps:
                                 the documents are cut into that many chunks

@@ -86,10 +86,14 @@ int device_lds_per_cu() {
 // Validates the layout arguments of `a` (those build_layout reads) and
 // fills their defaults; returns an error message or "".
 std::string check_layout_args(ezlda_args &a) {
-    if (a.block_dim <= 0) a.block_dim = 128;
+    // auto: one-wavefront workgroups, sixteen per CU, where sixteen samplers'
+    // LDS fits in a CU at this K (K <= ~1200 on MI355X), else 128 threads
+    if (a.block_dim <= 0)
+        a.block_dim = 16 * ez::sample_lds_bytes(a.K, 64) <= (size_t)device_lds_per_cu() ? 64 : 128;
     if (a.block_dim % 64 != 0 || a.block_dim > 1024) return "block_dim must be a multiple of 64 and <= 1024";
     if (a.token_segment <= 0) a.token_segment = 4096;
-    if (a.grid_dim < 0) a.grid_dim = 8 * device_cus();  // auto: 8 persistent workgroups per CU
+    // auto: 16 persistent workgroups per CU at 64 threads, else 8
+    if (a.grid_dim < 0) a.grid_dim = (a.block_dim == 64 ? 16 : 8) * device_cus();
     if (a.run_cap == 0) a.run_cap = 4;
     if (a.run_cap < 1 || a.run_cap > 32) return "run_cap must be in [1, 32]";
     return "";
@@ -702,8 +706,8 @@ void ezlda_default_args(ezlda_args *a) {
     a->K = 1024;             // Argument.cuh:6
     a->alpha = 50.0f / 1024; // Argument.cuh:9
     a->beta = 0.01f;         // Argument.cuh:8
-    a->block_dim = 128;      // Argument.cuh:4 has 256; 128 measures faster here
-    a->grid_dim = -1;        // auto: 8 persistent workgroups per CU
+    a->block_dim = 0;        // auto (Argument.cuh:4 has 256; 64 or 128 measure faster here)
+    a->grid_dim = -1;        // auto: 16 or 8 persistent workgroups per CU
     a->num_chunks = 1;
     a->num_streams = 1;
     a->token_segment = 4096;

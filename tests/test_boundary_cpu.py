@@ -50,11 +50,12 @@ def test_create_fails_loudly_without_device():
 def test_default_args_mirror_reference_argument_surface():
     a = ezlda.default_args()
     # EZLDA/Argument.cuh:3-9 (K, alpha = 50/K, beta); the launch shape is
-    # this build's measured default: 128-thread workgroups, 8 persistent
-    # workgroups per CU (grid_dim < 0), runs of up to 16 tokens
+    # this build's measured default: workgroup size and persistent grid
+    # chosen from K and the device (block_dim 0, grid_dim < 0), runs of up
+    # to 4 tokens
     assert a.K == 1024 and a.beta == pytest.approx(0.01)
     assert a.alpha == pytest.approx(50.0 / 1024)
-    assert a.block_dim == 128 and a.grid_dim == -1 and a.run_cap == 4 and a.token_segment == 4096
+    assert a.block_dim == 0 and a.grid_dim == -1 and a.run_cap == 4 and a.token_segment == 4096
 
 
 # -------------------------------------------------------------- generator
