@@ -156,6 +156,17 @@ int ezlda_sample(ezlda_ctx *ctx);
  * counts it needs).  Replaces PerplexityKernel (SamplingKernel.cuh:13);
  * perplexity = exp(-ll / N). */
 int ezlda_loglik(ezlda_ctx *ctx, double *ll);
+/* The same log-likelihood, bit for bit, from a call that allocates nothing:
+ * no reference counterpart.  It is for callers that cannot allocate device
+ * memory at this point (code under graph capture, tight serving loops,
+ * harnesses that forbid hipMalloc).  Its per-item partials and result live
+ * in a static device array of EZLDA_LOGLIK_NOALLOC_MAX_ITEMS + 1 doubles
+ * (8 MiB) that each GPU holds until the process ends; a corpus with more
+ * sampler items (ezlda_info.num_sample_items; a chunk's items when chunked)
+ * is rejected with EZLDA_EINVAL.  Every call on one GPU shares the array:
+ * calls that overlap on one GPU (from different contexts or threads) race. */
+#define EZLDA_LOGLIK_NOALLOC_MAX_ITEMS 1048576
+int ezlda_loglik_noalloc(ezlda_ctx *ctx, double *ll);
 
 /* ---- whole sweeps (mainKernel.cu:158-380) */
 int ezlda_sweep(ezlda_ctx *ctx, int32_t num_sweeps);

@@ -72,6 +72,11 @@ hipError_t launch_z_to_doc(const DevState &s, hipStream_t st);
 hipError_t launch_sample(const DevState &s, bool shortcut, hipStream_t st);
 hipError_t launch_advance_sweep(const DevState &s, hipStream_t st);
 hipError_t launch_loglik(const DevState &s, hipStream_t st);
+// Log-likelihood partials the allocation-free entry point can hold (one per
+// sampler item); its static device array holds these plus the result.
+static constexpr size_t kLLScratchItems = (size_t)1 << 20;
+// Device address of that array on the current device (hipGetSymbolAddress).
+hipError_t scratch_address(double **p);
 hipError_t launch_set_u32(uint32_t *p, uint32_t v, hipStream_t st);
 
 hipError_t launch_selftest_philox(const uint32_t *ctr, const uint32_t *key, uint32_t *out, int n, hipStream_t st);

@@ -61,6 +61,10 @@ struct ezlda_ctx {
     double *cllo[2] = {nullptr, nullptr};
     int32_t *wt_next = nullptr, *nk_next = nullptr;  // counts of the sweep's new topics
     hipStream_t st2 = nullptr;  // second chunk stream
+    // the device's static log-likelihood array (ezlda_loglik_noalloc), looked
+    // up once per device when the first context on it is created
+    double *ll_scratch = nullptr;
+    std::string scratch_err;
 };
 
 namespace ezc {

@@ -1594,6 +1594,20 @@ hipError_t launch_copy(const void *src, void *dst, size_t bytes, hipStream_t st)
     return hipGetLastError();
 }
 
+// ------------------------------------ allocation-free log-likelihood memory
+// The device memory of ezlda_loglik_noalloc: kLLScratchItems per-item
+// log-likelihood partials, then the result.  A file-scope array, so the call
+// needs no allocation; every GPU holds its own copy (8 MiB) until the process
+// ends, and every call on a GPU shares it.  Host code never passes the
+// array's name to a kernel (on the host it is only a stand-in for the
+// symbol): scratch_address() returns the device address, once per device.
+static __device__ __align__(256) double scratch[kLLScratchItems + 1];
+
+hipError_t scratch_address(double **p) {
+    *p = nullptr;
+    return hipGetSymbolAddress(reinterpret_cast<void **>(p), HIP_SYMBOL(scratch));
+}
+
 hipError_t launch_selftest_philox(const uint32_t *ctr, const uint32_t *key, uint32_t *out, int n, hipStream_t st) {
     hipLaunchKernelGGL(k_selftest_philox, dim3((n + 255) / 256), dim3(256), 0, st,
                        reinterpret_cast<const uint4 *>(ctr), reinterpret_cast<const uint2 *>(key),

@@ -8,6 +8,7 @@
 #include <cstdio>
 #include <cmath>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <thread>
 #include <vector>
@@ -63,6 +64,39 @@ uint32_t dt_cap(uint32_t n, int K) {
     const uint32_t m = std::min<uint32_t>(n, (uint32_t)K);
     if ((uint32_t)K > kCompactK) return m;
     return std::min<uint32_t>(m + n / dt_piece(K), n);
+}
+
+// Device address of the static log-likelihood array on each device, looked
+// up once per device (with that device current) by the first ezlda_create
+// there; kept for the rest of the process, as the array itself is.
+constexpr int kMaxDevices = 64;
+struct ScratchSlot {
+    bool looked_up = false;
+    double *ptr = nullptr;
+    std::string err;
+};
+std::mutex g_scratch_mu;
+std::array<ScratchSlot, kMaxDevices> g_scratch;
+
+void scratch_for_current_device(double **ptr, std::string *err) {
+    int dev = 0;
+    *ptr = nullptr;
+    if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= kMaxDevices) {
+        *err = "no device index for the static scratch array";
+        return;
+    }
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    ScratchSlot &sl = g_scratch[dev];
+    if (!sl.looked_up) {
+        const hipError_t e = ez::scratch_address(&sl.ptr);
+        if (e != hipSuccess || !sl.ptr) {
+            sl.ptr = nullptr;
+            sl.err = std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e);
+        }
+        sl.looked_up = true;
+    }
+    *ptr = sl.ptr;
+    *err = sl.err;
 }
 
 // Compute units of the current device (256 on MI355X).
@@ -782,6 +816,8 @@ ezlda_ctx *ezlda_create(const ezlda_args *args) {
         delete c;
         return nullptr;
     }
+    // a failed lookup only disables ezlda_loglik_noalloc, which reports it
+    scratch_for_current_device(&c->ll_scratch, &c->scratch_err);
     return c;
 }
 
@@ -1355,14 +1391,22 @@ int ezlda_reset_stats(ezlda_ctx *c) {
     return EZLDA_OK;
 }
 
-int ezlda_loglik(ezlda_ctx *c, double *ll) {
-    if (int r = need_loaded(c)) return r;
+namespace {
+
+// The log-likelihood of the current z.  part / out (both null, or both set)
+// replace the context's per-item partials and result: the allocation-free
+// entry point passes its static array there.  Same kernels, same order.
+int loglik_into(ezlda_ctx *c, double *ll, double *part, double *out) {
     if (is_chunked(c)) {  // per chunk, summed in chunk order
         if (int r = ezc::refresh_counts(c)) return r;
         double tot = 0.0;
         for (int k = 0; k < (int)c->chunks.size(); ++k) {
             const EzChunk &ch = c->chunks[k];
-            const ez::DevState cs = chunk_state(c, k, 0);
+            ez::DevState cs = chunk_state(c, k, 0);
+            if (part) {
+                cs.ll_part = part;
+                cs.ll_out = out;
+            }
             double v = 0.0;
             EZ_TRY_HIP(c, hipMemcpyAsync(c->cimg[0], ch.img, ch.bytes, hipMemcpyHostToDevice, c->st));
             EZ_TRY_HIP(c, ez::launch_build_dt(cs, c->st));
@@ -1374,14 +1418,46 @@ int ezlda_loglik(ezlda_ctx *c, double *ll) {
         *ll = tot;
         return EZLDA_OK;
     }
-    if (!c->counts_fresh) EZ_TRY_HIP(c, ez::launch_build_wt(c->s, c->st));
+    ez::DevState s = c->s;
+    if (part) {
+        s.ll_part = part;
+        s.ll_out = out;
+    }
+    if (!c->counts_fresh) EZ_TRY_HIP(c, ez::launch_build_wt(s, c->st));
     c->counts_fresh = true;
-    EZ_TRY_HIP(c, ez::launch_topk(c->s, c->st));
-    EZ_TRY_HIP(c, ez::launch_build_dt(c->s, c->st));
-    EZ_TRY_HIP(c, ez::launch_loglik(c->s, c->st));
-    EZ_TRY_HIP(c, hipMemcpyAsync(ll, c->s.ll_out, sizeof(double), hipMemcpyDeviceToHost, c->st));
+    EZ_TRY_HIP(c, ez::launch_topk(s, c->st));
+    EZ_TRY_HIP(c, ez::launch_build_dt(s, c->st));
+    EZ_TRY_HIP(c, ez::launch_loglik(s, c->st));
+    EZ_TRY_HIP(c, hipMemcpyAsync(ll, s.ll_out, sizeof(double), hipMemcpyDeviceToHost, c->st));
     EZ_TRY_HIP(c, hipStreamSynchronize(c->st));
     return EZLDA_OK;
+}
+
+}  // namespace
+
+int ezlda_loglik(ezlda_ctx *c, double *ll) {
+    if (int r = need_loaded(c)) return r;
+    if (!ll) return fail(c, EZLDA_EINVAL, "null result pointer");
+    return loglik_into(c, ll, nullptr, nullptr);
+}
+
+int ezlda_loglik_noalloc(ezlda_ctx *c, double *ll) {
+    if (int r = need_loaded(c)) return r;
+    if (!ll) return fail(c, EZLDA_EINVAL, "null result pointer");
+    if (!c->ll_scratch) return fail(c, EZLDA_EHIP, "the static log-likelihood array is unavailable: " + c->scratch_err);
+    // one partial per sampler item of the launch (a chunk's items when chunked)
+    uint64_t items = (uint64_t)std::max<int32_t>(c->s.n_smp_segs, 0);
+    if (is_chunked(c)) {
+        items = 0;
+        for (const EzChunk &ch : c->chunks) items = std::max<uint64_t>(items, ch.n_items);
+    }
+    if (items > ez::kLLScratchItems)
+        return fail(c, EZLDA_EINVAL, "the corpus has " + std::to_string(items) + " sampler items; "
+                    "ezlda_loglik_noalloc holds at most " + std::to_string(ez::kLLScratchItems) +
+                    " (EZLDA_LOGLIK_NOALLOC_MAX_ITEMS): use ezlda_loglik");
+    // every partial the final sum reads is written by the same launch, so
+    // nothing needs clearing
+    return loglik_into(c, ll, c->ll_scratch, c->ll_scratch + ez::kLLScratchItems);
 }
 
 int ezlda_get_topics(ezlda_ctx *c, uint16_t *zout) {

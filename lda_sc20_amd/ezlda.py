@@ -17,13 +17,14 @@ LIB_PATH = os.path.join(_PKG, "libezlda.so")
 
 OK, EINVAL, EHIP, EIO, ESTATE, ENOMEM = 0, 1, 2, 3, 4, 5
 FOLDIN_TOKEN_BYTES = 16  # EZLDA_FOLDIN_TOKEN_BYTES: server device bytes per resident token
+LOGLIK_NOALLOC_MAX_ITEMS = 1 << 20  # EZLDA_LOGLIK_NOALLOC_MAX_ITEMS
 STAGES = ("wt", "topk", "dt", "sample")
 
 # Every symbol include/ezlda.h declares (checked by the CPU test suite).
 EXPORTS = (
     "ezlda_default_args", "ezlda_last_error", "ezlda_create", "ezlda_destroy", "ezlda_load_docs",
     "ezlda_load_chunked", "ezlda_get_info", "ezlda_build_wt", "ezlda_topk", "ezlda_build_dt",
-    "ezlda_sample", "ezlda_loglik", "ezlda_sweep", "ezlda_sweep_graph", "ezlda_time_sweeps",
+    "ezlda_sample", "ezlda_loglik", "ezlda_loglik_noalloc", "ezlda_sweep", "ezlda_sweep_graph", "ezlda_time_sweeps",
     "ezlda_get_topics", "ezlda_set_topics", "ezlda_set_sweep", "ezlda_get_nk", "ezlda_get_wt_dense",
     "ezlda_get_dt_raw", "ezlda_get_top", "ezlda_get_aux", "ezlda_get_inv", "ezlda_save_checkpoint",
     "ezlda_load_checkpoint", "ezlda_gen_corpus", "ezlda_corpus_num_tokens", "ezlda_corpus_num_docs",
@@ -109,6 +110,7 @@ def lib() -> C.CDLL:
         "ezlda_build_dt": (C.c_int, [P]),
         "ezlda_sample": (C.c_int, [P]),
         "ezlda_loglik": (C.c_int, [P, C.POINTER(C.c_double)]),
+        "ezlda_loglik_noalloc": (C.c_int, [P, C.POINTER(C.c_double)]),
         "ezlda_sweep": (C.c_int, [P, C.c_int32]),
         "ezlda_sweep_graph": (C.c_int, [P, C.c_int32]),
         "ezlda_time_sweeps": (C.c_int, [P, C.c_int32, C.c_int32, C.POINTER(C.c_double), P]),
@@ -318,6 +320,13 @@ class Context:
     def loglik(self) -> float:
         x = C.c_double()
         self._chk(self._L.ezlda_loglik(self._h, C.byref(x)))
+        return x.value
+
+    def loglik_noalloc(self) -> float:
+        """ezlda_loglik_noalloc: the same value from a call that allocates no
+        device memory (partials in the library's static device array)."""
+        x = C.c_double()
+        self._chk(self._L.ezlda_loglik_noalloc(self._h, C.byref(x)))
         return x.value
 
     def time_sweeps(self, n: int, use_graph: bool = False):
