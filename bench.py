@@ -3,7 +3,7 @@
 corpus (V=102,660, D=300,000, log-normal doc lengths mean 330 / sigma 0.8,
 Zipf(1.07) vocabulary, dense WT head = top 1% of words), on one MI355X.
 
-    python bench.py [--gpus 1] [--steps K] [--warmup W]
+    python bench.py [--gpus 1] [--steps K] [--warmup W]     (defaults: K=20, W=5)
 
 A step is one whole sweep: WT/n_k rebuild, top-3, DT rebuild, sampling.  The
 corpus and model are resident in HBM before the timed region; the K timed
@@ -11,6 +11,10 @@ sweeps run back to back on the library's stream with HIP events around every
 stage (no host synchronisation inside the region); they give the value, the
 stage split and the sampler's roofline.  K more sweeps replayed from one
 captured hipGraph are reported beside them.  Prints one JSON line.
+
+Other modes (each prints one JSON line and exits): --copy-calibration and
+--gather-calibration run known-size kernels for scaling traffic counters;
+--cpu-protocol runs the CPU-baseline protocol on the host cores.
 """
 from __future__ import annotations
 
@@ -19,6 +23,7 @@ import ctypes
 import json
 import math
 import os
+import subprocess
 import sys
 import time
 
@@ -67,38 +72,113 @@ def device_sync() -> None:
         pass
 
 
-def measured_traffic(workload: str):
-    """HBM bytes per k_sample launch measured by rocprofv3 on this workload
-    (two --pmc runs, FETCH_SIZE then WRITE_SIZE, scaled by a copy kernel of
-    known size), or None when no such measurement is committed."""
+def measured_traffic(workload: str, run: dict):
+    """Bytes per k_sample launch that crossed the L2 boundary, measured by
+    rocprofv3 on this workload (FETCH_SIZE then WRITE_SIZE, each scaled by a
+    calibration kernel of known bytes), with where it came from -- or None
+    when the committed measurement was taken with another configuration or
+    sweep window than this run's (its `config` must match `run` key by key)."""
     path = os.path.join(ROOT, "profiles", f"hbm_traffic_{workload}.json")
     try:
         with open(path) as f:
-            return float(json.load(f)["hbm_bytes_per_launch"])
-    except (OSError, KeyError, ValueError):
+            m = json.load(f)
+        prof = m["config"]
+        if any(prof.get(k) != v for k, v in run.items()):
+            return None
+        return m
+    except (OSError, KeyError, ValueError, TypeError):
         return None
 
 
-def cpu_baseline(corpus, K: int, seed: int, max_docs: int, sweeps: int = 2):
-    """The OpenMP build of the CPU oracle (same sweep semantics, bit-exact)
-    on the first max_docs docs of the same corpus; tokens/s over `sweeps`
-    timed sweeps after one warm-up sweep."""
+def host_cpu() -> dict:
+    """Model name, sockets and cores of this host (lscpu)."""
+    out = {"model": None, "sockets": None, "cores_per_socket": None, "threads": os.cpu_count()}
+    try:
+        txt = subprocess.run(["lscpu"], stdout=subprocess.PIPE, stderr=subprocess.DEVNULL, text=True,
+                             timeout=20).stdout
+        for line in txt.splitlines():
+            k, _, v = line.partition(":")
+            k, v = k.strip(), v.strip()
+            if k == "Model name":
+                out["model"] = v
+            elif k == "Socket(s)":
+                out["sockets"] = int(v)
+            elif k == "Core(s) per socket":
+                out["cores_per_socket"] = int(v)
+    except (OSError, ValueError, subprocess.SubprocessError):
+        pass
+    return out
+
+
+def oracle_sweep_rates(corpus, K: int, seed: int, max_docs: int, sweeps: int, omp: bool, label: str):
+    """The CPU oracle (OpenMP build or single-threaded) on the first max_docs
+    docs of the corpus, full vocabulary: tokens/s of each of `sweeps` sweeps
+    (each a whole sweep: counts, top-3, DT rows, sampling), timed one by one."""
     from oracle import binding as ob
 
     D = min(max_docs, corpus.D)
     off = corpus.doc_off[: D + 1].copy()
     words = corpus.words[: int(off[-1])].copy()
-    orc = ob.Oracle(K, float(np.float32(50.0 / K)), 0.01, seed, corpus.V, off, words, omp=True)
-    orc.sweep(1)
-    t = time.perf_counter()
-    orc.sweep(sweeps)
-    dt = time.perf_counter() - t
-    cores = int(ob.lib(True).orc_num_threads())
+    orc = ob.Oracle(K, float(np.float32(50.0 / K)), 0.01, seed, corpus.V, off, words, omp=omp)
     n = int(off[-1])
+    rates = []
+    for i in range(sweeps):
+        t = time.perf_counter()
+        orc.sweep(1)
+        dt = time.perf_counter() - t
+        rates.append(n / dt)
+        log(f"{label}: sweep {i + 1}/{sweeps} {dt:.2f}s ({n / dt:.3e} tokens/s)")
+    cores = int(ob.lib(omp).orc_num_threads())
     orc.close()
-    return {"value": n * sweeps / dt, "unit": "tokens/s", "cores": cores, "kind": "port",
-            "sample": f"first {D} docs ({n} tokens) of the same corpus, full vocabulary, K={K}, "
-                      f"{sweeps} sweeps after 1 warm-up sweep, OpenMP oracle ({cores} threads)"}
+    return D, n, cores, rates
+
+
+def cpu_baseline(corpus, K: int, seed: int, max_docs: int, sweeps: int = 11):
+    """The OpenMP build of the CPU oracle (same sweep semantics, bit-exact) on
+    the first max_docs docs of the same corpus: the median tokens/s over
+    sweeps 2..sweeps (sweep 1 is warm-up)."""
+    D, n, cores, rates = oracle_sweep_rates(corpus, K, seed, max_docs, sweeps, True, "cpu baseline")
+    cpu = host_cpu()
+    return {"value": float(np.median(rates[1:])), "unit": "tokens/s", "cores": cores, "kind": "port",
+            "sample": f"first {D} docs ({n} tokens) of the same corpus, full vocabulary, K={K}; median tokens/s of "
+                      f"sweeps 2-{sweeps} (sweep 1 is warm-up), OpenMP oracle ({cores} threads)",
+            "host_cpu": cpu, "per_sweep_tokens_per_s": rates}
+
+
+def cpu_protocol(parts) -> dict:
+    """The CPU-baseline protocol: the single-threaded oracle on the reference's
+    CPU-sized corpus (all 100 sweeps) and on a 3-sweep sample of the ~99M-token
+    corpus; the OpenMP oracle on those and on the 738M-token corpus at K=1024.
+    Median tokens/s over sweeps 2-11 (sweep 1 is warm-up; fewer where noted).
+    Prints progress per sweep; returns one dict per part."""
+    from lda_sc20_amd import ezlda
+
+    out = {"host_cpu": host_cpu()}
+    plan = {
+        "cfg1_1t": ("cfg1", None, 100, False),
+        "cfg2_1t": ("cfg2", 30000, 3, False),
+        "cfg1_omp": ("cfg1", None, 11, True),
+        "cfg2_omp": ("cfg2", None, 11, True),
+        "cfg3_omp": ("cfg3", None, 4, True),
+    }
+    corpora = {}
+    for part in parts:
+        name, docs, sweeps, omp = plan[part]
+        wl = WORKLOADS[name]
+        if name not in corpora:
+            t = time.perf_counter()
+            corpora[name] = ezlda.gen_corpus(**wl["gen"])
+            log(f"{name}: generated {corpora[name].N} tokens in {time.perf_counter() - t:.1f}s")
+        c = corpora[name]
+        D, n, cores, rates = oracle_sweep_rates(c, wl["K"], wl["seed"], docs or c.D, sweeps, omp, part)
+        out[part] = {"workload": f"{name}: {wl['desc']}", "docs": D, "tokens": n, "K": wl["K"], "threads": cores,
+                     "sweeps": sweeps, "median_tokens_per_s_after_first": float(np.median(rates[1:])),
+                     "median_window": f"sweeps 2-{min(sweeps, 11)}" if sweeps <= 11 else "sweeps 2-11",
+                     "per_sweep_tokens_per_s": rates}
+        if sweeps > 11:
+            out[part]["median_tokens_per_s_after_first"] = float(np.median(rates[1:11]))
+            out[part]["all_sweeps_tokens_per_s"] = n * sweeps / sum(n / r for r in rates)
+    return out
 
 
 # fold-in requests: further documents of the training corpus's generative
@@ -234,8 +314,8 @@ def serving_bench(ctx, ezlda, wl: dict, args) -> dict:
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=190)
-    ap.add_argument("--warmup", type=int, default=10)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--workload", default="cfg2", choices=sorted(WORKLOADS))
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-docs", type=int, default=30000)
@@ -263,6 +343,12 @@ def main() -> int:
     ap.add_argument("--serve-plain-docs", type=int, default=32)
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
                     help="only run the plain copy kernel over BYTES (to scale HBM counters) and exit")
+    ap.add_argument("--gather-calibration", type=int, default=0, metavar="BYTES",
+                    help="only run scattered 64-byte block reads (the sampler's row pattern), one per distinct "
+                         "block of a BYTES buffer, and exit")
+    ap.add_argument("--cpu-protocol", default="", metavar="PARTS",
+                    help="only run the CPU-baseline protocol parts (comma list of cfg1_1t, cfg2_1t, cfg1_omp, "
+                         "cfg2_omp, cfg3_omp) and exit")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU: the Gibbs sweep here is single-GPU (--gpus {args.gpus} is not "
@@ -275,6 +361,17 @@ def main() -> int:
         ms = ezlda.selftest_copy(args.copy_calibration, 10)
         print(json.dumps({"copy_bytes": args.copy_calibration, "copies": 11, "ms_per_copy": ms,
                           "GBs": 2 * args.copy_calibration / ms / 1e6}), flush=True)
+        return 0
+    if args.gather_calibration:
+        nblk = 1
+        while nblk * 2 * 64 <= args.gather_calibration:
+            nblk *= 2
+        ms = ezlda.selftest_gather(args.gather_calibration, nblk, 10)
+        print(json.dumps({"buffer_bytes": args.gather_calibration, "reads": nblk, "read_bytes": 64 * nblk,
+                          "launches": 11, "ms_per_launch": ms, "GBs": 64 * nblk / ms / 1e6}), flush=True)
+        return 0
+    if args.cpu_protocol:
+        print(json.dumps(cpu_protocol([p for p in args.cpu_protocol.split(",") if p])), flush=True)
         return 0
 
     wl = WORKLOADS[args.workload]
@@ -361,6 +458,13 @@ def main() -> int:
                + 4 * K * VD + 4 * tail_nnz + 8 * info.num_words             # top-3
                + 2 * N + 4 * N + 4 * N + eb * dt_nnz + 8 * info.num_docs)   # DT + aux (z_doc, word, aux)
     ll = ctx.loglik()
+    # the committed rocprof measurement of k_sample, used only when it was
+    # taken on this configuration over this sweep window (the topics, hence
+    # the work, are then the same, sweep by sweep)
+    run_cfg = {"workload": args.workload, "K": K, "num_head_words": VD, "run_cap": int(info.run_cap),
+               "grid_dim": int(info.grid_dim), "block_dim": int(info.block_dim), "doc_tiles": int(info.doc_tiles),
+               "sweeps_before_timing": sweeps_before, "steps": K_steps}
+    tr = None if chunked else measured_traffic(args.workload, run_cfg)
     out = {
         "metric": "sampled tokens/s per sweep at K=1024 (+% HBM peak)" if K == 1024 else
                   f"sampled tokens/s per sweep at K={K}",
@@ -382,9 +486,12 @@ def main() -> int:
                    "token_segment": int(info.token_segment), "num_chunks": int(info.num_chunks)},
         "roofline": None if chunked else
                     {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": measured_traffic(args.workload),
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": tr["bytes_per_launch"] if tr else None,
                      "kernel": "k_sample", "bytes_per_launch": b_sample, "avg_launch_ms": t_sample * 1e3,
-                     "traffic_source": "profiles/hbm_traffic_cfg2.md (FETCH_SIZE, WRITE_SIZE; copy-calibrated)"},
+                     "traffic_kind": tr["kind"] if tr else None,
+                     "traffic_source": tr["source"] if tr else
+                     "none: no committed measurement of this configuration and sweep window",
+                     "traffic_window_algorithmic_bytes": tr["algorithmic_bytes_per_launch"] if tr else None},
         "stage_ms_per_sweep": {k: v / K_steps for k, v in stage_ms.items()},
         "sweep_algorithmic_GBs": b_sweep / (ms_per_step / 1e3) / 1e9,
         "full_draw_fraction": n_full / (N * K_steps),

@@ -245,6 +245,13 @@ int ezlda_selftest_philox(const uint32_t *ctr, const uint32_t *key, uint32_t *ou
  * plain copy kernel; *ms = mean time per copy.  The known-size workload
  * that scales hardware traffic counters. */
 int ezlda_selftest_copy(int64_t bytes, int32_t reps, double *ms);
+/* n_reads scattered 64-byte block reads (four 16-byte loads per lane, the
+ * sampler's DT row-block pattern) over a `bytes` buffer, each of them a
+ * distinct block (n_reads must not exceed the largest power of two of 64-byte
+ * blocks the buffer holds), `reps` times; *ms = mean time per launch.  The
+ * known-size workload that scales the traffic counters for that request mix
+ * (64 * n_reads bytes per launch). */
+int ezlda_selftest_gather(int64_t bytes, int64_t n_reads, int32_t reps, double *ms);
 
 /* ---- checkpoint (the state is z plus the sweep counter) */
 int ezlda_save_checkpoint(ezlda_ctx *ctx, const char *path);

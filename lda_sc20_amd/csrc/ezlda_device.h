@@ -81,6 +81,11 @@ hipError_t launch_set_u32(uint32_t *p, uint32_t v, hipStream_t st);
 
 hipError_t launch_selftest_philox(const uint32_t *ctr, const uint32_t *key, uint32_t *out, int n, hipStream_t st);
 hipError_t launch_copy(const void *src, void *dst, size_t bytes, hipStream_t st);
+// n_reads scattered 64-byte block reads (four 16-byte loads per lane), each
+// a distinct block of the largest power-of-two block count within bytes of
+// src (n_reads may not exceed it); sink (>= 64 words) is written only on an
+// impossible value.
+hipError_t launch_gather64(const void *src, size_t bytes, uint64_t n_reads, uint32_t *sink, hipStream_t st);
 
 // Fold-in against a frozen model: per-word phi rows, Q running sums inside
 // 64-topic slabs (qs, V x K) and over slab totals (qacc, V x ceil(K/64)).

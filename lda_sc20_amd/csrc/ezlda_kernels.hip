@@ -1587,6 +1587,36 @@ __global__ void k_copy_u4(const uint4 *__restrict__ src, uint4 *__restrict__ dst
         dst[i] = src[i];
 }
 
+// Scattered 64-byte reads (the sampler's row-block pattern: each lane loads
+// four 16-byte pieces of one 64-byte block at an address no neighbouring lane
+// shares), n_reads of them over a buffer of nblk = 2^m 64-byte blocks: read i
+// takes block (i * an odd constant) mod nblk, a bijection, so n_reads <= nblk
+// reads touch n_reads distinct blocks, each once (64 * n_reads bytes, none of
+// them cached by an earlier read).  The known-size workload that scales the
+// traffic counters for that request mix.  Nothing is written unless the data
+// holds an impossible value (the loads cannot be removed).
+__global__ void k_gather64(const uint4 *__restrict__ src, uint64_t blk_mask, uint64_t n_reads,
+                           uint32_t *__restrict__ sink) {
+    uint32_t acc = 0;
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n_reads;
+         i += (uint64_t)gridDim.x * blockDim.x) {
+        const uint64_t blk = (i * 0x9E3779B97F4A7C15ull) & blk_mask;
+        const uint4 *p = src + blk * 4u;
+        const uint4 a = p[0], b = p[1], c = p[2], d = p[3];
+        acc ^= a.x ^ a.w ^ b.y ^ c.z ^ d.x ^ d.w;
+    }
+    if (acc == 0x9E3779B9u) sink[threadIdx.x & 63u] = acc;
+}
+
+hipError_t launch_gather64(const void *src, size_t bytes, uint64_t n_reads, uint32_t *sink, hipStream_t st) {
+    uint64_t nblk = 1;
+    while (nblk * 2 * 64 <= bytes) nblk *= 2;
+    if (bytes < 64 || n_reads > nblk) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(k_gather64, dim3(grid_for((size_t)n_reads, 256, 8192)), dim3(256), 0, st,
+                       reinterpret_cast<const uint4 *>(src), nblk - 1, n_reads, sink);
+    return hipGetLastError();
+}
+
 hipError_t launch_copy(const void *src, void *dst, size_t bytes, hipStream_t st) {
     const size_t n = bytes / 16;
     hipLaunchKernelGGL(k_copy_u4, dim3(grid_for(n, 256, 8192)), dim3(256), 0, st,

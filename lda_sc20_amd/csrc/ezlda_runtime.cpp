@@ -1699,6 +1699,39 @@ int ezlda_selftest_copy(int64_t bytes, int32_t reps, double *ms) {
     return EZLDA_OK;
 }
 
+int ezlda_selftest_gather(int64_t bytes, int64_t n_reads, int32_t reps, double *ms) {
+    if (bytes < 64 || n_reads < 1 || reps < 1) return fail(nullptr, EZLDA_EINVAL, "bad gather calibration arguments");
+    bytes &= ~(int64_t)63;
+    int64_t nblk = 1;
+    while (nblk * 2 * 64 <= bytes) nblk *= 2;
+    if (n_reads > nblk)
+        return fail(nullptr, EZLDA_EINVAL, "n_reads exceeds the buffer's " + std::to_string(nblk) + " distinct blocks");
+    void *a = nullptr;
+    uint32_t *sink = nullptr;
+    EZ_TRY_HIP(nullptr, hipMalloc(&a, (size_t)bytes));
+    EZ_TRY_HIP(nullptr, hipMalloc((void **)&sink, 64 * sizeof(uint32_t)));
+    EZ_TRY_HIP(nullptr, hipMemset(a, 1, (size_t)bytes));
+    EZ_TRY_HIP(nullptr, hipDeviceSynchronize());
+    hipEvent_t e0, e1;
+    EZ_TRY_HIP(nullptr, hipEventCreate(&e0));
+    EZ_TRY_HIP(nullptr, hipEventCreate(&e1));
+    EZ_TRY_HIP(nullptr, ez::launch_gather64(a, (size_t)bytes, (uint64_t)n_reads, sink, nullptr));
+    EZ_TRY_HIP(nullptr, hipEventRecord(e0, nullptr));
+    for (int i = 0; i < reps; ++i)
+        EZ_TRY_HIP(nullptr, ez::launch_gather64(a, (size_t)bytes, (uint64_t)n_reads, sink, nullptr));
+    EZ_TRY_HIP(nullptr, hipEventRecord(e1, nullptr));
+    EZ_TRY_HIP(nullptr, hipEventSynchronize(e1));
+    float t = 0.f;
+    const hipError_t et = hipEventElapsedTime(&t, e0, e1);
+    if (ms) *ms = t / reps;
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    (void)hipFree(a);
+    (void)hipFree(sink);
+    if (et != hipSuccess) return fail(nullptr, EZLDA_EHIP, "hipEventElapsedTime failed");
+    return EZLDA_OK;
+}
+
 // Checkpoint: "EZLDACK1", then int64 K, V, D, N, uint64 seed, uint32 sweep,
 // then z (doc-major, uint16).  Restoring z and the sweep counter restores
 // the whole sampler state: counts are rebuilt from z and the draws are keyed

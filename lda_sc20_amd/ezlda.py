@@ -30,7 +30,7 @@ EXPORTS = (
     "ezlda_load_checkpoint", "ezlda_gen_corpus", "ezlda_corpus_num_tokens", "ezlda_corpus_num_docs",
     "ezlda_corpus_num_words", "ezlda_corpus_doc_off", "ezlda_corpus_words", "ezlda_corpus_topics",
     "ezlda_corpus_free", "ezlda_corpus_from_arrays", "ezlda_read_docword", "ezlda_write_chunked",
-    "ezlda_read_chunked", "ezlda_selftest_philox", "ezlda_selftest_copy", "ezlda_get_stats",
+    "ezlda_read_chunked", "ezlda_selftest_philox", "ezlda_selftest_copy", "ezlda_selftest_gather", "ezlda_get_stats",
     "ezlda_reset_stats", "ezlda_server_create", "ezlda_server_destroy", "ezlda_server_last_error",
     "ezlda_server_submit", "ezlda_server_submit_many", "ezlda_server_wait", "ezlda_server_drain", "ezlda_server_get_stats",
     "ezlda_server_reset_stats", "ezlda_foldin_plain", "ezlda_set_layout", "ezlda_autotune",
@@ -139,6 +139,7 @@ def lib() -> C.CDLL:
         "ezlda_read_chunked": (P, [C.c_char_p, C.c_int32]),
         "ezlda_selftest_philox": (C.c_int, [P, P, P, C.c_int32]),
         "ezlda_selftest_copy": (C.c_int, [C.c_int64, C.c_int32, C.POINTER(C.c_double)]),
+        "ezlda_selftest_gather": (C.c_int, [C.c_int64, C.c_int64, C.c_int32, C.POINTER(C.c_double)]),
         "ezlda_get_stats": (C.c_int, [P, P]),
         "ezlda_reset_stats": (C.c_int, [P]),
         "ezlda_set_layout": (C.c_int, [P, C.POINTER(Args)]),
@@ -514,6 +515,16 @@ def selftest_copy(nbytes: int, reps: int = 10) -> float:
     """Mean ms of a plain device-to-device copy kernel over nbytes."""
     ms = C.c_double()
     r = lib().ezlda_selftest_copy(nbytes, reps, C.byref(ms))
+    if r:
+        raise EzldaError(r, last_error())
+    return ms.value
+
+
+def selftest_gather(nbytes: int, n_reads: int, reps: int = 10) -> float:
+    """Mean ms of n_reads scattered 64-byte block reads over an nbytes buffer
+    (the sampler's row-block access pattern; 64 * n_reads bytes per launch)."""
+    ms = C.c_double()
+    r = lib().ezlda_selftest_gather(nbytes, n_reads, reps, C.byref(ms))
     if r:
         raise EzldaError(r, last_error())
     return ms.value
