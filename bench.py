@@ -340,7 +340,7 @@ def main() -> int:
                     help="> 1: out-of-core sweeps over that many chunks in pinned host memory (times whole sweeps, "
                          "PCIe transfers included)")
     ap.add_argument("--serve-budget", type=int, default=1 << 30, help="HBM bytes for resident requests")
-    ap.add_argument("--serve-plain-docs", type=int, default=32)
+    ap.add_argument("--serve-plain-docs", type=int, default=256)
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
                     help="only run the plain copy kernel over BYTES (to scale HBM counters) and exit")
     ap.add_argument("--gather-calibration", type=int, default=0, metavar="BYTES",

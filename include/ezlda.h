@@ -75,7 +75,11 @@ typedef struct ezlda_args {
                                 <= 0 = K */
     int32_t shortcut;        /* 1 = max-topic shortcut, 0 = plain sampler;
                                 both give identical topics */
-    int32_t topic_bits;      /* width of a stored topic; only 16 */
+    int32_t topic_bits;      /* width of a stored topic (the CLI's
+                                --topic-bits): 16 (0 = 16); K <= 16383 keeps
+                                every topic in 16 bits, other widths are
+                                refused.  Compact DT rows pack a topic in
+                                max(10, ceil(log2 K)) bits internally */
     int32_t doc_tiles;       /* head-word sampler items cut by doc tiles for
                                 L2 locality, dealt to XCDs: 0 = auto,
                                 > 0 = that many tiles, < 0 = off; topics do
@@ -152,6 +156,29 @@ int ezlda_build_dt(ezlda_ctx *ctx);
  * Replaces UpdateProbKernelD (SamplingKernel.cuh:15), SampleKernelD (:10)
  * and SampleKernel (:11). */
 int ezlda_sample(ezlda_ctx *ctx);
+/* The sampler and count stages at the reference's launcher granularity.  A
+ * token's draw depends only on the counts built at the start of the sweep,
+ * so ezlda_shortcut, ezlda_sample_head and ezlda_sample_tail (in any order)
+ * followed by ezlda_commit_head give exactly the topics of ezlda_sample.
+ * Max-topic shortcut of the head words' tokens: each accepted token gets its
+ * word's top topic, the others keep theirs for the head sampler.  A no-op
+ * when args.shortcut is 0.  Replaces UpdateProbKernelD (SamplingKernel.cuh:15). */
+int ezlda_shortcut(ezlda_ctx *ctx);
+/* Draws the head words' tokens (the shortcut test first; tokens it accepts
+ * get the top topic, as after ezlda_shortcut).  Replaces SampleKernelD
+ * (SamplingKernel.cuh:10). */
+int ezlda_sample_head(ezlda_ctx *ctx);
+/* Draws the tail words' tokens, with the same inline shortcut.  Replaces
+ * SampleKernel (SamplingKernel.cuh:11). */
+int ezlda_sample_tail(ezlda_ctx *ctx);
+/* Ends the sweep: the dense head rows are rebuilt from the new z (the
+ * reference copies its accumulated head counts back, mainKernel.cu:355-356)
+ * and the sweep counter advances.  Tail rows and n_k are rebuilt by the next
+ * ezlda_build_wt. */
+int ezlda_commit_head(ezlda_ctx *ctx);
+/* n_k = column sums of WT, from the current z.  Replaces
+ * UpdateWTDenRowSumKernel (WTDenUpdateKernel.cuh:26). */
+int ezlda_wt_rowsum(ezlda_ctx *ctx);
 /* FP64 log-likelihood sum_t log p(w_t | d_t) of the current z (rebuilds the
  * counts it needs).  Replaces PerplexityKernel (SamplingKernel.cuh:13);
  * perplexity = exp(-ll / N). */
@@ -254,8 +281,22 @@ int ezlda_selftest_copy(int64_t bytes, int32_t reps, double *ms);
 int ezlda_selftest_gather(int64_t bytes, int64_t n_reads, int32_t reps, double *ms);
 
 /* ---- checkpoint (the state is z plus the sweep counter) */
+/* A private binary file: header, seed, sweep, z in doc-major order. */
 int ezlda_save_checkpoint(ezlda_ctx *ctx, const char *path);
 int ezlda_load_checkpoint(ezlda_ctx *ctx, const char *path);
+/* The reference's own form (no reference counterpart writes it; its
+ * preprocess writes the initial topics there, preprocess/streamTestBalance.cpp:469,
+ * and its loader reads them, DataChunk.cu:85): every chunk<c>/TL.txt of the
+ * chunked folder `prefix` rewritten with the current topics (1-based, in the
+ * file's own token order) and <prefix>/ezlda_state.txt = "seed S\nsweep N\nK K\n".
+ * The folder must hold the context's corpus (the one ezlda_load_chunked read
+ * from it; checked token by token).  A current binary sidecar is rewritten
+ * with the same topics. */
+int ezlda_save_tl(ezlda_ctx *ctx, const char *prefix);
+/* ezlda_load_chunked(prefix) plus the sweep counter from ezlda_state.txt
+ * (when present; its seed and K must be the context's), so the next sweeps
+ * are those the saving context would have run. */
+int ezlda_load_tl(ezlda_ctx *ctx, const char *prefix);
 
 /* ---- fold-in serving (no reference counterpart: EZLDA only trains)
  * A server freezes phi from a trained context's current counts and infers
@@ -300,7 +341,8 @@ int ezlda_server_submit(ezlda_server *srv, uint32_t request_id, const int32_t *w
 int ezlda_server_submit_many(ezlda_server *srv, int32_t n_docs, const uint32_t *request_ids, const int64_t *doc_off,
                              const int32_t *words);
 /* Blocks until the request is done (timeout_ms < 0: no limit) and copies its
- * final topics (n values) and topic counts (K values); either may be NULL.
+ * final topics (as many values as the request was submitted with: z_out
+ * must hold them) and topic counts (K values); either may be NULL.
  * The result is then released.  An id that is neither queued, resident nor
  * done is EZLDA_EINVAL; so is submitting an id that is in flight or has an
  * unclaimed result. */
@@ -310,8 +352,9 @@ int ezlda_server_wait(ezlda_server *srv, uint32_t request_id, uint16_t *z_out, i
 int ezlda_server_drain(ezlda_server *srv);
 int ezlda_server_get_stats(ezlda_server *srv, ezlda_server_stats *stats);
 int ezlda_server_reset_stats(ezlda_server *srv);
-/* The plain path: one document, one step per sweep, synchronous, on its
- * own stream; same results as the server. */
+/* The plain path: one document per kernel launch (one workgroup runs all of
+ * its sweeps, its row in LDS), synchronous, on its own stream; same results
+ * as the server.  Calls on one server are serialized. */
 int ezlda_foldin_plain(ezlda_server *srv, uint32_t request_id, const int32_t *words, int32_t n, uint16_t *z_out,
                        int32_t *counts_out);
 
@@ -358,7 +401,11 @@ int ezlda_write_chunked(const char *prefix, const ezlda_corpus *c, const uint16_
                         int32_t num_chunks, uint64_t seed);
 /* Reads that folder (DataChunk.cu:50-100, Doc.cu:45-108): docs of chunk 0
  * first; word ids are the file's ranks; topics 0-based.  num_chunks <= 0
- * reads the count from TLLength.txt.  NULL on failure. */
+ * reads the count from TLLength.txt.  NULL on failure.
+ * The binary sidecar: ezlda_write_chunked also writes <prefix>/corpus.ezb,
+ * the same corpus in binary with the size and modification time of every
+ * chunk's TL.txt, word2DocMap.txt and docCountOffset.txt; this reader takes
+ * it (no text parsing) while all of them still match, else the text. */
 ezlda_corpus *ezlda_read_chunked(const char *prefix, int32_t num_chunks);
 
 #ifdef __cplusplus

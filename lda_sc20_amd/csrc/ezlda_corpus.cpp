@@ -13,7 +13,9 @@
 #include <cstdio>
 #include <cstring>
 #include <fstream>
+#include <new>
 #include <numeric>
+#include <stdexcept>
 #include <string>
 #include <sys/stat.h>
 #include <thread>
@@ -31,6 +33,7 @@ struct ezlda_corpus {
 
 namespace ezc {
 void set_error(const std::string &m);
+int exception_status(ezlda_ctx *c);
 }
 
 namespace {
@@ -154,11 +157,286 @@ bool make_dir(const std::string &p) {
     return false;
 }
 
+// ---- binary sidecar of the chunked folder
+// <prefix>/corpus.ezb holds the doc-major corpus exactly as the text reader
+// returns it (docs of chunk 0 first; word ids are ranks; topics 0-based),
+// with the size and modification time of every chunk's TL.txt,
+// word2DocMap.txt and docCountOffset.txt: the reader takes it only while all
+// of them still match, so a rewritten or edited text file falls back to the
+// text path.  Layout (little-endian): "EZLDACB1", u32 version 1, u32 chunks,
+// i64 V, D, N, per chunk and file i64 size and mtime (ns), i64 doc_off[D+1],
+// i32 words[N], u16 topics[N].
+constexpr char kSideMagic[8] = {'E', 'Z', 'L', 'D', 'A', 'C', 'B', '1'};
+constexpr const char *kSideFiles[3] = {"TL.txt", "word2DocMap.txt", "docCountOffset.txt"};
+
+bool file_stamp(const std::string &path, int64_t out[2]) {
+    struct stat st;
+    if (stat(path.c_str(), &st) != 0) return false;
+    out[0] = (int64_t)st.st_size;
+    out[1] = (int64_t)st.st_mtim.tv_sec * 1000000000ll + (int64_t)st.st_mtim.tv_nsec;
+    return true;
+}
+
+bool chunk_stamps(const std::string &root, int chunks, std::vector<int64_t> &st) {
+    st.assign((size_t)chunks * 6, 0);
+    for (int ch = 0; ch < chunks; ++ch)
+        for (int f = 0; f < 3; ++f)
+            if (!file_stamp(root + "/chunk" + std::to_string(ch) + "/" + kSideFiles[f], &st[(size_t)ch * 6 + 2 * f]))
+                return false;
+    return true;
+}
+
+std::string sidecar_path(const std::string &root) { return root + "/corpus.ezb"; }
+
+// Writes the sidecar of a complete folder; false (and the error set) on an
+// I/O failure.
+bool write_sidecar(const std::string &root, int chunks, const ezlda_corpus &c) {
+    std::vector<int64_t> st;
+    if (!chunk_stamps(root, chunks, st)) {
+        err("cannot stat the chunk files of " + root);
+        return false;
+    }
+    const std::string path = sidecar_path(root), tmp = path + ".tmp";
+    std::FILE *f = std::fopen(tmp.c_str(), "wb");
+    if (!f) {
+        err("cannot write " + tmp);
+        return false;
+    }
+    const uint32_t hdr[2] = {1u, (uint32_t)chunks};
+    const int64_t dims[3] = {c.V, c.D, c.doc_off.back()};
+    bool ok = std::fwrite(kSideMagic, 1, 8, f) == 8 && std::fwrite(hdr, 4, 2, f) == 2 && std::fwrite(dims, 8, 3, f) == 3 &&
+              std::fwrite(st.data(), 8, st.size(), f) == st.size() &&
+              std::fwrite(c.doc_off.data(), 8, c.doc_off.size(), f) == c.doc_off.size() &&
+              std::fwrite(c.words.data(), 4, c.words.size(), f) == c.words.size() &&
+              std::fwrite(c.topics.data(), 2, c.topics.size(), f) == c.topics.size();
+    ok = (std::fclose(f) == 0) && ok;
+    if (!ok || std::rename(tmp.c_str(), path.c_str()) != 0) {
+        std::remove(tmp.c_str());
+        err("cannot write " + path);
+        return false;
+    }
+    return true;
+}
+
+// The sidecar's corpus when it exists, describes this folder (V, D, N and
+// chunk count as the text files give them) and every stamped file is
+// unchanged; else nullptr (no error: the caller reads the text).
+ezlda_corpus *read_sidecar(const std::string &root, int chunks, int64_t V, int64_t D, int64_t N) {
+    std::FILE *f = std::fopen(sidecar_path(root).c_str(), "rb");
+    if (!f) return nullptr;
+    ezlda_corpus *c = nullptr;
+    char magic[8];
+    uint32_t hdr[2];
+    int64_t dims[3];
+    std::vector<int64_t> st((size_t)chunks * 6), now;
+    if (std::fread(magic, 1, 8, f) == 8 && std::memcmp(magic, kSideMagic, 8) == 0 && std::fread(hdr, 4, 2, f) == 2 &&
+        hdr[0] == 1u && hdr[1] == (uint32_t)chunks && std::fread(dims, 8, 3, f) == 3 && dims[0] == V &&
+        dims[1] == D && dims[2] == N && std::fread(st.data(), 8, st.size(), f) == st.size() &&
+        chunk_stamps(root, chunks, now) && now == st) {
+        c = new ezlda_corpus();
+        c->V = (int32_t)V;
+        c->D = (int32_t)D;
+        c->doc_off.resize((size_t)D + 1);
+        c->words.resize((size_t)N);
+        c->topics.resize((size_t)N);
+        const bool ok = std::fread(c->doc_off.data(), 8, c->doc_off.size(), f) == c->doc_off.size() &&
+                        std::fread(c->words.data(), 4, c->words.size(), f) == c->words.size() &&
+                        std::fread(c->topics.data(), 2, c->topics.size(), f) == c->topics.size() &&
+                        c->doc_off[0] == 0 && c->doc_off[(size_t)D] == N;
+        bool sane = ok;
+        for (size_t d = 0; sane && d < (size_t)D; ++d) sane = c->doc_off[d] <= c->doc_off[d + 1];
+        for (size_t i = 0; sane && i < (size_t)N; ++i) sane = c->words[i] >= 0 && c->words[i] < V;
+        if (!sane) {
+            delete c;
+            c = nullptr;
+        }
+    }
+    std::fclose(f);
+    return c;
+}
+
+// The chunked folder's text files read into a doc-major corpus (chunk 0's
+// docs first, local doc order; word ids are ranks; topics from TL.txt made
+// 0-based).  pos (nullable) receives, per doc-major token, chunk << 32 | its
+// position in the chunk's TL.txt.  tll, dl: per-chunk tokens and docs.
+ezlda_corpus *read_chunked_text(const std::string &root, int num_chunks, int64_t V, const std::vector<int64_t> &tll,
+                                const std::vector<int64_t> &dl, std::vector<uint64_t> *pos) {
+    auto *c = new ezlda_corpus();
+    c->V = (int32_t)V;
+    if (pos) pos->clear();
+    for (int ch = 0; ch < num_chunks; ++ch) {
+        const std::string dir = root + "/chunk" + std::to_string(ch);
+        const int64_t Nc = tll[ch], Dc = dl[ch];
+        if (Nc < 0 || Dc < 0 || Nc > 0xFFFFFFFFll || Dc > 0x7FFFFFFFll) {
+            err(root + ": chunk " + std::to_string(ch) + " has a negative or oversized token or document count");
+            delete c;
+            return nullptr;
+        }
+        std::vector<int64_t> tl, w2d, wco, dco;
+        if (!read_ints(dir + "/TL.txt", tl, (size_t)Nc) || !read_ints(dir + "/word2DocMap.txt", w2d, (size_t)Nc) ||
+            !read_ints(dir + "/wordCountOffset.txt", wco, (size_t)(2 * V)) ||
+            !read_ints(dir + "/docCountOffset.txt", dco, (size_t)(2 * Dc))) {
+            delete c;
+            return nullptr;
+        }
+        // rank of each TL position
+        std::vector<int32_t> rank_at(Nc, -1);
+        for (int64_t r = 0; r < V; ++r) {
+            const int64_t n = wco[2 * r], o = wco[2 * r + 1];
+            if (n < 0 || o < 0 || o + n > Nc) {
+                err(dir + "/wordCountOffset.txt: range out of bounds at rank " + std::to_string(r));
+                delete c;
+                return nullptr;
+            }
+            for (int64_t i = o; i < o + n; ++i) rank_at[i] = (int32_t)r;
+        }
+        int64_t expect = 0;
+        for (int64_t ld = 0; ld < Dc; ++ld) {
+            const int64_t n = dco[2 * ld], o = dco[2 * ld + 1];
+            if (o != expect || n < 0 || o + n > Nc) {
+                err(dir + "/docCountOffset.txt: inconsistent entry " + std::to_string(ld));
+                delete c;
+                return nullptr;
+            }
+            expect += n;
+            for (int64_t i = o; i < o + n; ++i) {
+                const int64_t p = w2d[i];
+                if (p < 0 || p >= Nc || rank_at[p] < 0 || tl[p] < 1 || tl[p] > 65535) {
+                    err(dir + ": bad token " + std::to_string(i));
+                    delete c;
+                    return nullptr;
+                }
+                c->words.push_back(rank_at[p]);
+                c->topics.push_back((uint16_t)(tl[p] - 1));
+                if (pos) pos->push_back(((uint64_t)ch << 32) | (uint64_t)p);
+            }
+            c->doc_off.push_back((int64_t)c->words.size());
+        }
+        if (expect != Nc) {
+            err(dir + ": docCountOffset covers " + std::to_string(expect) + " of " + std::to_string(Nc) + " tokens");
+            delete c;
+            return nullptr;
+        }
+        c->D += (int32_t)Dc;
+    }
+    return c;
+}
+
+// lengthVec.txt's V, and TLLength.txt / docLength.txt of the folder.
+bool read_folder_sizes(const std::string &root, int32_t &num_chunks, int64_t &V, std::vector<int64_t> &tll,
+                       std::vector<int64_t> &dl) {
+    std::vector<int64_t> lv;
+    if (!read_ints(root + "/lengthVec.txt", lv, 8)) return false;
+    V = lv[4];
+    if (!read_ints(root + "/TLLength.txt", tll, (size_t)-1)) return false;
+    if (num_chunks <= 0) num_chunks = (int32_t)tll.size();
+    if ((int64_t)tll.size() != num_chunks) {
+        err(root + "/TLLength.txt: " + std::to_string(tll.size()) + " chunks, expected " + std::to_string(num_chunks));
+        return false;
+    }
+    if (!read_ints(root + "/docLength.txt", dl, (size_t)num_chunks)) return false;
+    if (V < 1 || V > 0x7fffffff) {
+        err(root + "/lengthVec.txt: bad vocabulary size");
+        return false;
+    }
+    for (int ch = 0; ch < num_chunks; ++ch)
+        if (tll[ch] < 0 || dl[ch] < 0 || tll[ch] > 0xFFFFFFFFll || dl[ch] > 0x7FFFFFFFll) {
+            err(root + ": chunk " + std::to_string(ch) + " has a negative or oversized token or document count");
+            return false;
+        }
+    return true;
+}
+
 }  // namespace
+
+namespace ezc {
+
+// The chunked folder at root: per doc-major token (the reader's order),
+// chunk << 32 | its position in the chunk's TL.txt, and its word (rank);
+// tll: tokens per chunk.  false with the error set.
+bool chunked_layout(const std::string &root, std::vector<uint64_t> &pos, std::vector<int32_t> &words,
+                    std::vector<int64_t> &tll) {
+    int32_t chunks = 0;
+    int64_t V = 0;
+    std::vector<int64_t> dl;
+    if (!read_folder_sizes(root, chunks, V, tll, dl)) return false;
+    ezlda_corpus *c = read_chunked_text(root, chunks, V, tll, dl, &pos);
+    if (!c) return false;
+    words = std::move(c->words);
+    delete c;
+    return true;
+}
+
+// Rewrites every chunk's TL.txt with the doc-major topics z (0-based; one per
+// entry of pos, as chunked_layout gives it), 1-based as the reference writes
+// them, and the folder's binary sidecar with the same topics when it was
+// current before.  false with the error set.
+bool rewrite_tl(const std::string &root, const std::vector<uint64_t> &pos, const std::vector<int64_t> &tll,
+                const uint16_t *z, int K) {
+    const int chunks = (int)tll.size();
+    int64_t N = 0, D = 0;
+    std::vector<int64_t> dl;
+    {
+        int32_t nc = chunks;
+        int64_t V = 0;
+        std::vector<int64_t> t2;
+        if (!read_folder_sizes(root, nc, V, t2, dl)) return false;
+        for (int ch = 0; ch < chunks; ++ch) {
+            N += tll[ch];
+            D += dl[ch];
+        }
+        ezlda_corpus *side = read_sidecar(root, chunks, V, D, N);
+        std::vector<std::vector<uint16_t>> tl(chunks);
+        for (int ch = 0; ch < chunks; ++ch) tl[ch].assign((size_t)tll[ch], 0);
+        for (size_t i = 0; i < pos.size(); ++i) {
+            const int ch = (int)(pos[i] >> 32);
+            const uint32_t p = (uint32_t)pos[i];
+            if (z[i] >= K) {
+                err("topic out of range at token " + std::to_string(i));
+                delete side;
+                return false;
+            }
+            tl[ch][p] = (uint16_t)(z[i] + 1);
+        }
+        for (int ch = 0; ch < chunks; ++ch) {
+            const std::string path = root + "/chunk" + std::to_string(ch) + "/TL.txt", tmp = path + ".tmp";
+            std::string buf;
+            buf.reserve(tl[ch].size() * 6);
+            char num[16];
+            for (const uint16_t t : tl[ch]) {
+                if (t == 0) {
+                    err(path + ": a position without a token");
+                    delete side;
+                    return false;
+                }
+                const int n = std::snprintf(num, sizeof num, "%u\n", (unsigned)t);
+                buf.append(num, (size_t)n);
+            }
+            std::FILE *f = std::fopen(tmp.c_str(), "wb");
+            const bool ok = f && std::fwrite(buf.data(), 1, buf.size(), f) == buf.size();
+            if (f && std::fclose(f) != 0) f = nullptr;
+            if (!ok || !f || std::rename(tmp.c_str(), path.c_str()) != 0) {
+                std::remove(tmp.c_str());
+                err("cannot write " + path);
+                delete side;
+                return false;
+            }
+        }
+        if (side) {
+            std::memcpy(side->topics.data(), z, sizeof(uint16_t) * (size_t)N);
+            const bool ok = write_sidecar(root, chunks, *side);
+            delete side;
+            if (!ok) return false;
+        }
+    }
+    return true;
+}
+
+}  // namespace ezc
 
 extern "C" {
 
-ezlda_corpus *ezlda_gen_corpus(const ezlda_gen_params *p) {
+ezlda_corpus *ezlda_gen_corpus(const ezlda_gen_params *p) try {
     if (!p || p->V < 1 || p->D < 0 || p->K_true < 1 || p->first_doc < 0) {
         err("bad generator parameters");
         return nullptr;
@@ -226,6 +504,9 @@ ezlda_corpus *ezlda_gen_corpus(const ezlda_gen_params *p) {
         }
     });
     return c;
+} catch (...) {
+    (void)ezc::exception_status(nullptr);
+    return nullptr;
 }
 
 int64_t ezlda_corpus_num_tokens(const ezlda_corpus *c) { return c ? c->doc_off.back() : 0; }
@@ -255,7 +536,7 @@ ezlda_corpus *ezlda_corpus_from_arrays(int32_t V, int32_t D, const int64_t *doc_
 
 // UCI docword: "D W NNZ" then NNZ lines "doc word count", 1-based, grouped
 // by doc (the input of preprocess/streamTestBalance.cpp:90-145).
-ezlda_corpus *ezlda_read_docword(const char *path) {
+ezlda_corpus *ezlda_read_docword(const char *path) try {
     std::ifstream f(path, std::ios::binary);
     if (!f.is_open()) {
         err(std::string("cannot open ") + path);
@@ -292,6 +573,9 @@ ezlda_corpus *ezlda_read_docword(const char *path) {
     for (size_t i = 0; i < trip.size(); ++i)
         for (int32_t k = 0; k < trip[i].second; ++k) c->words[cur[docs[i]]++] = trip[i].first;
     return c;
+} catch (...) {
+    (void)ezc::exception_status(nullptr);
+    return nullptr;
 }
 
 // Writes the chunked folder of preprocess/streamTestBalance.cpp:356-547.
@@ -300,7 +584,7 @@ ezlda_corpus *ezlda_read_docword(const char *path) {
 // off-by-one docCountVec[j+1] (:145) is not reproduced, and TL.txt holds the
 // given initial topics (or Philox draws) instead of unseeded rand() (:399).
 int ezlda_write_chunked(const char *prefix, const ezlda_corpus *c, const uint16_t *z0, int32_t K, int32_t num_chunks,
-                        uint64_t seed) {
+                        uint64_t seed) try {
     if (!c || !prefix || K < 1 || num_chunks < 1) {
         err("bad arguments to ezlda_write_chunked");
         return EZLDA_EINVAL;
@@ -337,6 +621,10 @@ int ezlda_write_chunked(const char *prefix, const ezlda_corpus *c, const uint16_
         nHead(num_chunks, 0), nTail(num_chunks, 0);
     std::vector<int64_t> gWT(V, 0);
     int64_t maxTL = 0, maxDocs = 0;
+    ezlda_corpus side;  // the corpus as ezlda_read_chunked returns it
+    side.V = V;
+    side.words.reserve((size_t)N);
+    side.topics.reserve((size_t)N);
     for (int ch = 0; ch < num_chunks; ++ch) {
         const std::string dir = root + "/chunk" + std::to_string(ch);
         if (!make_dir(dir)) return EZLDA_EIO;
@@ -364,6 +652,13 @@ int ezlda_write_chunked(const char *prefix, const ezlda_corpus *c, const uint16_
             }
             dcnt[ld] = (int64_t)toks.size() - doff[ld];
         }
+        // the reader's doc-major order of this chunk, for the binary sidecar
+        for (const Tok &t : toks) {
+            side.words.push_back(t.rank);
+            side.topics.push_back(t.topic);
+        }
+        for (size_t ld = 0; ld < docs.size(); ++ld) side.doc_off.push_back(side.doc_off.back() + dcnt[ld]);
+        side.D += (int32_t)docs.size();
         std::vector<Tok> wm(toks);
         std::stable_sort(wm.begin(), wm.end(), [](const Tok &a, const Tok &b) { return a.rank < b.rank; });
         std::vector<int64_t> w2d(toks.size());
@@ -454,81 +749,31 @@ int ezlda_write_chunked(const char *prefix, const ezlda_corpus *c, const uint16_
             return EZLDA_EIO;
         }
     }
+    if (!write_sidecar(root, num_chunks, side)) return EZLDA_EIO;
     return EZLDA_OK;
+} catch (...) {
+    return ezc::exception_status(nullptr);
 }
 
 // Reads the chunked folder back into a doc-major corpus (docs of chunk 0
 // first, local doc order inside a chunk; word ids are the ranks; topics from
-// TL.txt made 0-based).  num_chunks <= 0 takes the count from TLLength.txt.
-ezlda_corpus *ezlda_read_chunked(const char *prefix, int32_t num_chunks) {
+// TL.txt made 0-based), from its binary sidecar while that is current, else
+// from the text files.  num_chunks <= 0 takes the count from TLLength.txt.
+ezlda_corpus *ezlda_read_chunked(const char *prefix, int32_t num_chunks) try {
     const std::string root(prefix ? prefix : "");
-    std::vector<int64_t> lv;
-    if (!read_ints(root + "/lengthVec.txt", lv, 8)) return nullptr;
-    const int64_t V = lv[4];
+    int64_t V = 0;
     std::vector<int64_t> tll, dl;
-    if (!read_ints(root + "/TLLength.txt", tll, (size_t)-1)) return nullptr;
-    if (num_chunks <= 0) num_chunks = (int32_t)tll.size();
-    if ((int64_t)tll.size() != num_chunks) {
-        err(root + "/TLLength.txt: " + std::to_string(tll.size()) + " chunks, expected " + std::to_string(num_chunks));
-        return nullptr;
-    }
-    if (!read_ints(root + "/docLength.txt", dl, (size_t)num_chunks)) return nullptr;
-    if (V < 1 || V > 0x7fffffff) {
-        err(root + "/lengthVec.txt: bad vocabulary size");
-        return nullptr;
-    }
-    auto *c = new ezlda_corpus();
-    c->V = (int32_t)V;
+    if (!read_folder_sizes(root, num_chunks, V, tll, dl)) return nullptr;
+    int64_t N = 0, D = 0;
     for (int ch = 0; ch < num_chunks; ++ch) {
-        const std::string dir = root + "/chunk" + std::to_string(ch);
-        const int64_t Nc = tll[ch], Dc = dl[ch];
-        std::vector<int64_t> tl, w2d, wco, dco;
-        if (!read_ints(dir + "/TL.txt", tl, (size_t)Nc) || !read_ints(dir + "/word2DocMap.txt", w2d, (size_t)Nc) ||
-            !read_ints(dir + "/wordCountOffset.txt", wco, (size_t)(2 * V)) ||
-            !read_ints(dir + "/docCountOffset.txt", dco, (size_t)(2 * Dc))) {
-            delete c;
-            return nullptr;
-        }
-        // rank of each TL position
-        std::vector<int32_t> rank_at(Nc, -1);
-        for (int64_t r = 0; r < V; ++r) {
-            const int64_t n = wco[2 * r], o = wco[2 * r + 1];
-            if (n < 0 || o < 0 || o + n > Nc) {
-                err(dir + "/wordCountOffset.txt: range out of bounds at rank " + std::to_string(r));
-                delete c;
-                return nullptr;
-            }
-            for (int64_t i = o; i < o + n; ++i) rank_at[i] = (int32_t)r;
-        }
-        int64_t expect = 0;
-        for (int64_t ld = 0; ld < Dc; ++ld) {
-            const int64_t n = dco[2 * ld], o = dco[2 * ld + 1];
-            if (o != expect || n < 0 || o + n > Nc) {
-                err(dir + "/docCountOffset.txt: inconsistent entry " + std::to_string(ld));
-                delete c;
-                return nullptr;
-            }
-            expect += n;
-            for (int64_t i = o; i < o + n; ++i) {
-                const int64_t p = w2d[i];
-                if (p < 0 || p >= Nc || rank_at[p] < 0 || tl[p] < 1 || tl[p] > 65535) {
-                    err(dir + ": bad token " + std::to_string(i));
-                    delete c;
-                    return nullptr;
-                }
-                c->words.push_back(rank_at[p]);
-                c->topics.push_back((uint16_t)(tl[p] - 1));
-            }
-            c->doc_off.push_back((int64_t)c->words.size());
-        }
-        if (expect != Nc) {
-            err(dir + ": docCountOffset covers " + std::to_string(expect) + " of " + std::to_string(Nc) + " tokens");
-            delete c;
-            return nullptr;
-        }
-        c->D += (int32_t)Dc;
+        N += tll[ch];
+        D += dl[ch];
     }
-    return c;
+    if (ezlda_corpus *c = read_sidecar(root, num_chunks, V, D, N)) return c;
+    return read_chunked_text(root, num_chunks, V, tll, dl, nullptr);
+} catch (...) {
+    (void)ezc::exception_status(nullptr);
+    return nullptr;
 }
 
 }  // extern "C"

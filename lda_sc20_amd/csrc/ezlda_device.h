@@ -69,7 +69,17 @@ hipError_t launch_accumulate_wt(const DevState &s, hipStream_t st);
 hipError_t launch_topk(const DevState &s, hipStream_t st);
 hipError_t launch_build_dt(const DevState &s, hipStream_t st);
 hipError_t launch_z_to_doc(const DevState &s, hipStream_t st);
-hipError_t launch_sample(const DevState &s, bool shortcut, hipStream_t st);
+// Sampler stages (mode bits): the items of head words, of tail words, and
+// the max-topic shortcut alone (tokens it accepts get their top topic; the
+// others keep theirs for a later full draw).  A token's draw depends only on
+// the counts of the sweep, so the stages give the fused sampler's topics in
+// any order.
+static constexpr uint32_t kSampleHead = 1u, kSampleTail = 2u, kShortcutOnly = 4u;
+hipError_t launch_sample(const DevState &s, bool shortcut, hipStream_t st,
+                         uint32_t mode = kSampleHead | kSampleTail);
+// Dense head rows of the current z (zeroed first) and n_k alone.
+hipError_t launch_head_rows(const DevState &s, hipStream_t st);
+hipError_t launch_topic_hist(const DevState &s, hipStream_t st);
 hipError_t launch_advance_sweep(const DevState &s, hipStream_t st);
 hipError_t launch_loglik(const DevState &s, hipStream_t st);
 // Log-likelihood partials the allocation-free entry point can hold (one per
@@ -112,6 +122,11 @@ hipError_t launch_foldin_admit(const uint4 *adm, int nadm, const uint32_t *src, 
 // tokens, sweep); fin[j] = (output index or ~0, topic offset) for the
 // documents that finish with this sweep; ntok = their tokens.  nslots <=
 // 65535 documents of <= 65535 tokens.
+// The plain path: every sweep of one document (n tokens, word ranks) in one
+// launch of one workgroup; z receives its n topics.
+hipError_t launch_foldin_doc(const uint32_t *words, uint32_t n, uint32_t rid, int sweeps, int K, float alpha,
+                             const float *phi, const float *qs, const float *qacc, uint64_t seed, uint16_t *z,
+                             hipStream_t st);
 hipError_t launch_foldin_step(const FoldinBatch &fb, const uint4 *slots, const uint2 *fin, int nslots, uint32_t ntok,
                               int V, int K, float alpha, const float *phi, const float *qs, const float *qacc,
                               const uint32_t *wpool, uint16_t *zpool, uint64_t seed, hipStream_t st);

@@ -12,8 +12,9 @@
 // back of the finishing documents' counts and topics.  A request leaves
 // after `sweeps` sweeps; its pool range and slot are reused.  Draws are keyed
 // by (seed, request id, token, sweep), never by slot or position, so a result
-// is the same alone and in any mix.  ezlda_foldin_plain runs one document
-// per step on its own stream, the comparison path.
+// is the same alone and in any mix.  ezlda_foldin_plain, the comparison
+// path, runs one document per launch (all of its sweeps in one workgroup) on
+// its own stream.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -23,6 +24,8 @@
 #include <deque>
 #include <map>
 #include <mutex>
+#include <new>
+#include <stdexcept>
 #include <string>
 #include <thread>
 #include <unordered_map>
@@ -94,11 +97,9 @@ struct ezlda_server {
     uint32_t budget_tokens = 0, pool_tokens = 0, up_cap = 0;
     StepRec rec[2];     // two steps in flight: host buffers and finishing documents
     int64_t retiring = 0;  // documents done on the device, results not yet published
-    // plain path buffers
+    // plain path: one document's words (ranks) and topics
     uint32_t *pw = nullptr;
     uint16_t *pz = nullptr;
-    uint8_t *phdr = nullptr;
-    ez::FoldinBatch pfb;
     hipStream_t st = nullptr, pst = nullptr;
     std::vector<void *> allocs;
     // token pool allocator: free ranges offset -> length
@@ -130,6 +131,22 @@ int sfail(ezlda_server *s, int code, const std::string &m) {
     ezc::set_error(m);
     return code;
 }
+
+}  // namespace
+
+int ezc::server_exception_status(ezlda_server *s) {
+    try {
+        throw;
+    } catch (const std::bad_alloc &) {
+        return sfail(s, EZLDA_ENOMEM, "out of host memory");
+    } catch (const std::exception &e) {
+        return sfail(s, EZLDA_EINVAL, std::string("unexpected error: ") + e.what());
+    } catch (...) {
+        return sfail(s, EZLDA_EINVAL, "unexpected error");
+    }
+}
+
+namespace {
 
 #define SV_HIP(srv, call)                                                                   \
     do {                                                                                    \
@@ -394,7 +411,7 @@ bool alloc_batch(ezlda_server *s, ez::FoldinBatch *b, uint32_t tokens, int nslot
 
 extern "C" {
 
-ezlda_server *ezlda_server_create(ezlda_ctx *ctx, const ezlda_server_args *a) {
+ezlda_server *ezlda_server_create(ezlda_ctx *ctx, const ezlda_server_args *a) try {
     if (!ctx || !a || a->max_slots < 1 || a->max_slots > 65535 || a->sweeps < 1 ||
         a->hbm_budget_bytes < EZLDA_FOLDIN_TOKEN_BYTES) {
         ezc::set_error("bad fold-in server arguments (max_slots 1..65535, sweeps >= 1, budget >= one token)");
@@ -428,8 +445,7 @@ ezlda_server *ezlda_server_create(ezlda_ctx *ctx, const ezlda_server_args *a) {
          salloc(s, &s->wpool, s->pool_tokens) == EZLDA_OK && salloc(s, &s->zpool, s->pool_tokens) == EZLDA_OK &&
          salloc(s, &s->up_d, s->up_bytes) == EZLDA_OK &&
          alloc_batch(s, &s->fb, s->pool_tokens, s->max_slots, 8 * cus) &&
-         salloc(s, &s->pw, 65536) == EZLDA_OK && salloc(s, &s->pz, 65536) == EZLDA_OK &&
-         salloc(s, &s->phdr, up_layout(1, 0, 0).end) == EZLDA_OK && alloc_batch(s, &s->pfb, 65536, 1, 8 * cus);
+         salloc(s, &s->pw, 65536) == EZLDA_OK && salloc(s, &s->pz, 65536) == EZLDA_OK;
     for (StepRec &R : s->rec)
         ok = ok && hipHostMalloc((void **)&R.up_h, s->up_bytes, hipHostMallocDefault) == hipSuccess &&
              hipEventCreateWithFlags(&R.ev, hipEventDisableTiming) == hipSuccess;
@@ -445,6 +461,9 @@ ezlda_server *ezlda_server_create(ezlda_ctx *ctx, const ezlda_server_args *a) {
     s->free_ranges[0] = s->pool_tokens;
     s->worker = std::thread(worker_main, s);
     return s;
+} catch (...) {
+    (void)ezc::exception_status(ctx);
+    return nullptr;
 }
 
 void ezlda_server_destroy(ezlda_server *s) {
@@ -474,7 +493,7 @@ void ezlda_server_destroy(ezlda_server *s) {
 
 const char *ezlda_server_last_error(const ezlda_server *s) { return s ? s->err.c_str() : ezlda_last_error(nullptr); }
 
-int ezlda_server_submit(ezlda_server *s, uint32_t request_id, const int32_t *words, int32_t n) {
+int ezlda_server_submit(ezlda_server *s, uint32_t request_id, const int32_t *words, int32_t n) try {
     if (!s || n < 0 || (n > 0 && !words)) return sfail(s, EZLDA_EINVAL, "bad fold-in request");
     if (n > 65535) return sfail(s, EZLDA_EINVAL, "a fold-in document has more than 65535 tokens");
     Request rq;
@@ -499,10 +518,12 @@ int ezlda_server_submit(ezlda_server *s, uint32_t request_id, const int32_t *wor
     s->pending.push_back(std::move(rq));
     s->cv_work.notify_one();
     return EZLDA_OK;
+} catch (...) {
+    return ezc::server_exception_status(s);
 }
 
 int ezlda_server_submit_many(ezlda_server *s, int32_t n_docs, const uint32_t *request_ids, const int64_t *doc_off,
-                             const int32_t *words) {
+                             const int32_t *words) try {
     if (!s || n_docs < 0 || (n_docs > 0 && (!request_ids || !doc_off)) || (n_docs > 0 && doc_off[0] != 0))
         return sfail(s, EZLDA_EINVAL, "bad fold-in batch");
     std::vector<Request> rq((size_t)n_docs);
@@ -541,10 +562,12 @@ int ezlda_server_submit_many(ezlda_server *s, int32_t n_docs, const uint32_t *re
     s->submitted += n_docs;
     s->cv_work.notify_one();
     return EZLDA_OK;
+} catch (...) {
+    return ezc::server_exception_status(s);
 }
 
 int ezlda_server_wait(ezlda_server *s, uint32_t request_id, uint16_t *z_out, int32_t *counts_out, double *latency_ms,
-                      int32_t timeout_ms) {
+                      int32_t timeout_ms) try {
     if (!s) return sfail(nullptr, EZLDA_EINVAL, "null server");
     std::unique_lock<std::mutex> lk(s->mu);
     if (!s->done.count(request_id) && !s->inflight.count(request_id))
@@ -561,9 +584,11 @@ int ezlda_server_wait(ezlda_server *s, uint32_t request_id, uint16_t *z_out, int
     if (latency_ms) *latency_ms = it->second.latency_ms;
     s->done.erase(it);
     return EZLDA_OK;
+} catch (...) {
+    return ezc::server_exception_status(s);
 }
 
-int ezlda_server_drain(ezlda_server *s) {
+int ezlda_server_drain(ezlda_server *s) try {
     if (!s) return sfail(nullptr, EZLDA_EINVAL, "null server");
     std::unique_lock<std::mutex> lk(s->mu);
     s->cv_done.wait(lk, [&] {
@@ -571,9 +596,11 @@ int ezlda_server_drain(ezlda_server *s) {
         return s->pending.empty() && s->active.empty() && s->retiring == 0;
     });
     return s->worker_rc;
+} catch (...) {
+    return ezc::server_exception_status(s);
 }
 
-int ezlda_server_get_stats(ezlda_server *s, ezlda_server_stats *st) {
+int ezlda_server_get_stats(ezlda_server *s, ezlda_server_stats *st) try {
     if (!s || !st) return sfail(s, EZLDA_EINVAL, "null argument");
     std::lock_guard<std::mutex> lk(s->mu);
     std::memset(st, 0, sizeof(*st));
@@ -599,9 +626,11 @@ int ezlda_server_get_stats(ezlda_server *s, ezlda_server_stats *st) {
         if (s->t_last > s->t_first) st->docs_per_s = 1e3 * (double)s->completed / (s->t_last - s->t_first);
     }
     return EZLDA_OK;
+} catch (...) {
+    return ezc::server_exception_status(s);
 }
 
-int ezlda_server_reset_stats(ezlda_server *s) {
+int ezlda_server_reset_stats(ezlda_server *s) try {
     if (!s) return sfail(nullptr, EZLDA_EINVAL, "null server");
     std::lock_guard<std::mutex> lk(s->mu);
     s->lat.clear();
@@ -610,10 +639,12 @@ int ezlda_server_reset_stats(ezlda_server *s) {
     s->t_first = -1.0;
     s->t_last = 0.0;
     return EZLDA_OK;
+} catch (...) {
+    return ezc::server_exception_status(s);
 }
 
 int ezlda_foldin_plain(ezlda_server *s, uint32_t request_id, const int32_t *words, int32_t n, uint16_t *z_out,
-                       int32_t *counts_out) {
+                       int32_t *counts_out) try {
     if (!s || n < 0 || (n > 0 && !words)) return sfail(s, EZLDA_EINVAL, "bad fold-in request");
     if (n > 65535) return sfail(s, EZLDA_EINVAL, "a fold-in document has more than 65535 tokens");
     std::vector<uint32_t> r(n);
@@ -622,30 +653,21 @@ int ezlda_foldin_plain(ezlda_server *s, uint32_t request_id, const int32_t *word
         r[i] = (uint32_t)s->rank_of[words[i]];
     }
     std::lock_guard<std::mutex> lk(s->plain_mu);
+    // one document, one launch: every sweep runs inside k_fi_doc
     if (n) SV_HIP(s, hipMemcpyAsync(s->pw, r.data(), n * sizeof(uint32_t), hipMemcpyHostToDevice, s->pst));
-    const UpLayout L = up_layout(1, 0, 0);
-    struct {
-        uint4 slot;
-        uint2 fin;
-    } hdr;
-    static_assert(sizeof(hdr) >= sizeof(uint4) + sizeof(uint2), "slot header layout");
-    for (int sw = 0; sw < s->sweeps; ++sw) {
-        hdr.slot = make_uint4(request_id, 0u, (uint32_t)n, (uint32_t)sw);
-        hdr.fin = sw + 1 == s->sweeps ? make_uint2(0u, 0u) : make_uint2(0xFFFFFFFFu, 0u);
-        SV_HIP(s, hipMemcpyAsync(s->phdr, &hdr.slot, sizeof(uint4), hipMemcpyHostToDevice, s->pst));
-        SV_HIP(s, hipMemcpyAsync(s->phdr + L.fin, &hdr.fin, sizeof(uint2), hipMemcpyHostToDevice, s->pst));
-        SV_HIP(s, ez::launch_foldin_step(s->pfb, (const uint4 *)s->phdr, (const uint2 *)(s->phdr + L.fin), 1,
-                                         (uint32_t)n, s->V, s->K, s->alpha, s->phi, s->qs, s->qacc, s->pw, s->pz, s->seed, s->pst));
-        SV_HIP(s, hipStreamSynchronize(s->pst));
-    }
+    SV_HIP(s, ez::launch_foldin_doc(s->pw, (uint32_t)n, request_id, s->sweeps, s->K, s->alpha, s->phi, s->qs, s->qacc,
+                                    s->seed, s->pz, s->pst));
     std::vector<uint16_t> z((size_t)n);
-    if (n) SV_HIP(s, hipMemcpy(z.data(), s->pfb.zout, n * sizeof(uint16_t), hipMemcpyDeviceToHost));
+    if (n) SV_HIP(s, hipMemcpyAsync(z.data(), s->pz, n * sizeof(uint16_t), hipMemcpyDeviceToHost, s->pst));
+    SV_HIP(s, hipStreamSynchronize(s->pst));
     if (n && z_out) std::memcpy(z_out, z.data(), n * sizeof(uint16_t));
     if (counts_out) {
         std::memset(counts_out, 0, sizeof(int32_t) * (size_t)s->K);
         for (const uint16_t t : z) counts_out[t]++;
     }
     return EZLDA_OK;
+} catch (...) {
+    return ezc::server_exception_status(s);
 }
 
 }  // extern "C"

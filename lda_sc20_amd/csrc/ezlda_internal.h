@@ -69,6 +69,20 @@ struct ezlda_ctx {
 
 namespace ezc {
 void set_error(const std::string &m);
+// Inside a catch block: the status for the exception in flight (std::bad_alloc
+// -> EZLDA_ENOMEM, anything else -> EZLDA_EINVAL), with its message recorded
+// for ezlda_last_error(c) (c may be null) or ezlda_server_last_error(s).  No
+// exception leaves an extern "C" entry point.
+int exception_status(ezlda_ctx *c);
+int server_exception_status(ezlda_server *s);
 // Rebuilds ctx's counts from its current z and 1/(n_k + V beta), synchronously.
 int refresh_counts(ezlda_ctx *c);
+// The chunked folder at root (ezlda_corpus.cpp): per doc-major token,
+// chunk << 32 | its TL.txt position, and its word (rank); tokens per chunk.
+bool chunked_layout(const std::string &root, std::vector<uint64_t> &pos, std::vector<int32_t> &words,
+                    std::vector<int64_t> &tll);
+// Rewrites the folder's TL.txt files (and a current binary sidecar) with the
+// doc-major topics z (0-based, < K).
+bool rewrite_tl(const std::string &root, const std::vector<uint64_t> &pos, const std::vector<int64_t> &tll,
+                const uint16_t *z, int K);
 }  // namespace ezc

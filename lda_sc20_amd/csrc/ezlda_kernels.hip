@@ -22,6 +22,12 @@
 #ifndef EZ_SAMPLE_CP
 #define EZ_SAMPLE_CP 12
 #endif
+// Largest K whose sampler keeps a table of the Q running sums inside each
+// 64-topic slab in LDS (a Q draw then bisects it); above it a Q draw
+// recomputes its slab's running sums, which halves the sampler's LDS.
+#ifndef EZ_Q_TABLE_MAX_K
+#define EZ_Q_TABLE_MAX_K 1024
+#endif
 
 namespace ez {
 
@@ -595,6 +601,7 @@ struct SampleArgs {
     uint32_t tb;  // compact rows: topic bits
     unsigned long long *stats;
     uint64_t seed;
+    uint32_t mode;  // kSampleHead | kSampleTail: which words' items; | kShortcutOnly: no full draws
 };
 
 template <bool SHORTCUT, bool COMPACT>
@@ -603,13 +610,15 @@ __device__ __forceinline__ void sample_item(const SampleArgs &A, uint32_t item, 
     const int K = A.K, NS = (K + 63) >> 6;
     const float alpha = A.alpha, beta = A.beta;
     const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6, nwv = blockDim.x >> 6;
-    float *phi = smf;              // NS*64
-    float *qs = phi + NS * 64;     // NS*64: running sums inside each slab
-    float *qt = qs + NS * 64;      // NS: slab totals
-    float *qacc = qt + NS;         // NS: running sum of the slab totals
+    const bool qtab = K <= EZ_Q_TABLE_MAX_K;  // uniform
+    float *phi = smf;                         // NS*64
+    float *qs = phi + NS * 64;                // NS*64 when qtab: running sums inside each slab
+    float *qt = qs + (qtab ? NS * 64 : 0);    // NS: slab totals
+    float *qacc = qt + NS;                    // NS: running sum of the slab totals
     const uint4 sg = A.segs[item];
     if (sg.x == 0xFFFFFFFFu) return;  // padding item (uniform over the block)
     const int w = (int)sg.x;
+    if (!(A.mode & (w < A.VD ? kSampleHead : kSampleTail))) return;  // the other words' stage
     const ushort4 tp = A.wtop[w];
 
     if (w < A.VD) {
@@ -634,12 +643,13 @@ __device__ __forceinline__ void sample_item(const SampleArgs &A, uint32_t item, 
         phi[tp.x] = 0.0f;
     }
     __syncthreads();
-    // Q running sums inside each 64-topic slab, left to right (phi is 0 past
-    // K, which leaves the sum unchanged): the slab's 64 values are loaded at
-    // once, then summed in order
+    // Q: the running sums inside each 64-topic slab, left to right (phi is 0
+    // past K, which leaves the sum unchanged); the slab's 64 values are
+    // loaded at once, then summed in order.  With qtab the running sums are
+    // kept; else only the slab total, and a Q draw recomputes its slab's
+    // running sums in the same order
     for (int j = tid; j < NS; j += blockDim.x) {
         const float4 *p4 = reinterpret_cast<const float4 *>(phi + j * 64);
-        float4 *q4 = reinterpret_cast<float4 *>(qs + j * 64);
         float4 v[16];
 #pragma unroll
         for (int i = 0; i < 16; ++i) v[i] = p4[i];
@@ -655,8 +665,11 @@ __device__ __forceinline__ void sample_item(const SampleArgs &A, uint32_t item, 
             run = run + alpha * v[i].w;
             v[i].w = run;
         }
+        if (qtab) {
+            float4 *q4 = reinterpret_cast<float4 *>(qs + j * 64);
 #pragma unroll
-        for (int i = 0; i < 16; ++i) q4[i] = v[i];
+            for (int i = 0; i < 16; ++i) q4[i] = v[i];
+        }
         qt[j] = run;
     }
     __syncthreads();
@@ -711,7 +724,7 @@ __device__ __forceinline__ void sample_item(const SampleArgs &A, uint32_t item, 
                 if (head) A.z_doc[p0 + j] = (uint16_t)k1;
             }
         }
-        if (!need) continue;
+        if (!need || (A.mode & kShortcutOnly)) continue;
         // the row: 32-bit entries, or 16-bit entries at a 16-bit offset
         const uint4 *rp = COMPACT ? reinterpret_cast<const uint4 *>(reinterpret_cast<const uint16_t *>(A.dt) + di.x)
                                   : reinterpret_cast<const uint4 *>(A.dt + di.x);
@@ -838,11 +851,33 @@ __device__ __forceinline__ void sample_item(const SampleArgs &A, uint32_t item, 
                 if (js >= 0) {
                     const float t3 = t2 - (js > 0 ? qacc[js - 1] : 0.0f);
                     const int k0 = js * 64, ke = min(k0 + 64, K);
-                    lo = k0;
-                    hi = ke;
-                    while (lo < hi) {
-                        const int mid = (lo + hi) >> 1;
-                        if (qs[mid] > t3) hi = mid; else lo = mid + 1;
+                    lo = ke;
+                    if (qtab) {  // first running sum above t3, by bisection
+                        lo = k0;
+                        hi = ke;
+                        while (lo < hi) {
+                            const int mid = (lo + hi) >> 1;
+                            if (qs[mid] > t3) hi = mid; else lo = mid + 1;
+                        }
+                    }
+                    // else the slab's running sums, recomputed in the order
+                    // its total was summed: the first above t3
+                    const float4 *p4 = reinterpret_cast<const float4 *>(phi + k0);
+                    float run = 0.0f;
+                    for (int i4 = 0; i4 < 16 && !qtab && lo == ke; i4 += 4) {
+                        float4 v[4];
+#pragma unroll
+                        for (int u = 0; u < 4; ++u) v[u] = p4[i4 + u];
+#pragma unroll
+                        for (int u = 0; u < 4; ++u) {
+                            const float x[4] = {v[u].x, v[u].y, v[u].z, v[u].w};
+#pragma unroll
+                            for (int c = 0; c < 4; ++c) {
+                                const int k = k0 + 4 * (i4 + u) + c;
+                                run = run + alpha * x[c];
+                                if (lo == ke && k < ke && run > t3) lo = k;
+                            }
+                        }
                     }
                     if (lo < ke) {
                         newz = (uint32_t)lo;
@@ -872,7 +907,7 @@ __global__ void __launch_bounds__(1024) k_sample(const SampleArgs A) {
     // low half is directly the LDS address of its phi value
     extern __shared__ float smf[];
     const int NS = (A.K + 63) >> 6;
-    float *f123 = smf + 2 * NS * 64 + 2 * NS;  // after phi, qs, qt, qacc
+    float *f123 = smf + (A.K <= EZ_Q_TABLE_MAX_K ? 2 : 1) * NS * 64 + 2 * NS;  // after phi, qs, qt, qacc
     uint32_t &s_item = *reinterpret_cast<uint32_t *>(f123 + 3);
     const uint32_t sweep = A.counters[0];
     uint32_t n_full = 0, n_scanned = 0;
@@ -907,8 +942,9 @@ __global__ void __launch_bounds__(1024) k_sample(const SampleArgs A) {
 // ------------------------------------------------ FP64 log-likelihood
 // sum_t log( sum_k (n_dk + alpha)/(n_d + K alpha) * (n_wk + beta)/(n_k + V beta) )
 // per token, the term of profilePerp/utility.cu:2398,2810 reduced in FP64
-// (profilePerp/utility.cu:2858-2928).  One workgroup per sampler item; the
-// per-item partials are summed in fixed order by k_ll_final.
+// (profilePerp/utility.cu:2858-2928).  One workgroup per sampler item (phi_w
+// in FP64 in LDS), one lane per run; the per-item partials are summed in
+// fixed order by k_ll_final.
 __global__ void __launch_bounds__(1024) k_ll(const uint4 *__restrict__ segs, const uint4 *__restrict__ runs, int VD,
                                              int K, double alpha, double beta,
                                              double vbeta, const int32_t *__restrict__ wt_head,
@@ -951,27 +987,41 @@ __global__ void __launch_bounds__(1024) k_ll(const uint4 *__restrict__ segs, con
     for (int i = 0; i < nwv; ++i) q += red[i];
     q *= alpha;
     __syncthreads();
+    // one lane per run (a word's tokens in one doc share the term): its DT
+    // row in 16-byte loads (rows are 128-byte aligned and zero padded; a zero
+    // entry adds nothing)
     double acc = 0.0;
-    for (uint32_t r = sg.y + (uint32_t)wv; r < sg.z; r += (uint32_t)nwv) {
-        const uint4 rn = runs[r];  // rn.w tokens of this word in doc rn.z share the term
+    const double kalpha = (double)K * alpha;
+    const uint32_t tmask = (1u << tb) - 1u;
+    for (uint32_t r = sg.y + (uint32_t)tid; r < sg.z; r += (uint32_t)blockDim.x) {
+        const uint4 rn = runs[r];
         const uint2 di = doc_info[rn.z];
         const uint32_t nnz = di.y & 0xFFFFu;
         double s = 0.0;
         if (compact) {
-            const uint16_t *r16 = reinterpret_cast<const uint16_t *>(dt) + di.x;
-            for (uint32_t i = lane; i < nnz; i += 64) {
-                const uint32_t e = r16[i];
-                s += (double)(e >> tb) * phi[e & ((1u << tb) - 1u)];
+            const uint4 *rp = reinterpret_cast<const uint4 *>(reinterpret_cast<const uint16_t *>(dt) + di.x);
+            for (uint32_t i = 0; i < (nnz + 7u) / 8u; ++i) {
+                const uint4 v = rp[i];
+                const uint32_t e[4] = {v.x, v.y, v.z, v.w};
+#pragma unroll
+                for (int u = 0; u < 4; ++u) {
+                    const uint32_t a = e[u] & 0xFFFFu, b = e[u] >> 16;
+                    s += (double)(a >> tb) * phi[a & tmask];
+                    s += (double)(b >> tb) * phi[b & tmask];
+                }
             }
         } else {
-            for (uint32_t i = lane; i < nnz; i += 64) {
-                const uint32_t e = dt[di.x + i];
-                s += (double)(e >> 16) * phi[dt_topic(e)];
+            const uint4 *rp = reinterpret_cast<const uint4 *>(dt + di.x);
+            for (uint32_t i = 0; i < (nnz + 3u) / 4u; ++i) {
+                const uint4 v = rp[i];
+                const uint32_t e[4] = {v.x, v.y, v.z, v.w};
+#pragma unroll
+                for (int u = 0; u < 4; ++u) s += (double)(e[u] >> 16) * phi[dt_topic(e[u])];
             }
         }
-        for (int off = 32; off >= 1; off >>= 1) s += __shfl_xor(s, off);
-        acc += (double)rn.w * log((s + q) / ((double)(di.y >> 16) + (double)K * alpha));
+        acc += (double)rn.w * log((s + q) / ((double)(di.y >> 16) + kalpha));
     }
+    for (int off = 32; off >= 1; off >>= 1) acc += __shfl_xor(acc, off);
     if (lane == 0) red[wv] = acc;
     __syncthreads();
     if (tid == 0) {
@@ -1508,6 +1558,70 @@ __global__ void __launch_bounds__(256) k_fi_sample(const uint4 *__restrict__ slo
     }
 }
 
+// The plain fold-in path: one launch runs every sweep of one document.  One
+// workgroup: the document's topics live in z (device memory, n values), its
+// 16-bit topic histogram and its row (topic | count << 16, ascending topic,
+// zero padded to a multiple of kFB) in LDS.  Each sweep rebuilds the row from
+// z, then every token is drawn against it (Jacobi: the row is complete before
+// any token of the sweep moves) with the draw the batched step uses, so the
+// topics are the server's and the oracle's.
+static size_t foldin_doc_lds(int K) {
+    const size_t kh4 = (((size_t)K + 1) / 2 + 3) & ~(size_t)3;  // 16-byte aligned row after the counts
+    return (kh4 + (((size_t)K + 7) & ~(size_t)7) + 8 + 4) * sizeof(uint32_t);
+}
+
+__global__ void __launch_bounds__(1024) k_fi_doc(const uint32_t *__restrict__ words, uint32_t n, uint32_t rid,
+                                                 int sweeps, int K, float alpha, const float *__restrict__ phi,
+                                                 const float *__restrict__ qs, const float *__restrict__ qacc,
+                                                 uint64_t seed, uint16_t *__restrict__ z) {
+    extern __shared__ uint32_t shd[];
+    const int NS = (K + 63) >> 6, tid = threadIdx.x, bd = blockDim.x, lane = tid & 63, wv = tid >> 6;
+    const int KH = (K + 1) >> 1, KH4 = (KH + 3) & ~3;
+    uint32_t *h = shd;                                   // KH words: 16-bit counts
+    uint32_t *row = shd + KH4;                           // the row, 16-byte aligned
+    uint32_t *s_nz = row + ((K + 7) & ~7) + 8;           // the row's entries
+    for (uint32_t i = tid; i < n; i += bd)
+        z[i] = (uint16_t)(ez_philox_x0(seed, ((uint64_t)rid << 32) | i, 0u, EZ_TAG_FOLDIN_INIT) % (uint32_t)K);
+    for (int k = tid; k < KH; k += bd) h[k] = 0;
+    __syncthreads();
+    for (int sw = 0; sw < sweeps; ++sw) {
+        for (uint32_t i = tid; i < n; i += bd) h16_add(h, z[i]);
+        __syncthreads();
+        if (wv == 0) {  // one wavefront compacts the row in ascending topic order and clears the counts
+            uint32_t nz = 0;
+            for (int c0 = 0; c0 < K; c0 += 64) {
+                const int k = c0 + lane;
+                const uint32_t c = k < K ? h16_get(h, (uint32_t)k) : 0u;
+                const uint64_t m = __ballot(c != 0);
+                if (c) row[nz + __popcll(m & lanemask_lt(lane))] = (uint32_t)k | (c << 16);
+                nz += (uint32_t)__popcll(m);
+                wave_lds_sync();
+                if (k < K && !(k & 1)) h[k >> 1] = 0;
+            }
+            // zero entries up to the next block: a count of 0 adds exactly nothing
+            if ((uint32_t)lane < ((kFB - (nz % kFB)) % kFB)) row[nz + lane] = 0u;
+            if (lane == 0) *s_nz = nz;
+        }
+        __syncthreads();
+        const uint32_t nz = *s_nz;
+        const uint4 sl = make_uint4(rid, 0u, n, (uint32_t)sw);
+        for (uint32_t i = tid; i < n; i += bd) {
+            const size_t w = words[i];
+            fi_draw(sl, i, nz, row, phi + w * K, qacc + w * NS, qs + w * K, K, NS, alpha, seed, z);
+        }
+        __syncthreads();  // every topic of the sweep is written before the next histogram
+    }
+}
+
+hipError_t launch_foldin_doc(const uint32_t *words, uint32_t n, uint32_t rid, int sweeps, int K, float alpha,
+                             const float *phi, const float *qs, const float *qacc, uint64_t seed, uint16_t *z,
+                             hipStream_t st) {
+    if (n == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_fi_doc, dim3(1), dim3(1024), foldin_doc_lds(K), st, words, n, rid, sweeps, K, alpha, phi, qs,
+                       qacc, seed, z);
+    return hipGetLastError();
+}
+
 // Documents that just drew their last sweep (fin[j] = (output index, topic
 // offset), output index ~0 for the others): their topics packed into zout at
 // the offset, for one copy to the host (which counts them).
@@ -1681,6 +1795,25 @@ hipError_t launch_accumulate_wt(const DevState &s, hipStream_t st) {
     return hipGetLastError();
 }
 
+hipError_t launch_head_rows(const DevState &s, hipStream_t st) {
+    const size_t nh = (size_t)s.VD * s.K;
+    if (nh) hipLaunchKernelGGL(k_fill_u32, dim3(grid_for(nh, 256, 4096)), dim3(256), 0, st,
+                               reinterpret_cast<uint32_t *>(s.wt_head), nh, 0u);
+    if (s.n_wt_segs)
+        hipLaunchKernelGGL(k_wt_head, dim3(s.n_wt_segs), dim3(256), (size_t)s.K * sizeof(uint32_t), st, s.z, s.wt_segs,
+                           s.K, s.wt_head);
+    return hipGetLastError();
+}
+
+hipError_t launch_topic_hist(const DevState &s, hipStream_t st) {
+    hipLaunchKernelGGL(k_fill_u32, dim3(grid_for(s.K, 256, 64)), dim3(256), 0, st, reinterpret_cast<uint32_t *>(s.nk),
+                       (size_t)s.K, 0u);
+    if (s.N)
+        hipLaunchKernelGGL(k_topic_hist, dim3(grid_for(s.N, 256 * 16, 2048)), dim3(256), (size_t)s.K * sizeof(uint32_t),
+                           st, s.z, s.N, s.K, s.nk);
+    return hipGetLastError();
+}
+
 hipError_t launch_topk(const DevState &s, hipStream_t st) {
     hipLaunchKernelGGL(k_inv, dim3(grid_for(s.K, 256, 1 << 16)), dim3(256), 0, st, s.nk, s.K, s.vbeta, s.inv);
     if (s.V > s.VD)
@@ -1708,7 +1841,8 @@ hipError_t launch_build_dt(const DevState &s, hipStream_t st) {
 size_t sample_lds_bytes(int K, int bd) {
     const int NS = (K + 63) >> 6;
     (void)bd;
-    return (2 * (size_t)NS * 64 + 2 * (size_t)NS + 4) * sizeof(float);  // + f123[3], item
+    const size_t tab = K <= EZ_Q_TABLE_MAX_K ? 2 : 1;  // phi (and the slab running sums)
+    return (tab * (size_t)NS * 64 + 2 * (size_t)NS + 4) * sizeof(float);  // + qt, qacc, f123[3], item
 }
 
 size_t sampler_static_lds() {
@@ -1731,10 +1865,11 @@ size_t max_lds_bytes(int K, int bd) {
     m = std::max(m, (size_t)K * sizeof(double));                     // log-likelihood
     m = std::max(m, (size_t)K * sizeof(float) + 1024 * sizeof(float));  // fold-in tables
     m = std::max(m, foldin_lds_bytes(K));
+    m = std::max(m, foldin_doc_lds(K));
     return m;
 }
 
-hipError_t launch_sample(const DevState &s, bool shortcut, hipStream_t st) {
+hipError_t launch_sample(const DevState &s, bool shortcut, hipStream_t st, uint32_t mode) {
     if (s.n_smp_segs == 0) return hipSuccess;
     const int bd = s.block_dim;
     const size_t lds = sample_lds_bytes(s.K, bd);
@@ -1763,6 +1898,7 @@ hipError_t launch_sample(const DevState &s, bool shortcut, hipStream_t st) {
     A.stats = s.stats;
     A.seed = s.seed;
     A.tb = s.topic_bits;
+    A.mode = mode;
     unsigned grid = (unsigned)s.n_smp_segs;
     if (A.n_queues) {
         // queue heads start at zero (a fill kernel, so a captured sweep

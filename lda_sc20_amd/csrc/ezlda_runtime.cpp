@@ -9,6 +9,8 @@
 #include <cmath>
 #include <cstring>
 #include <mutex>
+#include <new>
+#include <stdexcept>
 #include <string>
 #include <thread>
 #include <vector>
@@ -146,6 +148,22 @@ int fail(ezlda_ctx *c, int code, const std::string &msg) {
     g_err = msg;
     return code;
 }
+
+}  // namespace
+
+int ezc::exception_status(ezlda_ctx *c) {
+    try {
+        throw;
+    } catch (const std::bad_alloc &) {
+        return fail(c, EZLDA_ENOMEM, "out of host memory");
+    } catch (const std::exception &e) {
+        return fail(c, EZLDA_EINVAL, std::string("unexpected error: ") + e.what());
+    } catch (...) {
+        return fail(c, EZLDA_EINVAL, "unexpected error");
+    }
+}
+
+namespace {
 
 #define EZ_TRY_HIP(ctx, call)                                                                    \
     do {                                                                                         \
@@ -755,7 +773,7 @@ void ezlda_default_args(ezlda_args *a) {
 
 const char *ezlda_last_error(const ezlda_ctx *ctx) { return ctx ? ctx->err.c_str() : g_err.c_str(); }
 
-ezlda_ctx *ezlda_create(const ezlda_args *args) {
+ezlda_ctx *ezlda_create(const ezlda_args *args) try {
     if (!args) {
         fail(nullptr, EZLDA_EINVAL, "null args");
         return nullptr;
@@ -769,7 +787,7 @@ ezlda_ctx *ezlda_create(const ezlda_args *args) {
     if (a.beta <= 0.f) a.beta = 0.01f;
     if (a.topic_bits == 0) a.topic_bits = 16;
     if (a.topic_bits != 16) {
-        fail(nullptr, EZLDA_EINVAL, "only 16-bit topics are supported");
+        fail(nullptr, EZLDA_EINVAL, "topic_bits must be 16: topics are stored in 16 bits (K <= 16383)");
         return nullptr;
     }
     int ndev = 0;
@@ -819,6 +837,9 @@ ezlda_ctx *ezlda_create(const ezlda_args *args) {
     // a failed lookup only disables ezlda_loglik_noalloc, which reports it
     scratch_for_current_device(&c->ll_scratch, &c->scratch_err);
     return c;
+} catch (...) {
+    (void)ezc::exception_status(nullptr);
+    return nullptr;
 }
 
 void ezlda_destroy(ezlda_ctx *c) {
@@ -832,7 +853,7 @@ void ezlda_destroy(ezlda_ctx *c) {
 }
 
 int ezlda_load_docs(ezlda_ctx *c, int32_t V, int32_t D, const int64_t *doc_off, const int32_t *words,
-                    const uint16_t *z0) {
+                    const uint16_t *z0) try {
     if (!c) return fail(nullptr, EZLDA_EINVAL, "null context");
     if (V < 1 || D < 0 || !doc_off || (!words && D > 0 && doc_off[D] > 0))
         return fail(c, EZLDA_EINVAL, "bad corpus arguments");
@@ -1019,9 +1040,11 @@ int ezlda_load_docs(ezlda_ctx *c, int32_t V, int32_t D, const int64_t *doc_off, 
     c->counts_fresh = false;
     c->loaded = true;
     return EZLDA_OK;
+} catch (...) {
+    return ezc::exception_status(c);
 }
 
-int ezlda_load_chunked(ezlda_ctx *c, const char *prefix, int32_t num_chunks) {
+int ezlda_load_chunked(ezlda_ctx *c, const char *prefix, int32_t num_chunks) try {
     if (!c) return fail(nullptr, EZLDA_EINVAL, "null context");
     ezlda_corpus *cp = ezlda_read_chunked(prefix, num_chunks);
     if (!cp) return fail(c, EZLDA_EIO, g_err);
@@ -1029,9 +1052,11 @@ int ezlda_load_chunked(ezlda_ctx *c, const char *prefix, int32_t num_chunks) {
                                   ezlda_corpus_words(cp), ezlda_corpus_topics(cp));
     ezlda_corpus_free(cp);
     return r;
+} catch (...) {
+    return ezc::exception_status(c);
 }
 
-int ezlda_get_info(ezlda_ctx *c, ezlda_info *info) {
+int ezlda_get_info(ezlda_ctx *c, ezlda_info *info) try {
     if (int r = need_loaded(c)) return r;
     info->num_tokens = c->s.N;
     info->num_words = c->s.V;
@@ -1052,34 +1077,42 @@ int ezlda_get_info(ezlda_ctx *c, ezlda_info *info) {
     info->num_runs = c->s.n_runs;
     info->num_chunks = is_chunked(c) ? (int32_t)c->chunks.size() : 1;
     return EZLDA_OK;
+} catch (...) {
+    return ezc::exception_status(c);
 }
 
-int ezlda_build_wt(ezlda_ctx *c) {
+int ezlda_build_wt(ezlda_ctx *c) try {
     if (int r = need_loaded(c)) return r;
     if (is_chunked(c)) return refuse_chunked(c, "ezlda_build_wt");
     EZ_TRY_HIP(c, ez::launch_build_wt(c->s, c->st));
     EZ_TRY_HIP(c, hipStreamSynchronize(c->st));
     c->counts_fresh = true;
     return EZLDA_OK;
+} catch (...) {
+    return ezc::exception_status(c);
 }
 
-int ezlda_topk(ezlda_ctx *c) {
+int ezlda_topk(ezlda_ctx *c) try {
     if (int r = need_loaded(c)) return r;
     if (is_chunked(c)) return refuse_chunked(c, "ezlda_topk");
     EZ_TRY_HIP(c, ez::launch_topk(c->s, c->st));
     EZ_TRY_HIP(c, hipStreamSynchronize(c->st));
     return EZLDA_OK;
+} catch (...) {
+    return ezc::exception_status(c);
 }
 
-int ezlda_build_dt(ezlda_ctx *c) {
+int ezlda_build_dt(ezlda_ctx *c) try {
     if (int r = need_loaded(c)) return r;
     if (is_chunked(c)) return refuse_chunked(c, "ezlda_build_dt");
     EZ_TRY_HIP(c, ez::launch_build_dt(c->s, c->st));
     EZ_TRY_HIP(c, hipStreamSynchronize(c->st));
     return EZLDA_OK;
+} catch (...) {
+    return ezc::exception_status(c);
 }
 
-int ezlda_sample(ezlda_ctx *c) {
+int ezlda_sample(ezlda_ctx *c) try {
     if (int r = need_loaded(c)) return r;
     if (is_chunked(c)) return refuse_chunked(c, "ezlda_sample");
     EZ_TRY_HIP(c, ez::launch_sample(c->s, c->a.shortcut != 0, c->st));
@@ -1088,6 +1121,50 @@ int ezlda_sample(ezlda_ctx *c) {
     c->sweep++;
     c->counts_fresh = false;
     return EZLDA_OK;
+} catch (...) {
+    return ezc::exception_status(c);
+}
+
+// The reference's launcher-sized stages of the sampler and the counts
+// (SamplingKernel.cuh:10-15, WTDenUpdateKernel.cuh:25-26).  They run the
+// fused kernels on a part of the work, so any order of shortcut, sample_head
+// and sample_tail followed by commit_head gives ezlda_sample's topics.
+static int sample_stage(ezlda_ctx *c, uint32_t mode, const char *what) {
+    if (int r = need_loaded(c)) return r;
+    if (is_chunked(c)) return refuse_chunked(c, what);
+    EZ_TRY_HIP(c, ez::launch_sample(c->s, c->a.shortcut != 0, c->st, mode));
+    EZ_TRY_HIP(c, hipStreamSynchronize(c->st));
+    c->counts_fresh = false;
+    return EZLDA_OK;
+}
+
+int ezlda_shortcut(ezlda_ctx *c) { return sample_stage(c, ez::kSampleHead | ez::kShortcutOnly, "ezlda_shortcut"); }
+
+int ezlda_sample_head(ezlda_ctx *c) { return sample_stage(c, ez::kSampleHead, "ezlda_sample_head"); }
+
+int ezlda_sample_tail(ezlda_ctx *c) { return sample_stage(c, ez::kSampleTail, "ezlda_sample_tail"); }
+
+int ezlda_commit_head(ezlda_ctx *c) try {
+    if (int r = need_loaded(c)) return r;
+    if (is_chunked(c)) return refuse_chunked(c, "ezlda_commit_head");
+    EZ_TRY_HIP(c, ez::launch_head_rows(c->s, c->st));
+    EZ_TRY_HIP(c, ez::launch_advance_sweep(c->s, c->st));
+    EZ_TRY_HIP(c, hipStreamSynchronize(c->st));
+    c->sweep++;
+    c->counts_fresh = false;
+    return EZLDA_OK;
+} catch (...) {
+    return ezc::exception_status(c);
+}
+
+int ezlda_wt_rowsum(ezlda_ctx *c) try {
+    if (int r = need_loaded(c)) return r;
+    if (is_chunked(c)) return refuse_chunked(c, "ezlda_wt_rowsum");
+    EZ_TRY_HIP(c, ez::launch_topic_hist(c->s, c->st));
+    EZ_TRY_HIP(c, hipStreamSynchronize(c->st));
+    return EZLDA_OK;
+} catch (...) {
+    return ezc::exception_status(c);
 }
 
 // One sweep in stage order; ev (nullable) receives EZLDA_NUM_STAGES + 1
@@ -1106,7 +1183,7 @@ static int enqueue_sweep(ezlda_ctx *c, hipEvent_t *ev) {
     return EZLDA_OK;
 }
 
-int ezlda_sweep(ezlda_ctx *c, int32_t n) {
+int ezlda_sweep(ezlda_ctx *c, int32_t n) try {
     if (int r = need_loaded(c)) return r;
     if (is_chunked(c)) return chunk_sweeps(c, n, nullptr, nullptr);
     for (int i = 0; i < n; ++i) {
@@ -1116,6 +1193,8 @@ int ezlda_sweep(ezlda_ctx *c, int32_t n) {
     EZ_TRY_HIP(c, hipStreamSynchronize(c->st));
     c->counts_fresh = false;
     return EZLDA_OK;
+} catch (...) {
+    return ezc::exception_status(c);
 }
 
 static int ensure_graph(ezlda_ctx *c) {
@@ -1134,7 +1213,7 @@ static int ensure_graph(ezlda_ctx *c) {
     return EZLDA_OK;
 }
 
-int ezlda_sweep_graph(ezlda_ctx *c, int32_t n) {
+int ezlda_sweep_graph(ezlda_ctx *c, int32_t n) try {
     if (int r = need_loaded(c)) return r;
     if (is_chunked(c)) return refuse_chunked(c, "ezlda_sweep_graph");
     if (int r = ensure_graph(c)) return r;
@@ -1145,6 +1224,8 @@ int ezlda_sweep_graph(ezlda_ctx *c, int32_t n) {
     EZ_TRY_HIP(c, hipStreamSynchronize(c->st));
     c->counts_fresh = false;
     return EZLDA_OK;
+} catch (...) {
+    return ezc::exception_status(c);
 }
 
 // Back-to-back sweeps with no host synchronisation inside the timed region.
@@ -1178,7 +1259,7 @@ void layout_inputs(ezlda_ctx *c, std::vector<uint32_t> &tok_doc, std::vector<uin
 
 }  // namespace
 
-int ezlda_set_layout(ezlda_ctx *c, const ezlda_args *args) {
+int ezlda_set_layout(ezlda_ctx *c, const ezlda_args *args) try {
     if (int r = need_loaded(c)) return r;
     if (is_chunked(c)) return refuse_chunked(c, "ezlda_set_layout");
     if (!args) return fail(c, EZLDA_EINVAL, "null args");
@@ -1196,10 +1277,12 @@ int ezlda_set_layout(ezlda_ctx *c, const ezlda_args *args) {
     layout_inputs(c, tok_doc, tok_slot, slot_wrd);
     c->a = a;
     return build_layout(c, tok_doc, tok_slot, slot_wrd);
+} catch (...) {
+    return ezc::exception_status(c);
 }
 
 int ezlda_autotune(ezlda_ctx *c, int32_t sweeps_per_trial, ezlda_args *chosen, double *ms_per_sweep,
-                   ezlda_tune_trial *trials, int32_t max_trials, int32_t *num_trials) {
+                   ezlda_tune_trial *trials, int32_t max_trials, int32_t *num_trials) try {
     if (int r = need_loaded(c)) return r;
     if (is_chunked(c)) return refuse_chunked(c, "ezlda_autotune");
     if (sweeps_per_trial < 1) return fail(c, EZLDA_EINVAL, "sweeps_per_trial must be >= 1");
@@ -1276,9 +1359,11 @@ int ezlda_autotune(ezlda_ctx *c, int32_t sweeps_per_trial, ezlda_args *chosen, d
     if (ms_per_sweep) *ms_per_sweep = best_ms;
     if (num_trials) *num_trials = nt;
     return EZLDA_OK;
+} catch (...) {
+    return ezc::exception_status(c);
 }
 
-int ezlda_time_sweeps(ezlda_ctx *c, int32_t n, int32_t use_graph, double *total_ms, double *stage_ms) {
+int ezlda_time_sweeps(ezlda_ctx *c, int32_t n, int32_t use_graph, double *total_ms, double *stage_ms) try {
     if (int r = need_loaded(c)) return r;
     if (n < 1) return fail(c, EZLDA_EINVAL, "num_sweeps must be >= 1");
     if (is_chunked(c)) {  // whole sweeps only: stages overlap across the chunk streams
@@ -1354,9 +1439,11 @@ int ezlda_time_sweeps(ezlda_ctx *c, int32_t n, int32_t use_graph, double *total_
     if (stage_ms)
         for (int k = 0; k < EZLDA_NUM_STAGES; ++k) stage_ms[k] = st[k];
     return EZLDA_OK;
+} catch (...) {
+    return ezc::exception_status(c);
 }
 
-int ezlda_get_stats(ezlda_ctx *c, uint64_t *stats) {
+int ezlda_get_stats(ezlda_ctx *c, uint64_t *stats) try {
     if (int r = need_loaded(c)) return r;
     unsigned long long v[2] = {0, 0};
     EZ_TRY_HIP(c, hipMemcpy(v, c->s.stats, sizeof v, hipMemcpyDeviceToHost));
@@ -1382,13 +1469,17 @@ int ezlda_get_stats(ezlda_ctx *c, uint64_t *stats) {
     stats[2] = a;
     stats[3] = b;
     return EZLDA_OK;
+} catch (...) {
+    return ezc::exception_status(c);
 }
 
-int ezlda_reset_stats(ezlda_ctx *c) {
+int ezlda_reset_stats(ezlda_ctx *c) try {
     if (int r = need_loaded(c)) return r;
     EZ_TRY_HIP(c, hipMemsetAsync(c->s.stats, 0, 2 * sizeof(unsigned long long), c->st));
     EZ_TRY_HIP(c, hipStreamSynchronize(c->st));
     return EZLDA_OK;
+} catch (...) {
+    return ezc::exception_status(c);
 }
 
 namespace {
@@ -1435,13 +1526,15 @@ int loglik_into(ezlda_ctx *c, double *ll, double *part, double *out) {
 
 }  // namespace
 
-int ezlda_loglik(ezlda_ctx *c, double *ll) {
+int ezlda_loglik(ezlda_ctx *c, double *ll) try {
     if (int r = need_loaded(c)) return r;
     if (!ll) return fail(c, EZLDA_EINVAL, "null result pointer");
     return loglik_into(c, ll, nullptr, nullptr);
+} catch (...) {
+    return ezc::exception_status(c);
 }
 
-int ezlda_loglik_noalloc(ezlda_ctx *c, double *ll) {
+int ezlda_loglik_noalloc(ezlda_ctx *c, double *ll) try {
     if (int r = need_loaded(c)) return r;
     if (!ll) return fail(c, EZLDA_EINVAL, "null result pointer");
     if (!c->ll_scratch) return fail(c, EZLDA_EHIP, "the static log-likelihood array is unavailable: " + c->scratch_err);
@@ -1458,9 +1551,11 @@ int ezlda_loglik_noalloc(ezlda_ctx *c, double *ll) {
     // every partial the final sum reads is written by the same launch, so
     // nothing needs clearing
     return loglik_into(c, ll, c->ll_scratch, c->ll_scratch + ez::kLLScratchItems);
+} catch (...) {
+    return ezc::exception_status(c);
 }
 
-int ezlda_get_topics(ezlda_ctx *c, uint16_t *zout) {
+int ezlda_get_topics(ezlda_ctx *c, uint16_t *zout) try {
     if (int r = need_loaded(c)) return r;
     if (is_chunked(c)) {  // from the chunk images (z by doc slot)
         for (size_t i = 0; i < c->doc_tok.size(); ++i) {
@@ -1474,9 +1569,11 @@ int ezlda_get_topics(ezlda_ctx *c, uint16_t *zout) {
     EZ_TRY_HIP(c, hipMemcpy(z.data(), c->s.z, z.size() * sizeof(uint16_t), hipMemcpyDeviceToHost));
     for (size_t i = 0; i < z.size(); ++i) zout[i] = z[c->doc_tok[i]];
     return EZLDA_OK;
+} catch (...) {
+    return ezc::exception_status(c);
 }
 
-int ezlda_set_topics(ezlda_ctx *c, const uint16_t *zin) {
+int ezlda_set_topics(ezlda_ctx *c, const uint16_t *zin) try {
     if (int r = need_loaded(c)) return r;
     if (is_chunked(c)) {
         for (size_t i = 0; i < c->doc_tok.size(); ++i)
@@ -1502,29 +1599,37 @@ int ezlda_set_topics(ezlda_ctx *c, const uint16_t *zin) {
     EZ_TRY_HIP(c, hipMemcpy(c->s.z_doc, z_doc.data(), z_doc.size() * sizeof(uint16_t), hipMemcpyHostToDevice));
     c->counts_fresh = false;
     return EZLDA_OK;
+} catch (...) {
+    return ezc::exception_status(c);
 }
 
-int ezlda_set_sweep(ezlda_ctx *c, uint32_t sweep) {
+int ezlda_set_sweep(ezlda_ctx *c, uint32_t sweep) try {
     if (int r = need_loaded(c)) return r;
     EZ_TRY_HIP(c, ez::launch_set_u32(c->s.counters, sweep, c->st));
     EZ_TRY_HIP(c, hipStreamSynchronize(c->st));
     c->sweep = sweep;
     return EZLDA_OK;
+} catch (...) {
+    return ezc::exception_status(c);
 }
 
-int ezlda_get_nk(ezlda_ctx *c, int32_t *nk) {
+int ezlda_get_nk(ezlda_ctx *c, int32_t *nk) try {
     if (int r = need_loaded(c)) return r;
     EZ_TRY_HIP(c, hipMemcpy(nk, c->s.nk, sizeof(int32_t) * c->s.K, hipMemcpyDeviceToHost));
     return EZLDA_OK;
+} catch (...) {
+    return ezc::exception_status(c);
 }
 
-int ezlda_get_inv(ezlda_ctx *c, float *inv) {
+int ezlda_get_inv(ezlda_ctx *c, float *inv) try {
     if (int r = need_loaded(c)) return r;
     EZ_TRY_HIP(c, hipMemcpy(inv, c->s.inv, sizeof(float) * c->s.K, hipMemcpyDeviceToHost));
     return EZLDA_OK;
+} catch (...) {
+    return ezc::exception_status(c);
 }
 
-int ezlda_get_wt_dense(ezlda_ctx *c, int32_t *wt) {
+int ezlda_get_wt_dense(ezlda_ctx *c, int32_t *wt) try {
     if (int r = need_loaded(c)) return r;
     const ez::DevState &s = c->s;
     std::vector<int32_t> head((size_t)s.VD * s.K);
@@ -1548,9 +1653,11 @@ int ezlda_get_wt_dense(ezlda_ctx *c, int32_t *wt) {
         }
     }
     return EZLDA_OK;
+} catch (...) {
+    return ezc::exception_status(c);
 }
 
-int ezlda_export_model(ezlda_ctx *c, const char *dir) {
+int ezlda_export_model(ezlda_ctx *c, const char *dir) try {
     if (int r = need_loaded(c)) return r;
     if (!dir) return fail(c, EZLDA_EINVAL, "null directory");
     if (int r = ezc::refresh_counts(c)) return r;
@@ -1602,9 +1709,11 @@ int ezlda_export_model(ezlda_ctx *c, const char *dir) {
                                  c->sweep) > 0;
     if ((std::fclose(f) != 0) || !ok) return fail(c, EZLDA_EIO, "short write to " + d + "/model.txt");
     return EZLDA_OK;
+} catch (...) {
+    return ezc::exception_status(c);
 }
 
-int ezlda_get_dt_raw(ezlda_ctx *c, uint32_t *entries, int32_t *nnz) {
+int ezlda_get_dt_raw(ezlda_ctx *c, uint32_t *entries, int32_t *nnz) try {
     if (int r = need_loaded(c)) return r;
     if (is_chunked(c)) return refuse_chunked(c, "ezlda_get_dt_raw");
     const ez::DevState &s = c->s;
@@ -1631,9 +1740,11 @@ int ezlda_get_dt_raw(ezlda_ctx *c, uint32_t *entries, int32_t *nnz) {
         canon += dt_cap(c->doc_off[d + 1] - c->doc_off[d], s.K);
     }
     return EZLDA_OK;
+} catch (...) {
+    return ezc::exception_status(c);
 }
 
-int ezlda_get_top(ezlda_ctx *c, uint16_t *top) {
+int ezlda_get_top(ezlda_ctx *c, uint16_t *top) try {
     if (int r = need_loaded(c)) return r;
     std::vector<ushort4> t(c->s.V);
     EZ_TRY_HIP(c, hipMemcpy(t.data(), c->s.wtop, t.size() * sizeof(ushort4), hipMemcpyDeviceToHost));
@@ -1644,9 +1755,11 @@ int ezlda_get_top(ezlda_ctx *c, uint16_t *top) {
         top[3 * (size_t)w + 2] = x.z;
     }
     return EZLDA_OK;
+} catch (...) {
+    return ezc::exception_status(c);
 }
 
-int ezlda_get_aux(ezlda_ctx *c, uint32_t *aux) {
+int ezlda_get_aux(ezlda_ctx *c, uint32_t *aux) try {
     if (int r = need_loaded(c)) return r;
     if (is_chunked(c)) return refuse_chunked(c, "ezlda_get_aux");
     std::vector<uint32_t> a(c->s.N), at(c->s.N);
@@ -1654,9 +1767,11 @@ int ezlda_get_aux(ezlda_ctx *c, uint32_t *aux) {
     for (size_t p = 0; p < a.size(); ++p) at[c->slot_tok[p]] = a[p];
     for (size_t i = 0; i < a.size(); ++i) aux[i] = at[c->doc_tok[i]];
     return EZLDA_OK;
+} catch (...) {
+    return ezc::exception_status(c);
 }
 
-int ezlda_selftest_philox(const uint32_t *ctr, const uint32_t *key, uint32_t *out, int32_t n) {
+int ezlda_selftest_philox(const uint32_t *ctr, const uint32_t *key, uint32_t *out, int32_t n) try {
     if (n < 1 || !ctr || !key || !out) return fail(nullptr, EZLDA_EINVAL, "bad selftest arguments");
     uint32_t *dc = nullptr, *dk = nullptr, *dout = nullptr;
     EZ_TRY_HIP(nullptr, hipMalloc(&dc, (size_t)n * 16));
@@ -1670,9 +1785,11 @@ int ezlda_selftest_philox(const uint32_t *ctr, const uint32_t *key, uint32_t *ou
     (void)hipFree(dk);
     (void)hipFree(dout);
     return EZLDA_OK;
+} catch (...) {
+    return ezc::exception_status(nullptr);
 }
 
-int ezlda_selftest_copy(int64_t bytes, int32_t reps, double *ms) {
+int ezlda_selftest_copy(int64_t bytes, int32_t reps, double *ms) try {
     if (bytes < 16 || reps < 1) return fail(nullptr, EZLDA_EINVAL, "bad copy calibration arguments");
     bytes &= ~(int64_t)15;
     void *a = nullptr, *b = nullptr;
@@ -1697,9 +1814,11 @@ int ezlda_selftest_copy(int64_t bytes, int32_t reps, double *ms) {
     (void)hipFree(b);
     if (et != hipSuccess) return fail(nullptr, EZLDA_EHIP, "hipEventElapsedTime failed");
     return EZLDA_OK;
+} catch (...) {
+    return ezc::exception_status(nullptr);
 }
 
-int ezlda_selftest_gather(int64_t bytes, int64_t n_reads, int32_t reps, double *ms) {
+int ezlda_selftest_gather(int64_t bytes, int64_t n_reads, int32_t reps, double *ms) try {
     if (bytes < 64 || n_reads < 1 || reps < 1) return fail(nullptr, EZLDA_EINVAL, "bad gather calibration arguments");
     bytes &= ~(int64_t)63;
     int64_t nblk = 1;
@@ -1730,13 +1849,15 @@ int ezlda_selftest_gather(int64_t bytes, int64_t n_reads, int32_t reps, double *
     (void)hipFree(sink);
     if (et != hipSuccess) return fail(nullptr, EZLDA_EHIP, "hipEventElapsedTime failed");
     return EZLDA_OK;
+} catch (...) {
+    return ezc::exception_status(nullptr);
 }
 
 // Checkpoint: "EZLDACK1", then int64 K, V, D, N, uint64 seed, uint32 sweep,
 // then z (doc-major, uint16).  Restoring z and the sweep counter restores
 // the whole sampler state: counts are rebuilt from z and the draws are keyed
 // by (seed, token, sweep).
-int ezlda_save_checkpoint(ezlda_ctx *c, const char *path) {
+int ezlda_save_checkpoint(ezlda_ctx *c, const char *path) try {
     if (int r = need_loaded(c)) return r;
     std::vector<uint16_t> z(c->s.N);
     if (int r = ezlda_get_topics(c, z.data())) return r;
@@ -1750,9 +1871,74 @@ int ezlda_save_checkpoint(ezlda_ctx *c, const char *path) {
               (z.empty() || std::fwrite(z.data(), 2, z.size(), f) == z.size());
     ok = (std::fclose(f) == 0) && ok;
     return ok ? EZLDA_OK : fail(c, EZLDA_EIO, std::string("short write to ") + path);
+} catch (...) {
+    return ezc::exception_status(c);
 }
 
-int ezlda_load_checkpoint(ezlda_ctx *c, const char *path) {
+// The reference's own checkpoint form: the chunked folder's TL.txt files
+// rewritten with the current topics (streamTestBalance.cpp:469 writes the
+// initial ones there; DataChunk.cu:85 reads them back) and the Philox state
+// in <prefix>/ezlda_state.txt.
+int ezlda_save_tl(ezlda_ctx *c, const char *prefix) try {
+    if (int r = need_loaded(c)) return r;
+    if (!prefix) return fail(c, EZLDA_EINVAL, "null prefix");
+    const std::string root(prefix);
+    std::vector<uint64_t> pos;
+    std::vector<int32_t> words;
+    std::vector<int64_t> tll;
+    if (!ezc::chunked_layout(root, pos, words, tll)) return fail(c, EZLDA_EIO, ezlda_last_error(nullptr));
+    const size_t N = c->doc_tok.size();
+    if (pos.size() != N || words.size() != N || (int64_t)c->doc_off.size() - 1 != c->s.D)
+        return fail(c, EZLDA_EINVAL, root + " holds " + std::to_string(pos.size()) + " tokens; the context " +
+                                         std::to_string(N));
+    // the folder must hold the context's corpus: every doc-major token's word
+    std::vector<int32_t> id_of_rank(c->rank_of.size());
+    for (size_t w = 0; w < c->rank_of.size(); ++w) id_of_rank[c->rank_of[w]] = (int32_t)w;
+    std::vector<uint32_t> rank_of_tok(N);
+    for (size_t r = 0; r + 1 < c->word_off.size(); ++r)
+        for (uint32_t t = c->word_off[r]; t < c->word_off[r + 1]; ++t) rank_of_tok[t] = (uint32_t)r;
+    for (size_t i = 0; i < N; ++i)
+        if (id_of_rank[rank_of_tok[c->doc_tok[i]]] != words[i])
+            return fail(c, EZLDA_EINVAL, root + " does not hold the context's corpus (token " + std::to_string(i) + ")");
+    std::vector<uint16_t> z(N);
+    if (int r = ezlda_get_topics(c, z.data())) return r;
+    if (!ezc::rewrite_tl(root, pos, tll, z.data(), c->s.K)) return fail(c, EZLDA_EIO, ezlda_last_error(nullptr));
+    const std::string sp = root + "/ezlda_state.txt";
+    FILE *f = std::fopen(sp.c_str(), "w");
+    if (!f) return fail(c, EZLDA_EIO, "cannot write " + sp);
+    const bool ok = std::fprintf(f, "seed %llu\nsweep %u\nK %d\n", (unsigned long long)c->a.seed, c->sweep,
+                                 c->s.K) > 0;
+    if (std::fclose(f) != 0 || !ok) return fail(c, EZLDA_EIO, "cannot write " + sp);
+    return EZLDA_OK;
+} catch (...) {
+    return ezc::exception_status(c);
+}
+
+int ezlda_load_tl(ezlda_ctx *c, const char *prefix) try {
+    if (!c) return fail(nullptr, EZLDA_EINVAL, "null context");
+    if (!prefix) return fail(c, EZLDA_EINVAL, "null prefix");
+    const std::string root(prefix), sp = root + "/ezlda_state.txt";
+    unsigned long long seed = c->a.seed;
+    unsigned sweep = 0;
+    int K = c->a.K;
+    bool have_state = false;
+    if (FILE *f = std::fopen(sp.c_str(), "r")) {
+        const int got = std::fscanf(f, "seed %llu sweep %u K %d", &seed, &sweep, &K);
+        std::fclose(f);
+        if (got != 3) return fail(c, EZLDA_EIO, sp + ": expected \"seed S sweep N K K\"");
+        have_state = true;
+    }
+    if (have_state && (seed != c->a.seed || K != c->a.K))
+        return fail(c, EZLDA_EINVAL, sp + " was written with seed " + std::to_string(seed) + ", K " +
+                                         std::to_string(K) + "; the context has seed " + std::to_string(c->a.seed) +
+                                         ", K " + std::to_string(c->a.K));
+    if (int r = ezlda_load_chunked(c, prefix, 0)) return r;
+    return ezlda_set_sweep(c, sweep);
+} catch (...) {
+    return ezc::exception_status(c);
+}
+
+int ezlda_load_checkpoint(ezlda_ctx *c, const char *path) try {
     if (int r = need_loaded(c)) return r;
     FILE *f = std::fopen(path, "rb");
     if (!f) return fail(c, EZLDA_EIO, std::string("cannot read ") + path);
@@ -1777,6 +1963,8 @@ int ezlda_load_checkpoint(ezlda_ctx *c, const char *path) {
     if (!ok) return fail(c, EZLDA_EIO, std::string("truncated checkpoint: ") + path);
     if (int r = ezlda_set_topics(c, z.data())) return r;
     return ezlda_set_sweep(c, sweep);
+} catch (...) {
+    return ezc::exception_status(c);
 }
 
 }  // extern "C"

@@ -24,10 +24,11 @@ STAGES = ("wt", "topk", "dt", "sample")
 EXPORTS = (
     "ezlda_default_args", "ezlda_last_error", "ezlda_create", "ezlda_destroy", "ezlda_load_docs",
     "ezlda_load_chunked", "ezlda_get_info", "ezlda_build_wt", "ezlda_topk", "ezlda_build_dt",
-    "ezlda_sample", "ezlda_loglik", "ezlda_loglik_noalloc", "ezlda_sweep", "ezlda_sweep_graph", "ezlda_time_sweeps",
+    "ezlda_sample", "ezlda_shortcut", "ezlda_sample_head", "ezlda_sample_tail", "ezlda_commit_head",
+    "ezlda_wt_rowsum", "ezlda_loglik", "ezlda_loglik_noalloc", "ezlda_sweep", "ezlda_sweep_graph", "ezlda_time_sweeps",
     "ezlda_get_topics", "ezlda_set_topics", "ezlda_set_sweep", "ezlda_get_nk", "ezlda_get_wt_dense",
     "ezlda_get_dt_raw", "ezlda_get_top", "ezlda_get_aux", "ezlda_get_inv", "ezlda_save_checkpoint",
-    "ezlda_load_checkpoint", "ezlda_gen_corpus", "ezlda_corpus_num_tokens", "ezlda_corpus_num_docs",
+    "ezlda_load_checkpoint", "ezlda_save_tl", "ezlda_load_tl", "ezlda_gen_corpus", "ezlda_corpus_num_tokens", "ezlda_corpus_num_docs",
     "ezlda_corpus_num_words", "ezlda_corpus_doc_off", "ezlda_corpus_words", "ezlda_corpus_topics",
     "ezlda_corpus_free", "ezlda_corpus_from_arrays", "ezlda_read_docword", "ezlda_write_chunked",
     "ezlda_read_chunked", "ezlda_selftest_philox", "ezlda_selftest_copy", "ezlda_selftest_gather", "ezlda_get_stats",
@@ -109,6 +110,11 @@ def lib() -> C.CDLL:
         "ezlda_topk": (C.c_int, [P]),
         "ezlda_build_dt": (C.c_int, [P]),
         "ezlda_sample": (C.c_int, [P]),
+        "ezlda_shortcut": (C.c_int, [P]),
+        "ezlda_sample_head": (C.c_int, [P]),
+        "ezlda_sample_tail": (C.c_int, [P]),
+        "ezlda_commit_head": (C.c_int, [P]),
+        "ezlda_wt_rowsum": (C.c_int, [P]),
         "ezlda_loglik": (C.c_int, [P, C.POINTER(C.c_double)]),
         "ezlda_loglik_noalloc": (C.c_int, [P, C.POINTER(C.c_double)]),
         "ezlda_sweep": (C.c_int, [P, C.c_int32]),
@@ -125,6 +131,8 @@ def lib() -> C.CDLL:
         "ezlda_get_inv": (C.c_int, [P, P]),
         "ezlda_save_checkpoint": (C.c_int, [P, C.c_char_p]),
         "ezlda_load_checkpoint": (C.c_int, [P, C.c_char_p]),
+        "ezlda_save_tl": (C.c_int, [P, C.c_char_p]),
+        "ezlda_load_tl": (C.c_int, [P, C.c_char_p]),
         "ezlda_gen_corpus": (P, [C.POINTER(GenParams)]),
         "ezlda_corpus_num_tokens": (C.c_int64, [P]),
         "ezlda_corpus_num_docs": (C.c_int32, [P]),
@@ -312,6 +320,23 @@ class Context:
     def sample(self):
         self._chk(self._L.ezlda_sample(self._h))
 
+    # the reference's launcher-sized stages (any order of the first three,
+    # then commit_head, equals sample())
+    def shortcut(self):
+        self._chk(self._L.ezlda_shortcut(self._h))
+
+    def sample_head(self):
+        self._chk(self._L.ezlda_sample_head(self._h))
+
+    def sample_tail(self):
+        self._chk(self._L.ezlda_sample_tail(self._h))
+
+    def commit_head(self):
+        self._chk(self._L.ezlda_commit_head(self._h))
+
+    def wt_rowsum(self):
+        self._chk(self._L.ezlda_wt_rowsum(self._h))
+
     def sweep(self, n: int = 1):
         self._chk(self._L.ezlda_sweep(self._h, n))
 
@@ -386,6 +411,8 @@ class Context:
 
     def set_topics(self, z: np.ndarray):
         z = np.ascontiguousarray(z, dtype=np.uint16)
+        if z.ndim != 1 or len(z) != self._n():
+            raise EzldaError(EINVAL, f"set_topics needs {self._n()} topics, got {z.size}")
         self._chk(self._L.ezlda_set_topics(self._h, _ptr(z)))
 
     def set_sweep(self, s: int):
@@ -431,6 +458,15 @@ class Context:
     def load_checkpoint(self, path: str):
         self._chk(self._L.ezlda_load_checkpoint(self._h, path.encode()))
 
+    def save_tl(self, prefix: str):
+        """Rewrite the chunked folder's TL.txt files with the current topics
+        and write prefix/ezlda_state.txt (seed, sweep, K)."""
+        self._chk(self._L.ezlda_save_tl(self._h, prefix.encode()))
+
+    def load_tl(self, prefix: str):
+        """Load the chunked folder and resume at its saved sweep."""
+        self._chk(self._L.ezlda_load_tl(self._h, prefix.encode()))
+
 
 class Server:
     """Fold-in server over a trained Context (its model is frozen at creation)."""
@@ -444,6 +480,7 @@ class Server:
         self._h = self._L.ezlda_server_create(ctx._h, C.byref(a))
         if not self._h:
             raise EzldaError(EHIP, last_error())
+        self._lens = {}  # request id -> tokens, to size each result buffer
 
     def _chk(self, r):
         if r:
@@ -461,6 +498,7 @@ class Server:
     def submit(self, rid: int, words: np.ndarray):
         w = np.ascontiguousarray(words, dtype=np.int32)
         self._chk(self._L.ezlda_server_submit(self._h, rid, _ptr(w), len(w)))
+        self._lens[int(rid)] = len(w)
 
     def submit_many(self, rids, doc_off, words):
         """Queues len(rids) documents (words[doc_off[d]:doc_off[d+1]]) in one
@@ -471,14 +509,24 @@ class Server:
         if len(off) != len(ids) + 1:
             raise EzldaError(EINVAL, "doc_off needs len(rids) + 1 entries")
         self._chk(self._L.ezlda_server_submit_many(self._h, len(ids), _ptr(ids), _ptr(off), _ptr(w)))
+        for d, r in enumerate(ids.tolist()):
+            self._lens[r] = int(off[d + 1] - off[d])
 
-    def wait(self, rid: int, n: int, timeout_ms: int = -1):
-        """(z, topic counts, latency ms) of a finished request."""
-        z = np.zeros(max(n, 1), np.uint16)
+    def wait(self, rid: int, n: Optional[int] = None, timeout_ms: int = -1):
+        """(z, topic counts, latency ms) of a finished request.  The topic
+        buffer is sized from the length the request was submitted with through
+        this object (the library copies that many topics); a request submitted
+        elsewhere needs its length as n."""
+        known = self._lens.get(int(rid))
+        if known is None and n is None:
+            raise EzldaError(EINVAL, f"length of request {rid} unknown: pass n")
+        m = known if known is not None else int(n)
+        z = np.zeros(max(m, 1), np.uint16)
         c = np.zeros(self.K, np.int32)
         lat = C.c_double()
         self._chk(self._L.ezlda_server_wait(self._h, rid, _ptr(z), _ptr(c), C.byref(lat), timeout_ms))
-        return z[:n], c, lat.value
+        self._lens.pop(int(rid), None)
+        return z[:m], c, lat.value
 
     def drain(self):
         self._chk(self._L.ezlda_server_drain(self._h))
@@ -492,7 +540,7 @@ class Server:
         self._chk(self._L.ezlda_server_reset_stats(self._h))
 
     def plain(self, rid: int, words: np.ndarray):
-        """One document, one launch per sweep: (z, topic counts)."""
+        """One document per kernel launch (all of its sweeps): (z, topic counts)."""
         w = np.ascontiguousarray(words, dtype=np.int32)
         z = np.zeros(max(len(w), 1), np.uint16)
         c = np.zeros(self.K, np.int32)

@@ -1,18 +1,20 @@
-// C++ host surface over the C ABI (include/ezlda.h): RAII handles plus the
-// reference's launcher names, so a driver written against EZLDA's
-// mainKernel.cu reads the same.  Header-only; link with -lezlda.
+// C++ host surface over the C ABI (include/ezlda.h): RAII handles that
+// throw ezlda::Error (the library's status and message) where the reference's
+// void launchers leave errors unchecked.  Header-only; link with -lezlda.
+// The `ezlda` command (ezlda_cli.cpp) is written against it.
 //
 // Launcher mapping (reference -> this build):
-//   UpdateWTDenKernel, UpdateWTKernel, WTAdditionKernel,
-//   UpdateWTDenRowSumKernel (WTDenUpdateKernel.cuh:25-26, WTUpdateKernel.cuh:23,
-//   WTAddKernel.cuh:24)                    -> Context::build_wt (one call)
-//   MaxTopicKernel (SamplingKernel.cuh:12) -> Context::topk
-//   UpdateDTKernel (DTUpdateKernel.cuh:13) -> Context::build_dt
-//   UpdateProbKernelD, SampleKernelD, SampleKernel (SamplingKernel.cuh:10,11,15)
-//                                          -> Context::sample (one call)
+//   UpdateWTDenKernel, UpdateWTKernel, WTAdditionKernel (WTDenUpdateKernel.cuh:25,
+//   WTUpdateKernel.cuh:23, WTAddKernel.cuh:24) -> Context::build_wt
+//   UpdateWTDenRowSumKernel (WTDenUpdateKernel.cuh:26) -> Context::wt_rowsum
+//   MaxTopicKernel (SamplingKernel.cuh:12)   -> Context::topk
+//   UpdateDTKernel (DTUpdateKernel.cuh:13)   -> Context::build_dt
+//   UpdateProbKernelD (SamplingKernel.cuh:15) -> Context::shortcut
+//   SampleKernelD (SamplingKernel.cuh:10)    -> Context::sample_head
+//   SampleKernel (SamplingKernel.cuh:11)     -> Context::sample_tail
+//   the head commit (mainKernel.cu:355-356)  -> Context::commit_head
+//   (the last four fused)                    -> Context::sample
 //   PerplexityKernel (SamplingKernel.cuh:13) -> Context::loglik
-// The reference launchers return void and leave errors unchecked; these
-// throw ezlda::Error carrying the library's status and message.
 #pragma once
 
 #include <algorithm>
@@ -99,6 +101,11 @@ public:
     void topk() { check(ezlda_topk(h_), h_); }
     void build_dt() { check(ezlda_build_dt(h_), h_); }
     void sample() { check(ezlda_sample(h_), h_); }
+    void shortcut() { check(ezlda_shortcut(h_), h_); }
+    void sample_head() { check(ezlda_sample_head(h_), h_); }
+    void sample_tail() { check(ezlda_sample_tail(h_), h_); }
+    void commit_head() { check(ezlda_commit_head(h_), h_); }
+    void wt_rowsum() { check(ezlda_wt_rowsum(h_), h_); }
     void sweep(int n = 1) { check(ezlda_sweep(h_, n), h_); }
     void sweep_graph(int n = 1) { check(ezlda_sweep_graph(h_, n), h_); }
     double loglik() {
@@ -130,6 +137,10 @@ public:
     void export_model(const std::string &dir) { check(ezlda_export_model(h_, dir.c_str()), h_); }
     void save_checkpoint(const std::string &p) { check(ezlda_save_checkpoint(h_, p.c_str()), h_); }
     void load_checkpoint(const std::string &p) { check(ezlda_load_checkpoint(h_, p.c_str()), h_); }
+    // The chunked folder's TL.txt rewritten with the current topics, plus
+    // (seed, sweep); load_tl resumes from it.
+    void save_tl(const std::string &prefix) { check(ezlda_save_tl(h_, prefix.c_str()), h_); }
+    void load_tl(const std::string &prefix) { check(ezlda_load_tl(h_, prefix.c_str()), h_); }
     ezlda_ctx *handle() { return h_; }
 
 private:
@@ -181,11 +192,5 @@ private:
     ezlda_server *h_;
 };
 
-// The reference's launcher names (see the mapping above).
-inline void UpdateWTKernel(Context &c) { c.build_wt(); }
-inline void MaxTopicKernel(Context &c) { c.topk(); }
-inline void UpdateDTKernel(Context &c) { c.build_dt(); }
-inline void SampleKernel(Context &c) { c.sample(); }
-inline double PerplexityKernel(Context &c) { return c.loglik(); }
 
 }  // namespace ezlda

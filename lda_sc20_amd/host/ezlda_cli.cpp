@@ -49,8 +49,10 @@ const char *kUsage =
     "  train      --prefix DIR [--chunks N] [--iters N] [--K K] [--alpha A] [--beta B]\n"
     "             [--block-dim N] [--grid-dim N] [--streams N] [--token-segment N] [--seed S]\n"
     "             [--dense-threshold N] [--dense-words N] [--doc-tiles N] [--run-cap N] [--plain]\n"
-    "             [--graph] [--autotune] [--ll-every N] [--checkpoint FILE] [--resume FILE] [--record FILE]\n"
-    "             [--export DIR]\n"
+    "             [--topic-bits 16] [--graph] [--autotune] [--ll-every N] [--checkpoint FILE]\n"
+    "             [--resume FILE] [--checkpoint-tl] [--resume-tl] [--record FILE] [--export DIR]\n"
+    "             (--checkpoint-tl rewrites DIR's chunk TL.txt files with the final topics plus\n"
+    "             DIR/ezlda_state.txt; --resume-tl continues from them)\n"
     "  foldin     train options, plus --requests DOCWORD [--foldin-sweeps 20] [--slots N]\n"
     "             [--budget-mb N] [--out FILE]: folds the documents of a docword file into the\n"
     "             trained model and writes each one's topic counts\n"
@@ -88,6 +90,7 @@ ezlda_args args_from(const Opts &o) {
     a.doc_tiles = (int32_t)o.integer("doc-tiles", a.doc_tiles);
     a.run_cap = (int32_t)o.integer("run-cap", a.run_cap);
     a.shortcut = o.has("plain") ? 0 : 1;
+    a.topic_bits = (int32_t)o.integer("topic-bits", a.topic_bits);  // checked by ezlda_create
     return a;
 }
 
@@ -122,7 +125,10 @@ int run_training(const Opts &o, ezlda::Context &ctx) {
     const int ll_every = (int)o.integer("ll-every", 0);
     const bool graph = o.has("graph");
     auto t0 = std::chrono::steady_clock::now();
-    ctx.load_chunked(prefix, (int)o.integer("chunks", 0));
+    if (o.has("resume-tl"))
+        ctx.load_tl(prefix);  // TL.txt topics and the saved sweep
+    else
+        ctx.load_chunked(prefix, (int)o.integer("chunks", 0));
     const ezlda_info inf = ctx.info();
     std::printf("loaded %s: %lld tokens, %d docs, %d words (%d dense), K=%d, %.2f GB on device, %.2f s\n",
                 prefix.c_str(), (long long)inf.num_tokens, inf.num_docs, inf.num_words, inf.num_head_words, inf.K,
@@ -161,6 +167,10 @@ int run_training(const Opts &o, ezlda::Context &ctx) {
     std::printf("%d sweeps in %.1f ms (%.3e tokens/s); log-likelihood %.6e; perplexity %.4f\n", iters, total,
                 inf.num_tokens * (double)iters / (total / 1e3), ll, std::exp(-ll / (double)inf.num_tokens));
     if (o.has("checkpoint")) ctx.save_checkpoint(o.str("checkpoint"));
+    if (o.has("checkpoint-tl")) {
+        ctx.save_tl(prefix);
+        std::printf("topics of sweep %u written to %s/chunk*/TL.txt\n", ctx.info().sweep, prefix.c_str());
+    }
     if (o.has("export")) ctx.export_model(o.str("export"));
     return 0;
 }

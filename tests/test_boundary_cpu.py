@@ -181,3 +181,88 @@ def test_docword_reader(tmp_path):
     bad.write_text("2 5 3\n1 1 2\n2 9 1\n")
     with pytest.raises(ezlda.EzldaError):
         ezlda.read_docword(str(bad))
+
+
+def test_chunked_reader_rejects_negative_counts(tmp_path):
+    """A negative chunk token or document count is an error code, not an
+    exception escaping the C ABI (which would abort the host process)."""
+    off, words = make_corpus(6, 30, 50, 20)
+    ezlda.Corpus(50, off, words).write_chunked(str(tmp_path / "c"), K=8, num_chunks=2)
+    root = tmp_path / "c"
+    good_tll = open(root / "TLLength.txt").read()
+    vals = good_tll.split()
+    open(root / "TLLength.txt", "w").write(f"-1\n{vals[1]}\n")
+    with pytest.raises(ezlda.EzldaError, match="negative or oversized"):
+        ezlda.read_chunked(str(root))
+    open(root / "TLLength.txt", "w").write(good_tll)
+    dl = open(root / "docLength.txt").read().split()
+    open(root / "docLength.txt", "w").write(f"{dl[0]}\n-5\n")
+    with pytest.raises(ezlda.EzldaError, match="negative or oversized"):
+        ezlda.read_chunked(str(root))
+
+
+
+# ------------------------------------------------------ binary sidecar
+def _same(a: ezlda.Corpus, b: ezlda.Corpus):
+    return (a.V == b.V and np.array_equal(a.doc_off, b.doc_off) and np.array_equal(a.words, b.words)
+            and np.array_equal(a.topics, b.topics))
+
+
+def test_sidecar_equals_text_reader(tmp_path):
+    off, words = make_corpus(7, 200, 400, 50)
+    root = tmp_path / "c"
+    ezlda.Corpus(400, off, words).write_chunked(str(root), K=24, num_chunks=3, seed=5)
+    assert (root / "corpus.ezb").exists()
+    fast = ezlda.read_chunked(str(root))
+    os.remove(root / "corpus.ezb")
+    text = ezlda.read_chunked(str(root))
+    assert _same(fast, text)
+
+
+def test_sidecar_ignored_once_a_text_file_changes(tmp_path):
+    off, words = make_corpus(8, 100, 300, 40)
+    root = tmp_path / "c"
+    ezlda.Corpus(300, off, words).write_chunked(str(root), K=24, num_chunks=2, seed=5)
+    before = ezlda.read_chunked(str(root))
+    tl = root / "chunk1" / "TL.txt"
+    vals = open(tl).read().split()
+    vals[0] = str(int(vals[0]) % 24 + 1)  # another topic for the first token of chunk 1's TL
+    open(tl, "w").write("\n".join(vals) + "\n")
+    after = ezlda.read_chunked(str(root))
+    assert not np.array_equal(before.topics, after.topics)
+    os.remove(root / "corpus.ezb")
+    assert _same(after, ezlda.read_chunked(str(root)))
+
+
+def test_sidecar_rejects_a_foreign_or_truncated_file(tmp_path):
+    off, words = make_corpus(9, 80, 200, 30)
+    root = tmp_path / "c"
+    ezlda.Corpus(200, off, words).write_chunked(str(root), K=16, num_chunks=2, seed=5)
+    good = ezlda.read_chunked(str(root))
+    side = root / "corpus.ezb"
+    data = side.read_bytes()
+    side.write_bytes(data[: len(data) // 2])  # truncated: the text path is taken
+    assert _same(good, ezlda.read_chunked(str(root)))
+    side.write_bytes(b"NOTEZLDA" + data[8:])
+    assert _same(good, ezlda.read_chunked(str(root)))
+
+
+# ------------------------------------------- the integration example
+def test_integration_example_compiles_and_links(tmp_path):
+    """The C++ driver shown in INTEGRATION.md builds against include/ezlda.h
+    and links with libezlda.so (run on the GPU by tests/test_gpu_cli.py)."""
+    import re
+    import shutil
+    import subprocess
+
+    txt = open(os.path.join(ROOT, "INTEGRATION.md")).read()
+    blocks = [b for b in re.findall(r"```cpp\n(.*?)```", txt, flags=re.S) if "int main" in b]
+    assert blocks, "no C++ driver in INTEGRATION.md"
+    src = tmp_path / "driver.cpp"
+    src.write_text(blocks[0])
+    gxx = shutil.which("g++")
+    lib = os.path.join(ROOT, "lda_sc20_amd")
+    r = subprocess.run([gxx, "-std=c++17", "-O1", f"-I{os.path.join(ROOT, 'include')}", str(src), "-o",
+                        str(tmp_path / "driver"), f"-L{lib}", "-lezlda", f"-Wl,-rpath,{lib}"],
+                       stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True)
+    assert r.returncode == 0, r.stdout

@@ -162,3 +162,33 @@ def test_submit_many_all_or_none():
         assert np.array_equal(z, zo[off[d]: off[d + 1]]) and np.array_equal(c, co[d]), d
     assert srv.stats().completed == len(lens)
     srv.close()
+
+
+@pytest.mark.parametrize("K,lens", [(16383, [3000, 1, 0, 700]), (1024, [20000, 5000, 64, 2])])
+def test_plain_one_launch_per_document(K, lens):
+    """The plain path runs all of a document's sweeps in one launch (its row
+    in LDS): bit-exact with the oracle at the largest K (the largest LDS row)
+    and for documents far longer than the workgroup."""
+    ctx, orc = _model(K, dense_threshold=100)
+    off, words = _docs(300, lens, seed=K)
+    rids = np.arange(40, 40 + len(lens), dtype=np.uint32)
+    zo, co = orc.foldin(off, words, rids, 7, 13)
+    srv = ezlda.Server(ctx, max_slots=4, sweeps=7, hbm_budget_bytes=1 << 20, seed=13)
+    for r in range(len(lens)):
+        z, c = srv.plain(int(rids[r]), words[off[r]: off[r + 1]])
+        assert np.array_equal(z, zo[off[r]: off[r + 1]]), (K, r)
+        assert np.array_equal(c, co[r]), (K, r)
+    srv.close()
+
+
+def test_wait_buffer_comes_from_the_submitted_length():
+    """The library copies as many topics as the request was submitted with;
+    Server.wait sizes its buffer from that, whatever n the caller passes."""
+    ctx, orc = _model(32)
+    off, words = _docs(300, [150])
+    zo, co = orc.foldin(off, words, [5], 4, 2)
+    srv = ezlda.Server(ctx, max_slots=2, sweeps=4, seed=2)
+    srv.submit(5, words)
+    z, c, _ = srv.wait(5, 1, timeout_ms=30000)
+    assert len(z) == 150 and np.array_equal(z, zo) and np.array_equal(c, co[0])
+    srv.close()

@@ -298,3 +298,42 @@ def test_compact_rows_with_pieces(K):
         orc.sample()
         assert np.array_equal(ctx.get_topics(), orc.get_topics()), s
     assert ctx.loglik() == pytest.approx(orc.loglik(), rel=1e-9)
+
+
+# ------------------------------------------- launcher-granularity stages
+@pytest.mark.parametrize("order,kw", [
+    (("shortcut", "sample_head", "sample_tail"), {}),
+    (("sample_tail", "sample_head"), {}),
+    (("sample_head", "shortcut", "sample_tail"), {"grid_dim": 0}),
+    (("shortcut", "sample_tail", "sample_head"), {"shortcut": False}),
+    (("shortcut", "sample_head", "sample_tail"), {"K": 10000}),
+])
+def test_launcher_stages_equal_fused_sampler(order, kw):
+    """ezlda_shortcut / _sample_head / _sample_tail (the reference's
+    UpdateProbKernelD, SampleKernelD, SampleKernel) in any order, then
+    ezlda_commit_head, give the fused sampler's topics; commit_head leaves
+    the head rows of the new topics and ends the sweep; wt_rowsum gives n_k."""
+    kw = dict(kw)
+    K = kw.pop("K", 64)
+    shortcut = kw.pop("shortcut", True)
+    V = 400
+    off, words = make_corpus(12, 300, V, 80, zipf=1.2)
+    ctx, orc = _pair(off, words, V, K, shortcut=shortcut, dense_threshold=30, **kw)
+    VD = ctx.info().num_head_words
+    assert 0 < VD < V
+    cnt = np.bincount(words, minlength=V)
+    head = np.lexsort((np.arange(V), -cnt))[:VD]  # ranks 0..VD-1: count desc, id asc
+    for s in range(3):
+        ctx.build_wt()
+        ctx.topk()
+        ctx.build_dt()
+        for stage in order:
+            getattr(ctx, stage)()
+        ctx.commit_head()
+        orc.sweep(1)
+        assert ctx.info().sweep == s + 1
+        assert np.array_equal(ctx.get_topics(), orc.get_topics()), (order, s)
+        orc.build_wt()
+        assert np.array_equal(ctx.get_wt_dense()[head], orc.get_wt_dense()[head]), (order, s)
+    ctx.wt_rowsum()
+    assert np.array_equal(ctx.get_nk(), orc.get_nk())
