@@ -43,9 +43,10 @@ typedef struct ezlda_args {
     float beta;              /* Argument.cuh:8; <= 0 means 0.01 */
     int32_t block_dim;       /* Argument.cuh:4 BlockDim: threads per workgroup
                                 of the sampler; multiple of 64, <= 1024
-                                (<= 0 = auto: 64 where sixteen such
-                                workgroups' LDS fits in a CU at this K,
-                                else 128) */
+                                (<= 0 = auto: the smallest of 64, 128, ...
+                                with which the workgroups whose LDS fits in
+                                a CU at this K hold >= 12 wavefronts: 64 up
+                                to K ~ 2400, 256 at K = 10000) */
     int32_t grid_dim;        /* Argument.cuh:5 GridDim: > 0 = that many
                                 persistent sampler workgroups pulling items
                                 from per-XCD queues (then from the others');

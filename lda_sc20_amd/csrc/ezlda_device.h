@@ -12,6 +12,9 @@ namespace ez {
 // not serialize on one line.
 static constexpr int kQueueStride = 32;
 static constexpr int kCounterWords = kQueueStride * 9;
+// Documents of at most this many tokens can have their DT rows built by
+// sorting their topics (large K); longer ones use a K-sized histogram.
+static constexpr uint32_t kDtShort = 512;
 
 // Device image of one corpus + model.  Token ids are word-major positions;
 // N < 2^32.  Sizes: V words, VD head words, D docs, K topics.
@@ -32,6 +35,7 @@ struct DevState {
     uint32_t *aux_doc = nullptr;    // N: n_dk1 | n_dk2 << 16 of the slot's word
     // doc-topic rows
     uint2 *doc_info = nullptr;      // D, by doc q: x = DT row offset, y = nnz | len << 16
+    int32_t D_long = 0;             // docs of more than kDtShort tokens (the first D_long in doc order)
     uint32_t *dt = nullptr;         // rows: u32 (topic << 2) | count << 16, or (compact) u16 topic | piece << 10
     bool compact = false;           // K <= 8192: 16-bit entries topic | piece << topic_bits
     uint32_t topic_bits = 16;       // max(10, ceil(log2 K)); pieces of <= 2^(16 - topic_bits) - 1

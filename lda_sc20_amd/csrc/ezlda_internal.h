@@ -16,6 +16,7 @@ struct EzChunk {
     uint32_t q0 = 0, D = 0, N = 0;  // first doc (doc order), docs, tokens
     uint32_t slot0 = 0;             // first doc slot of the corpus layout
     uint32_t n_items = 0, n_wt_segs = 0, n_queues = 1;
+    int32_t D_long = 0;             // docs of more than ez::kDtShort tokens (the chunk's first)
     uint64_t n_runs = 0, dt_entries = 0;
     // image: z | z_doc | doc_info (copied back after each sweep), then
     // soff, slot_wrd, slot_tok, items, runs, WT segments

@@ -28,6 +28,7 @@
 #ifndef EZ_Q_TABLE_MAX_K
 #define EZ_Q_TABLE_MAX_K 1024
 #endif
+static_assert(EZ_Q_TABLE_MAX_K <= 8192, "the Q-table sampler variants assume compact (K <= 8192) DT rows");
 
 namespace ez {
 
@@ -303,7 +304,10 @@ __global__ void __launch_bounds__(256) k_zero_order(const float *__restrict__ in
 // wavefront per word; a tail word's counts are scattered into a 16-bit LDS
 // row first.  Per-lane lists are merged with a butterfly; the order is total,
 // so the merge order does not matter.
-__global__ void k_word_top(int V, int VD, int K, float beta, const int32_t *__restrict__ wt_head,
+// Words [w0, w1) of one launch.  Only tail words use the LDS count table:
+// a launch with w0 < VD and no LDS (w1 <= VD) runs head words at full
+// occupancy; a launch that reaches tail words has the table.
+__global__ void k_word_top(int w0, int w1, int VD, int K, float beta, const int32_t *__restrict__ wt_head,
                            const uint32_t *__restrict__ tail_off, const uint32_t *__restrict__ wt_tail,
                            const uint32_t *__restrict__ tail_nnz, const float *__restrict__ inv,
                            const uint32_t *__restrict__ zero_order, ushort4 *__restrict__ wtop) {
@@ -311,13 +315,28 @@ __global__ void k_word_top(int V, int VD, int K, float beta, const int32_t *__re
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6, nwv = blockDim.x >> 6;
     const int KH = (K + 1) >> 1;
     uint32_t *h = sh + wv * KH;
-    for (int k = lane; k < KH; k += 64) h[k] = 0;
-    wave_lds_sync();
-    for (int w = blockIdx.x * nwv + wv; w < V; w += gridDim.x * nwv) {
+    if (w1 > VD) {
+        for (int k = lane; k < KH; k += 64) h[k] = 0;
+        wave_lds_sync();
+    }
+    for (int w = w0 + blockIdx.x * nwv + wv; w < w1; w += gridDim.x * nwv) {
         Top3 t{-1.f, -1.f, -1.f, kNone, kNone, kNone};
         if (w < VD) {
+            // four loads of the row and of inv in flight per lane
             const int32_t *row = wt_head + (size_t)w * K;
-            for (int k = lane; k < K; k += 64) top3_insert(t, ((float)row[k] + beta) * inv[k], (uint32_t)k);
+            for (int k0 = lane; k0 < K; k0 += 256) {
+                int32_t c[4];
+                float iv[4];
+#pragma unroll
+                for (int u = 0; u < 4; ++u) {
+                    const int k = k0 + 64 * u;
+                    c[u] = k < K ? row[k] : 0;
+                    iv[u] = k < K ? inv[k] : 0.0f;
+                }
+#pragma unroll
+                for (int u = 0; u < 4; ++u)
+                    if (k0 + 64 * u < K) top3_insert(t, ((float)c[u] + beta) * iv[u], (uint32_t)(k0 + 64 * u));
+            }
         } else {
             // the word's topics with a count, then those without one: their
             // value is beta * inv[k], so the first three of zero_order that
@@ -385,7 +404,7 @@ hipError_t launch_z_to_doc(const DevState &s, hipStream_t st) {
 static constexpr int kDtU = 8;  // slots per lane per chunk in k_dt_build
 
 template <bool COMPACT>
-__global__ void k_dt_build(int D, int K, int VD, uint32_t tb, const uint32_t *__restrict__ soff,
+__global__ void k_dt_build(int q0, int q1, int K, int VD, uint32_t tb, const uint32_t *__restrict__ soff,
                            const uint32_t *__restrict__ slot_wrd, const uint32_t *__restrict__ slot_tok,
                            const uint16_t *__restrict__ z, const uint16_t *__restrict__ z_doc,
                            const ushort4 *__restrict__ wtop,
@@ -398,7 +417,7 @@ __global__ void k_dt_build(int D, int K, int VD, uint32_t tb, const uint32_t *__
     uint32_t *bm = h + KH;              // bitmap of the doc's topics
     for (int k = lane; k < KH + NW; k += 64) h[k] = 0;
     wave_lds_sync();
-    for (int q = blockIdx.x * nwv + wv; q < D; q += gridDim.x * nwv) {
+    for (int q = q0 + blockIdx.x * nwv + wv; q < q1; q += gridDim.x * nwv) {
         const uint32_t b = soff[q], e = soff[q + 1];
         // The doc in chunks of kDtU slots per lane, every load of a chunk
         // issued before the first is used.  A doc of one chunk keeps its
@@ -525,6 +544,150 @@ __global__ void k_dt_build(int D, int K, int VD, uint32_t tb, const uint32_t *__
     }
 }
 
+// The same DT rows and aux for short documents (n <= kDtShort slots) at
+// large K, where a K-sized LDS histogram per wavefront limits a CU to a few
+// wavefronts: one wavefront per doc sorts its topics in 2 KB of LDS (a
+// bitonic network over the next power of two >= n, 64..512 keys), reads the
+// runs of equal topics off the sorted keys (ascending topic, as the
+// histogram's compaction writes them) and finds each slot's (n_dk1, n_dk2)
+// by bisection.  Rows, pieces, padding and aux are those of k_dt_build.
+
+template <bool COMPACT>
+__global__ void __launch_bounds__(256) k_dt_sort(int q0, int q1, int VD, uint32_t tb,
+                                                 const uint32_t *__restrict__ soff, const uint32_t *__restrict__ slot_wrd,
+                                                 const uint32_t *__restrict__ slot_tok, const uint16_t *__restrict__ z,
+                                                 const uint16_t *__restrict__ z_doc, const ushort4 *__restrict__ wtop,
+                                                 uint32_t *__restrict__ aux_doc, uint2 *__restrict__ doc_info,
+                                                 uint32_t *__restrict__ dt) {
+    extern __shared__ uint32_t sh[];
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6, nwv = blockDim.x >> 6;
+    uint16_t *key = reinterpret_cast<uint16_t *>(sh + wv * kDtShort);  // kDtShort sorted topics
+    uint16_t *run = key + kDtShort;                                      // run length at each run's first key
+    const uint32_t piece = (1u << (16u - tb)) - 1u;
+    constexpr int kU = kDtShort / 64;
+    for (int q = q0 + blockIdx.x * nwv + wv; q < q1; q += gridDim.x * nwv) {
+        const uint32_t b = soff[q], e = soff[q + 1], n = e - b;  // n <= kDtShort
+        uint32_t P = 64;
+        while (P < n) P <<= 1;
+        const int nu = (int)(P >> 6);
+        uint32_t wrd[kU];
+        // topics (the doc-slot copy for head words, z by token id for tail
+        // words), padded with 0xFFFF, which sorts last
+#pragma unroll
+        for (int u = 0; u < kU; ++u) {
+            const uint32_t i = (uint32_t)lane + 64u * u;
+            wrd[u] = i < n ? slot_wrd[b + i] : 0u;
+        }
+#pragma unroll
+        for (int u = 0; u < kU; ++u) {
+            const uint32_t i = (uint32_t)lane + 64u * u;
+            if (u < nu) {
+                uint32_t t = 0xFFFFu;
+                if (i < n) t = wrd[u] < (uint32_t)VD ? z_doc[b + i] : z[slot_tok[b + i]];
+                key[i] = (uint16_t)t;
+            }
+        }
+        wave_lds_sync();
+        // bitonic sort, ascending: P / 2 compare-exchanges per pass
+        for (uint32_t k = 2; k <= P; k <<= 1)
+            for (uint32_t j = k >> 1; j > 0; j >>= 1) {
+                for (uint32_t p = (uint32_t)lane; p < (P >> 1); p += 64) {
+                    const uint32_t i = (p / j) * (j << 1) + (p % j), l = i + j;
+                    const uint16_t a = key[i], c = key[l];
+                    const bool up = (i & k) == 0;
+                    if ((a > c) == up && a != c) {
+                        key[i] = c;
+                        key[l] = a;
+                    }
+                }
+                wave_lds_sync();
+            }
+        // run heads and their lengths: the next head (or n) minus the head
+        uint64_t hm[kU];
+#pragma unroll
+        for (int u = 0; u < kU; ++u) {
+            const uint32_t i = (uint32_t)lane + 64u * u;
+            const bool head = u < nu && i < n && (i == 0 || key[i - 1] != key[i]);
+            hm[u] = __ballot(head);
+        }
+        const uint32_t base = doc_info[q].x;
+        uint32_t nz = 0;
+#pragma unroll
+        for (int u = 0; u < kU; ++u) {
+            if (u >= nu) break;
+            const uint32_t i = (uint32_t)lane + 64u * u;
+            const bool head = (hm[u] >> lane) & 1ull;
+            uint32_t len = 0, m = 0;
+            if (head) {
+                uint32_t nxt = n;  // the first head after i
+                const uint64_t above = hm[u] & ~((2ull << lane) - 1ull);
+                if (above) {
+                    nxt = 64u * u + (uint32_t)first_lane(above);
+                } else {
+#pragma unroll
+                    for (int v = kU - 1; v > u; --v)
+                        if (v < nu && hm[v]) nxt = 64u * v + (uint32_t)first_lane(hm[v]);
+                }
+                len = nxt - i;
+                m = COMPACT ? (len + piece - 1u) / piece : 1u;
+                run[i] = (uint16_t)len;
+            }
+            uint32_t incl = m;  // inclusive scan over lanes
+            for (int d = 1; d < 64; d <<= 1) {
+                const uint32_t v = __shfl_up(incl, d);
+                if (lane >= d) incl += v;
+            }
+            uint32_t pos = nz + incl - m;
+            if (head) {
+                const uint32_t k = key[i];
+                if (COMPACT) {
+                    uint16_t *dt16 = reinterpret_cast<uint16_t *>(dt) + base;
+                    uint32_t c = len;
+                    for (; c > piece; c -= piece) dt16[pos++] = (uint16_t)(k | (piece << tb));
+                    dt16[pos++] = (uint16_t)(k | (c << tb));
+                } else {
+                    dt[base + pos] = (k << 2) | (len << 16);
+                }
+            }
+            nz += __shfl(incl, 63);
+        }
+        // zero entries up to the next 128 bytes, as k_dt_build
+        if (n > 0) {
+            if (COMPACT) {
+                uint16_t *dt16 = reinterpret_cast<uint16_t *>(dt) + base;
+                if ((uint32_t)lane < ((64u - (nz & 63u)) & 63u)) dt16[nz + lane] = 0;
+            } else {
+                if ((uint32_t)lane < ((32u - (nz & 31u)) & 31u)) dt[base + nz + lane] = 0u;
+            }
+        }
+        if (lane == 0) doc_info[q].y = nz | (n << 16);
+        wave_lds_sync();  // run lengths visible
+        // (n_dk1, n_dk2) of each slot's word: its top topics' runs, found by
+        // bisection of the sorted keys
+#pragma unroll
+        for (int u = 0; u < kU; ++u) {
+            const uint32_t i = (uint32_t)lane + 64u * u;
+            if (u < nu && i < n) {
+                const ushort4 tp = wtop[wrd[u]];
+                uint32_t cnt[2] = {0u, 0u};
+                const uint32_t want[2] = {tp.x, tp.y};
+#pragma unroll
+                for (int h = 0; h < 2; ++h) {
+                    if (want[h] == kNone) continue;
+                    uint32_t lo = 0, hi = n;
+                    while (lo < hi) {
+                        const uint32_t mid = (lo + hi) >> 1;
+                        if (key[mid] < want[h]) lo = mid + 1; else hi = mid;
+                    }
+                    if (lo < n && key[lo] == want[h]) cnt[h] = run[lo];
+                }
+                aux_doc[b + i] = cnt[0] | (cnt[1] << 16);
+            }
+        }
+        wave_lds_sync();  // the next doc reuses the keys
+    }
+}
+
 // ------------------------------------------------------------- sampler
 // Checkpoints of the S running sum kept per lane (one per block of kBlkU4
 // 16-byte loads, i.e. 4*kBlkU4 DT entries).
@@ -604,13 +767,13 @@ struct SampleArgs {
     uint32_t mode;  // kSampleHead | kSampleTail: which words' items; | kShortcutOnly: no full draws
 };
 
-template <bool SHORTCUT, bool COMPACT>
+template <bool SHORTCUT, bool COMPACT, bool QTAB>
 __device__ __forceinline__ void sample_item(const SampleArgs &A, uint32_t item, float *smf, float *f123,
                                             uint32_t sweep, uint32_t &n_full, uint32_t &n_scanned) {
     const int K = A.K, NS = (K + 63) >> 6;
     const float alpha = A.alpha, beta = A.beta;
     const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6, nwv = blockDim.x >> 6;
-    const bool qtab = K <= EZ_Q_TABLE_MAX_K;  // uniform
+    constexpr bool qtab = QTAB;  // K <= EZ_Q_TABLE_MAX_K
     float *phi = smf;                         // NS*64
     float *qs = phi + NS * 64;                // NS*64 when qtab: running sums inside each slab
     float *qt = qs + (qtab ? NS * 64 : 0);    // NS: slab totals
@@ -619,6 +782,7 @@ __device__ __forceinline__ void sample_item(const SampleArgs &A, uint32_t item, 
     if (sg.x == 0xFFFFFFFFu) return;  // padding item (uniform over the block)
     const int w = (int)sg.x;
     if (!(A.mode & (w < A.VD ? kSampleHead : kSampleTail))) return;  // the other words' stage
+    const bool sc_only = (A.mode & kShortcutOnly) != 0;
     const ushort4 tp = A.wtop[w];
 
     if (w < A.VD) {
@@ -724,7 +888,7 @@ __device__ __forceinline__ void sample_item(const SampleArgs &A, uint32_t item, 
                 if (head) A.z_doc[p0 + j] = (uint16_t)k1;
             }
         }
-        if (!need || (A.mode & kShortcutOnly)) continue;
+        if (!need || sc_only) continue;
         // the row: 32-bit entries, or 16-bit entries at a 16-bit offset
         const uint4 *rp = COMPACT ? reinterpret_cast<const uint4 *>(reinterpret_cast<const uint16_t *>(A.dt) + di.x)
                                   : reinterpret_cast<const uint4 *>(A.dt + di.x);
@@ -901,18 +1065,18 @@ __device__ __forceinline__ void sample_item(const SampleArgs &A, uint32_t item, 
 // n_queues (the items of one XCD when items are dealt round-robin to XCDs),
 // a workgroup drains its own queue (blockIdx mod n_queues) and then the
 // others.  Every workgroup leaves once every queue is past its last item.
-template <bool SHORTCUT, bool COMPACT>
+template <bool SHORTCUT, bool COMPACT, bool QTAB>
 __global__ void __launch_bounds__(1024) k_sample(const SampleArgs A) {
     // the only LDS of the kernel, so phi starts at address 0 and a DT entry's
     // low half is directly the LDS address of its phi value
     extern __shared__ float smf[];
     const int NS = (A.K + 63) >> 6;
-    float *f123 = smf + (A.K <= EZ_Q_TABLE_MAX_K ? 2 : 1) * NS * 64 + 2 * NS;  // after phi, qs, qt, qacc
+    float *f123 = smf + (QTAB ? 2 : 1) * NS * 64 + 2 * NS;  // after phi, qs, qt, qacc
     uint32_t &s_item = *reinterpret_cast<uint32_t *>(f123 + 3);
     const uint32_t sweep = A.counters[0];
     uint32_t n_full = 0, n_scanned = 0;
     if (A.n_queues == 0) {
-        sample_item<SHORTCUT, COMPACT>(A, blockIdx.x, smf, f123, sweep, n_full, n_scanned);
+        sample_item<SHORTCUT, COMPACT, QTAB>(A, blockIdx.x, smf, f123, sweep, n_full, n_scanned);
     } else {
         const uint32_t nq = A.n_queues, x0 = blockIdx.x % nq;
         for (uint32_t h = 0; h < nq; ++h) {
@@ -923,7 +1087,7 @@ __global__ void __launch_bounds__(1024) k_sample(const SampleArgs A) {
                 __syncthreads();
                 const uint64_t item = (uint64_t)q + (uint64_t)s_item * nq;
                 if (item >= A.n_items) break;
-                sample_item<SHORTCUT, COMPACT>(A, (uint32_t)item, smf, f123, sweep, n_full, n_scanned);
+                sample_item<SHORTCUT, COMPACT, QTAB>(A, (uint32_t)item, smf, f123, sweep, n_full, n_scanned);
             }
         }
     }
@@ -1819,22 +1983,52 @@ hipError_t launch_topk(const DevState &s, hipStream_t st) {
     if (s.V > s.VD)
         hipLaunchKernelGGL(k_zero_order, dim3(grid_for(s.K, 64, 1 << 16)), dim3(256),
                            (size_t)s.K * sizeof(float) + 64 * sizeof(uint32_t), st, s.inv, s.K, s.beta, s.zero_order);
-    const WaveGrid g = wave_grid(s, (size_t)s.V, wave_hist_bytes(s.K), 1 << 16);
-    hipLaunchKernelGGL(k_word_top, dim3(g.grid), dim3(64 * g.nwv), g.nwv * wave_hist_bytes(s.K), st,
-                       s.V, s.VD, s.K, s.beta, s.wt_head, s.tail_off, s.wt_tail, s.tail_nnz, s.inv, s.zero_order,
-                       s.wtop);
+    // head words need no LDS: where the tail words' count tables limit the
+    // wavefronts per CU, they get a launch of their own at full occupancy
+    const bool split = s.VD > 0 && s.V > s.VD && (size_t)s.lds_cu / wave_hist_bytes(s.K) < 16;
+    const int w_tail = split || s.V == s.VD ? s.VD : 0;  // first word of the LDS launch
+    if (w_tail > 0) {
+        const WaveGrid g = wave_grid(s, (size_t)w_tail, 0, 1 << 16);
+        hipLaunchKernelGGL(k_word_top, dim3(g.grid), dim3(64 * g.nwv), 0, st, 0, w_tail, s.VD, s.K, s.beta,
+                           s.wt_head, s.tail_off, s.wt_tail, s.tail_nnz, s.inv, s.zero_order, s.wtop);
+    }
+    if (s.V > w_tail) {
+        const WaveGrid g = wave_grid(s, (size_t)(s.V - w_tail), wave_hist_bytes(s.K), 1 << 16);
+        hipLaunchKernelGGL(k_word_top, dim3(g.grid), dim3(64 * g.nwv), g.nwv * wave_hist_bytes(s.K), st, w_tail, s.V,
+                           s.VD, s.K, s.beta, s.wt_head, s.tail_off, s.wt_tail, s.tail_nnz, s.inv, s.zero_order,
+                           s.wtop);
+    }
     return hipGetLastError();
 }
 
 hipError_t launch_build_dt(const DevState &s, hipStream_t st) {
     if (s.D == 0) return hipSuccess;
-    const WaveGrid g = wave_grid(s, (size_t)s.D, dt_wave_bytes(s.K), 1 << 18);
-    if (s.compact)
-        hipLaunchKernelGGL(k_dt_build<true>, dim3(g.grid), dim3(64 * g.nwv), g.nwv * dt_wave_bytes(s.K), st, s.D, s.K, s.VD, s.topic_bits, s.soff, s.slot_wrd, s.slot_tok, s.z,
-                           s.z_doc, s.wtop, s.aux_doc, s.doc_info, s.dt);
-    else
-        hipLaunchKernelGGL(k_dt_build<false>, dim3(g.grid), dim3(64 * g.nwv), g.nwv * dt_wave_bytes(s.K), st, s.D, s.K, s.VD, s.topic_bits, s.soff, s.slot_wrd, s.slot_tok, s.z,
-                           s.z_doc, s.wtop, s.aux_doc, s.doc_info, s.dt);
+    // where a K-sized histogram per wavefront leaves fewer than 16 on a CU,
+    // the short documents (the last ones: docs come longest first) are sorted
+    // instead
+    const bool sort_short = (size_t)s.lds_cu / dt_wave_bytes(s.K) < 16;
+    const int q1 = sort_short ? s.D_long : s.D;
+    if (q1 > 0) {
+        const WaveGrid g = wave_grid(s, (size_t)q1, dt_wave_bytes(s.K), 1 << 18);
+        if (s.compact)
+            hipLaunchKernelGGL(k_dt_build<true>, dim3(g.grid), dim3(64 * g.nwv), g.nwv * dt_wave_bytes(s.K), st, 0, q1,
+                               s.K, s.VD, s.topic_bits, s.soff, s.slot_wrd, s.slot_tok, s.z, s.z_doc, s.wtop, s.aux_doc,
+                               s.doc_info, s.dt);
+        else
+            hipLaunchKernelGGL(k_dt_build<false>, dim3(g.grid), dim3(64 * g.nwv), g.nwv * dt_wave_bytes(s.K), st, 0, q1,
+                               s.K, s.VD, s.topic_bits, s.soff, s.slot_wrd, s.slot_tok, s.z, s.z_doc, s.wtop, s.aux_doc,
+                               s.doc_info, s.dt);
+    }
+    if (q1 < s.D) {
+        const size_t lds = 4 * kDtShort * sizeof(uint32_t);  // four wavefronts, 2 KB each
+        const unsigned grid = (unsigned)grid_for((size_t)(s.D - q1), 4, 1 << 18);
+        if (s.compact)
+            hipLaunchKernelGGL(k_dt_sort<true>, dim3(grid), dim3(256), lds, st, q1, s.D, s.VD, s.topic_bits, s.soff,
+                               s.slot_wrd, s.slot_tok, s.z, s.z_doc, s.wtop, s.aux_doc, s.doc_info, s.dt);
+        else
+            hipLaunchKernelGGL(k_dt_sort<false>, dim3(grid), dim3(256), lds, st, q1, s.D, s.VD, s.topic_bits, s.soff,
+                               s.slot_wrd, s.slot_tok, s.z, s.z_doc, s.wtop, s.aux_doc, s.doc_info, s.dt);
+    }
     return hipGetLastError();
 }
 
@@ -1846,10 +2040,12 @@ size_t sample_lds_bytes(int K, int bd) {
 }
 
 size_t sampler_static_lds() {
-    const void *fs[4] = {reinterpret_cast<const void *>(&k_sample<true, true>),
-                         reinterpret_cast<const void *>(&k_sample<true, false>),
-                         reinterpret_cast<const void *>(&k_sample<false, true>),
-                         reinterpret_cast<const void *>(&k_sample<false, false>)};
+    const void *fs[6] = {reinterpret_cast<const void *>(&k_sample<true, true, true>),
+                         reinterpret_cast<const void *>(&k_sample<true, true, false>),
+                         reinterpret_cast<const void *>(&k_sample<true, false, false>),
+                         reinterpret_cast<const void *>(&k_sample<false, true, true>),
+                         reinterpret_cast<const void *>(&k_sample<false, true, false>),
+                         reinterpret_cast<const void *>(&k_sample<false, false, false>)};
     size_t m = 0;
     for (const void *f : fs) {
         hipFuncAttributes a;
@@ -1907,14 +2103,19 @@ hipError_t launch_sample(const DevState &s, bool shortcut, hipStream_t st, uint3
                            (size_t)kQueueStride * A.n_queues, 0u);
         grid = (unsigned)s.grid_dim;
     }
-    if (shortcut && s.compact)
-        hipLaunchKernelGGL((k_sample<true, true>), dim3(grid), dim3(bd), lds, st, A);
+    const bool qtab = s.K <= EZ_Q_TABLE_MAX_K;  // implies compact rows
+    if (shortcut && qtab)
+        hipLaunchKernelGGL((k_sample<true, true, true>), dim3(grid), dim3(bd), lds, st, A);
+    else if (shortcut && s.compact)
+        hipLaunchKernelGGL((k_sample<true, true, false>), dim3(grid), dim3(bd), lds, st, A);
     else if (shortcut)
-        hipLaunchKernelGGL((k_sample<true, false>), dim3(grid), dim3(bd), lds, st, A);
+        hipLaunchKernelGGL((k_sample<true, false, false>), dim3(grid), dim3(bd), lds, st, A);
+    else if (qtab)
+        hipLaunchKernelGGL((k_sample<false, true, true>), dim3(grid), dim3(bd), lds, st, A);
     else if (s.compact)
-        hipLaunchKernelGGL((k_sample<false, true>), dim3(grid), dim3(bd), lds, st, A);
+        hipLaunchKernelGGL((k_sample<false, true, false>), dim3(grid), dim3(bd), lds, st, A);
     else
-        hipLaunchKernelGGL((k_sample<false, false>), dim3(grid), dim3(bd), lds, st, A);
+        hipLaunchKernelGGL((k_sample<false, false, false>), dim3(grid), dim3(bd), lds, st, A);
     return hipGetLastError();
 }
 

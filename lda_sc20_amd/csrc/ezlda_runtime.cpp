@@ -122,10 +122,17 @@ int device_lds_per_cu() {
 // Validates the layout arguments of `a` (those build_layout reads) and
 // fills their defaults; returns an error message or "".
 std::string check_layout_args(ezlda_args &a) {
-    // auto: one-wavefront workgroups, sixteen per CU, where sixteen samplers'
-    // LDS fits in a CU at this K (K <= ~1200 on MI355X), else 128 threads
-    if (a.block_dim <= 0)
-        a.block_dim = 16 * ez::sample_lds_bytes(a.K, 64) <= (size_t)device_lds_per_cu() ? 64 : 128;
+    // auto: the smallest workgroup (64, 128, ... threads) with which the
+    // sampler workgroups whose LDS fits in a CU hold at least 12 wavefronts
+    // (measured best for K = 1024 to 16383: 64 threads up to K ~ 2400, then
+    // 128, 256 at K = 10000, 512 at K = 16383)
+    if (a.block_dim <= 0) {
+        const size_t lds = ez::sample_lds_bytes(a.K, 64);
+        const int fit = std::max(1, (int)((size_t)device_lds_per_cu() / std::max<size_t>(lds, 1)));
+        int bd = 64;
+        while (bd < 1024 && fit * (bd / 64) < 12) bd *= 2;
+        a.block_dim = bd;
+    }
     if (a.block_dim % 64 != 0 || a.block_dim > 1024) return "block_dim must be a multiple of 64 and <= 1024";
     if (a.token_segment <= 0) a.token_segment = 4096;
     // auto: 16 persistent workgroups per CU at 64 threads, else 8
@@ -519,6 +526,8 @@ int load_chunked(ezlda_ctx *c, const std::vector<uint32_t> &tok_doc, const std::
         }
         for (int r = 0; r < V; ++r) woff[r + 1] += woff[r];
         for (uint32_t q = 0; q <= ch.D; ++q) soffc[q] = c->soff[ch.q0 + q] - ch.slot0;
+        ch.D_long = 0;
+        while ((uint32_t)ch.D_long < ch.D && soffc[ch.D_long + 1] - soffc[ch.D_long] > ez::kDtShort) ++ch.D_long;
         std::vector<uint2> di(ch.D);
         uint64_t dtc = 0;
         for (uint32_t q = 0; q < ch.D; ++q) {
@@ -636,6 +645,7 @@ ez::DevState chunk_state(const ezlda_ctx *c, int k, int j) {
     ez::DevState cs = c->s;
     uint8_t *img = c->cimg[j];
     cs.D = (int32_t)ch.D;
+    cs.D_long = ch.D_long;
     cs.N = ch.N;
     cs.VD = c->s.V;
     cs.word_off = nullptr;
@@ -1008,6 +1018,8 @@ int ezlda_load_docs(ezlda_ctx *c, int32_t V, int32_t D, const int64_t *doc_off, 
     s.alpha = c->a.alpha; s.beta = c->a.beta;
     s.vbeta = (float)((double)V * (double)c->a.beta);
     s.seed = c->a.seed;
+    s.D_long = 0;  // docs come longest first
+    while (s.D_long < D && c->soff[s.D_long + 1] - c->soff[s.D_long] > ez::kDtShort) ++s.D_long;
     c->dt_cap = (int64_t)dtc;
     c->dt_cap_canon = 0;
     for (int d = 0; d < D; ++d) c->dt_cap_canon += dt_cap(c->doc_off[d + 1] - c->doc_off[d], K);

@@ -337,3 +337,33 @@ def test_launcher_stages_equal_fused_sampler(order, kw):
         assert np.array_equal(ctx.get_wt_dense()[head], orc.get_wt_dense()[head]), (order, s)
     ctx.wt_rowsum()
     assert np.array_equal(ctx.get_nk(), orc.get_nk())
+
+
+@pytest.mark.parametrize("K", [8192, 10000, 16383])
+def test_dt_rows_and_aux_of_sorted_short_docs(K):
+    """At large K the DT rows of documents of at most 512 tokens are built by
+    sorting their topics, longer ones by a K-sized histogram: both give the
+    oracle's rows (with count pieces at K=8192) and per-token counts."""
+    rng = np.random.default_rng(K)
+    D, V = 120, 500
+    lens = rng.integers(1, 1200, D)
+    lens[:6] = [1, 2, 63, 511, 512, 513]
+    off = np.concatenate([[0], np.cumsum(lens)]).astype(np.int64)
+    words = ((rng.zipf(1.2, int(off[-1])) - 1) % V).astype(np.int32)
+    # few topics per doc, so counts exceed a compact entry's piece
+    z0 = (rng.integers(0, 6, len(words)) * (K // 7)).astype(np.uint16)
+    ctx, orc = _pair(off, words, V, K, dense_threshold=50, z0=z0)
+    doff = ezlda.dt_offsets(off, K)
+    for s in range(2):
+        for stage in ("build_wt", "topk", "build_dt"):
+            getattr(ctx, stage)()
+            getattr(orc, stage)()
+        ge, gn = ctx.get_dt_raw()
+        oe, on = orc.get_dt_raw()
+        assert np.array_equal(gn, on), (K, s)
+        for d in range(D):
+            assert np.array_equal(ge[doff[d]: doff[d] + gn[d]], oe[doff[d]: doff[d] + on[d]]), (K, s, d)
+        assert np.array_equal(ctx.get_aux(), orc.get_aux()), (K, s)
+        ctx.sample()
+        orc.sample()
+        assert np.array_equal(ctx.get_topics(), orc.get_topics()), (K, s)
