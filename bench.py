@@ -457,7 +457,11 @@ def main() -> int:
                + 2 * N + 2 * N + 4 * K * VD + 4 * tail_nnz                  # n_k + WT rebuild
                + 4 * K * VD + 4 * tail_nnz + 8 * info.num_words             # top-3
                + 2 * N + 4 * N + 4 * N + eb * dt_nnz + 8 * info.num_docs)   # DT + aux (z_doc, word, aux)
-    ll = ctx.loglik()
+    ll = ctx.loglik()  # first call rebuilds the counts of the final topics
+    device_sync()
+    t = time.perf_counter()
+    ll = ctx.loglik()  # the FP64 log-likelihood pass alone (DT rebuild + reduction), timed
+    loglik_ms = (time.perf_counter() - t) * 1e3
     # the committed rocprof measurement of k_sample, used only when it was
     # taken on this configuration over this sweep window (the topics, hence
     # the work, are then the same, sweep by sweep)
@@ -498,6 +502,7 @@ def main() -> int:
         "dt_entries_per_full_draw": (n_scanned / n_full) if n_full else 0.0,
         "graph_ms_per_step": graph_ms,
         "perplexity_after": math.exp(-ll / N),
+        "loglik_ms": loglik_ms,
         "wall_ms_timed_region": wall * 1e3,
     }
     if tuning is not None:
