@@ -690,9 +690,17 @@ __global__ void __launch_bounds__(256) k_dt_sort(int q0, int q1, int VD, uint32_
 
 // ------------------------------------------------------------- sampler
 // Checkpoints of the S running sum kept per lane (one per block of kBlkU4
-// 16-byte loads, i.e. 4*kBlkU4 DT entries).
+// 16-byte loads, i.e. 4*kBlkU4 DT entries, and with kHalfCp one more in the
+// middle of each block, so the S-bucket search re-reads half a block).
 static constexpr int kCP = EZ_SAMPLE_CP;
 static constexpr int kBlkU4 = EZ_SAMPLE_BLK;
+#ifndef EZ_SAMPLE_HALF_CP
+#define EZ_SAMPLE_HALF_CP 1
+#endif
+static constexpr bool kHalfCp = EZ_SAMPLE_HALF_CP != 0 && kBlkU4 % 2 == 0;
+// Uniforms kept in registers per run between the shortcut test and the draw
+// (tokens past them draw theirs again).
+static constexpr int kUKeep = 4;
 
 // Running FMA sum over one block of a DT row: kBlkU4 16-byte loads issued
 // together, then the entries in order.
@@ -707,26 +715,104 @@ __device__ __forceinline__ RowBlock load_block(const uint4 *__restrict__ p) {
     return r;
 }
 
+// The two compact entries of a dword (low half first) decoded together: both
+// counts by one packed 16-bit shift, both topics by one mask (tm2 = the topic
+// mask in each half), each phi address (topic * 4) by one shift that reads
+// its half of the word (SDWA word select; the compiler forms it for the high
+// half by itself, the low half is written out).
+typedef unsigned short ez_us2 __attribute__((ext_vector_type(2)));
+struct C16x2 {
+    float c0, c1;
+    uint32_t a0, a1;
+};
+__device__ __forceinline__ uint32_t lo16_times4(uint32_t t) {
+    uint32_t r;
+    asm("v_lshlrev_b32_sdwa %0, %1, %2 dst_sel:DWORD dst_unused:UNUSED_PAD src0_sel:DWORD src1_sel:WORD_0"
+        : "=v"(r)
+        : "v"(2u), "v"(t));
+    return r;
+}
+__device__ __forceinline__ C16x2 c16_x2(uint32_t w, uint32_t tb, uint32_t tm2) {
+    const ez_us2 c = __builtin_bit_cast(ez_us2, w) >> (ez_us2)((unsigned short)tb);
+    const uint32_t t = w & tm2;
+    return C16x2{(float)c.x, (float)c.y, lo16_times4(t), (t >> 16) << 2};
+}
+__device__ __forceinline__ float phi_a(uint32_t a) { return *(lds_cfloat *)a; }
+
+// The entries of one 16-byte load, in order.
 template <bool COMPACT>
-__device__ __forceinline__ float sum_block(const RowBlock &r, float s, uint32_t tb) {
-    const uint4 *e = r.e;
+__device__ __forceinline__ float sum_u4(const uint4 &e, float s, uint32_t tb) {
+    if (COMPACT) {  // two 16-bit entries per dword, low half first
+        const uint32_t w[4] = {e.x, e.y, e.z, e.w};
+        const uint32_t tm2 = ((1u << tb) - 1u) * 0x10001u;
 #pragma unroll
-    for (int i = 0; i < kBlkU4; ++i) {
-        if (COMPACT) {  // two 16-bit entries per dword, low half first
-            const uint32_t w[4] = {e[i].x, e[i].y, e[i].z, e[i].w};
-#pragma unroll
-            for (int q = 0; q < 4; ++q) {
-                s = __builtin_fmaf(c16_count(w[q], tb), phi_c16(w[q], tb), s);
-                s = __builtin_fmaf((float)(w[q] >> (16u + tb)), phi_c16(w[q] >> 16, tb), s);
-            }
-        } else {
-            s = __builtin_fmaf(dt_count(e[i].x), phi_at0(e[i].x), s);
-            s = __builtin_fmaf(dt_count(e[i].y), phi_at0(e[i].y), s);
-            s = __builtin_fmaf(dt_count(e[i].z), phi_at0(e[i].z), s);
-            s = __builtin_fmaf(dt_count(e[i].w), phi_at0(e[i].w), s);
+        for (int q = 0; q < 4; ++q) {
+            const C16x2 d = c16_x2(w[q], tb, tm2);
+            s = __builtin_fmaf(d.c0, phi_a(d.a0), s);
+            s = __builtin_fmaf(d.c1, phi_a(d.a1), s);
         }
+    } else {
+        s = __builtin_fmaf(dt_count(e.x), phi_at0(e.x), s);
+        s = __builtin_fmaf(dt_count(e.y), phi_at0(e.y), s);
+        s = __builtin_fmaf(dt_count(e.z), phi_at0(e.z), s);
+        s = __builtin_fmaf(dt_count(e.w), phi_at0(e.w), s);
     }
     return s;
+}
+
+// The block's entries in order; `half` gets the running sum after its
+// first kBlkU4 / 2 loads.
+template <bool COMPACT>
+__device__ __forceinline__ float sum_block(const RowBlock &r, float s, uint32_t tb, float &half) {
+#pragma unroll
+    for (int i = 0; i < kBlkU4; ++i) {
+        s = sum_u4<COMPACT>(r.e[i], s, tb);
+        if (i == kBlkU4 / 2 - 1) half = s;
+    }
+    return s;
+}
+
+// S-bucket search over n consecutive 16-byte loads of a row (issued
+// together) from running sum `run`: the first entry whose running sum
+// exceeds t2 (running sums never decrease).
+template <bool COMPACT, int N>
+__device__ __forceinline__ void search_u4(const uint4 *__restrict__ p, float &run, float t2, uint32_t tb,
+                                          uint32_t &newz, bool &found) {
+    uint4 blk[N];
+#pragma unroll
+    for (int i = 0; i < N; ++i) blk[i] = p[i];
+#pragma unroll
+    for (int gi = 0; gi < N; ++gi) {
+        const uint4 e = blk[gi];
+        if (COMPACT) {
+            const uint32_t w[4] = {e.x, e.y, e.z, e.w};
+            const uint32_t tm2 = ((1u << tb) - 1u) * 0x10001u;
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                const C16x2 d = c16_x2(w[q], tb, tm2);
+                const float ra = __builtin_fmaf(d.c0, phi_a(d.a0), run);
+                const float rb = __builtin_fmaf(d.c1, phi_a(d.a1), ra);
+                if (!found && ra > t2) {
+                    newz = d.a0 >> 2;
+                    found = true;
+                } else if (!found && rb > t2) {
+                    newz = d.a1 >> 2;
+                    found = true;
+                }
+                run = rb;
+            }
+        } else if (!found) {
+            const float r0 = __builtin_fmaf(dt_count(e.x), phi_at0(e.x), run);
+            const float r1 = __builtin_fmaf(dt_count(e.y), phi_at0(e.y), r0);
+            const float r2 = __builtin_fmaf(dt_count(e.z), phi_at0(e.z), r1);
+            const float r3 = __builtin_fmaf(dt_count(e.w), phi_at0(e.w), r2);
+            if (r3 > t2) {
+                newz = dt_topic(r0 > t2 ? e.x : r1 > t2 ? e.y : r2 > t2 ? e.z : e.w);
+                found = true;
+            }
+            run = r3;
+        }
+    }
 }
 
 // Sampler work item: a word, or a doc tile / token segment of a head word,
@@ -871,14 +957,30 @@ __device__ __forceinline__ void sample_item(const SampleArgs &A, uint32_t item, 
         const uint2 di = A.doc_info[rn.z];
         const uint32_t n1 = a & 0xFFFFu, n2 = a >> 16, nd = di.y >> 16;
         const float P1 = ((float)n1 + alpha) * f1;
-        // tokens of the run that need the full draw; shortcut tokens take k1
+        // tokens of the run that need the full draw; shortcut tokens take k1.
+        // The uniforms of the first kUKeep tokens are kept for the draw.
         uint32_t need = 0;
-        for (uint32_t j = 0; j < len; ++j) {
+        float uk[kUKeep];
+        const float maxS = (float)n2 * f2 + (float)(nd - n1 - n2) * f3;
+        const float B = (P1 + maxS) + Q;
+#pragma unroll
+        for (uint32_t j = 0; j < (uint32_t)kUKeep; ++j) {
+            uk[j] = 0.0f;
+            if (j < len) {
+                const float u = ez_u24(ez_philox_x0(A.seed, t0 + j + sg.w, sweep, EZ_TAG_SAMPLE));
+                uk[j] = u;
+                if (!SHORTCUT || !(u * B < P1 * kMargin)) {
+                    need |= 1u << j;
+                } else {
+                    A.z[t0 + j] = (uint16_t)k1;
+                    if (head) A.z_doc[p0 + j] = (uint16_t)k1;
+                }
+            }
+        }
+        for (uint32_t j = kUKeep; j < len; ++j) {
             bool full = true;
             if (SHORTCUT) {
                 const float u = ez_u24(ez_philox_x0(A.seed, t0 + j + sg.w, sweep, EZ_TAG_SAMPLE));
-                const float maxS = (float)n2 * f2 + (float)(nd - n1 - n2) * f3;
-                const float B = (P1 + maxS) + Q;
                 full = !(u * B < P1 * kMargin);
             }
             if (full) {
@@ -902,21 +1004,23 @@ __device__ __forceinline__ void sample_item(const SampleArgs &A, uint32_t item, 
         // re-reads one block instead of the row up to the target
         // Block b + 1 is loaded before block b is summed (only by lanes
         // whose row has it, so a wavefront stops loading at its longest row).
-        float cp[kCP];
+        float cp[kCP], ch[kCP];  // running sum at the end / the middle of block b
         float S = 0.0f;
         RowBlock cur = load_block(rp);
 #pragma unroll
         for (int b = 0; b < kCP; ++b) {
             RowBlock nxt;
             if ((uint32_t)b + 1u < nb) nxt = load_block(rp + kBlkU4 * (b + 1));
-            if ((uint32_t)b < nb) S = sum_block<COMPACT>(cur, S, tb);
+            ch[b] = S;
+            if ((uint32_t)b < nb) S = sum_block<COMPACT>(cur, S, tb, ch[b]);
             cp[b] = S;
             cur = nxt;
         }
         for (uint32_t b = kCP; b < nb; ++b) {
             RowBlock nxt;
             if (b + 1u < nb) nxt = load_block(rp + kBlkU4 * (b + 1u));
-            S = sum_block<COMPACT>(cur, S, tb);
+            float unused;
+            S = sum_block<COMPACT>(cur, S, tb, unused);
             cur = nxt;
         }
         const float Aa = P1 + S;
@@ -925,7 +1029,11 @@ __device__ __forceinline__ void sample_item(const SampleArgs &A, uint32_t item, 
             const uint32_t j = (uint32_t)__ffs(need) - 1u;
             need &= need - 1u;
             const uint32_t t = t0 + j;
-            const float u = ez_u24(ez_philox_x0(A.seed, t + sg.w, sweep, EZ_TAG_SAMPLE));
+            float u = uk[0];
+#pragma unroll
+            for (int i = 1; i < kUKeep; ++i)
+                if (j == (uint32_t)i) u = uk[i];
+            if (j >= (uint32_t)kUKeep) u = ez_u24(ez_philox_x0(A.seed, t + sg.w, sweep, EZ_TAG_SAMPLE));
             const float target = u * total;
             uint32_t newz = k1;
             if (target < P1) {
@@ -937,55 +1045,34 @@ __device__ __forceinline__ void sample_item(const SampleArgs &A, uint32_t item, 
                 for (int b = kCP - 1; b >= 0; --b)
                     if (cp[b] > t2) bs = b;
                 float run = 0.0f;
-                uint32_t b0 = 0, b1 = nb;
+                bool found = false;
                 if (bs >= 0) {
+                    // the checkpointed block that holds the target: from the
+                    // running sum before it, or with kHalfCp from the one in
+                    // its middle, at most half the block's loads
+                    float mid = ch[0];
 #pragma unroll
                     for (int b = 0; b < kCP - 1; ++b)
-                        if (b + 1 == bs) run = cp[b];
-                    b0 = (uint32_t)bs;
-                    b1 = b0 + 1u;
-                } else if (nb > (uint32_t)kCP) {
-                    run = cp[kCP - 1];
-                    b0 = kCP;
-                } else {
-                    b1 = 0;  // no running sum exceeds t2
-                }
-                bool found = false;
-                // the block(s) from the checkpoint, all four loads at once; the
-                // first running sum above t2 (running sums never decrease)
-                for (uint32_t gb = b0; gb < b1 && !found; ++gb) {
-                    const RowBlock blk = load_block(rp + kBlkU4 * gb);
-#pragma unroll
-                    for (int gi = 0; gi < kBlkU4; ++gi) {
-                        const uint4 e = blk.e[gi];
-                        if (COMPACT) {
-                            const uint32_t w[4] = {e.x, e.y, e.z, e.w};
-#pragma unroll
-                            for (int q = 0; q < 4; ++q) {
-                                const float ra = __builtin_fmaf(c16_count(w[q], tb), phi_c16(w[q], tb), run);
-                                const float rb =
-                                    __builtin_fmaf((float)(w[q] >> (16u + tb)), phi_c16(w[q] >> 16, tb), ra);
-                                if (!found && ra > t2) {
-                                    newz = c16_topic(w[q], tb);
-                                    found = true;
-                                } else if (!found && rb > t2) {
-                                    newz = c16_topic(w[q] >> 16, tb);
-                                    found = true;
-                                }
-                                run = rb;
-                            }
-                        } else if (!found) {
-                            const float r0 = __builtin_fmaf(dt_count(e.x), phi_at0(e.x), run);
-                            const float r1 = __builtin_fmaf(dt_count(e.y), phi_at0(e.y), r0);
-                            const float r2 = __builtin_fmaf(dt_count(e.z), phi_at0(e.z), r1);
-                            const float r3 = __builtin_fmaf(dt_count(e.w), phi_at0(e.w), r2);
-                            if (r3 > t2) {
-                                newz = dt_topic(r0 > t2 ? e.x : r1 > t2 ? e.y : r2 > t2 ? e.z : e.w);
-                                found = true;
-                            }
-                            run = r3;
+                        if (b + 1 == bs) {
+                            run = cp[b];
+                            mid = ch[b + 1];
                         }
+                    const uint4 *bp = rp + kBlkU4 * (uint32_t)bs;
+                    if (kHalfCp) {
+                        if (mid > t2) {
+                            search_u4<COMPACT, kBlkU4 / 2>(bp, run, t2, tb, newz, found);
+                        } else {
+                            run = mid;
+                            search_u4<COMPACT, kBlkU4 / 2>(bp + kBlkU4 / 2, run, t2, tb, newz, found);
+                        }
+                    } else {
+                        search_u4<COMPACT, kBlkU4>(bp, run, t2, tb, newz, found);
                     }
+                } else if (nb > (uint32_t)kCP) {
+                    // past the checkpoints: the remaining blocks in order
+                    run = cp[kCP - 1];
+                    for (uint32_t gb = kCP; gb < nb && !found; ++gb)
+                        search_u4<COMPACT, kBlkU4>(rp + kBlkU4 * gb, run, t2, tb, newz, found);
                 }
                 if (!found) {  // rounding left every running sum at or below t2
                     if (COMPACT) {
