@@ -74,11 +74,10 @@ def device_sync() -> None:
 
 def measured_traffic(workload: str, run: dict):
     """Bytes per k_sample launch that crossed the L2 boundary, measured by
-    rocprofv3 on this workload (read requests by size, then WRITE_SIZE; both
-    checked against calibration kernels of known bytes), with where it came
-    from -- or None when the committed measurement was taken with another
-    configuration, DT row format or sweep window than this run's (its
-    `config` must match `run` key by key)."""
+    rocprofv3 on this workload (FETCH_SIZE then WRITE_SIZE, each scaled by a
+    calibration kernel of known bytes), with where it came from -- or None
+    when the committed measurement was taken with another configuration or
+    sweep window than this run's (its `config` must match `run` key by key)."""
     path = os.path.join(ROOT, "profiles", f"hbm_traffic_{workload}.json")
     try:
         with open(path) as f:
@@ -445,11 +444,10 @@ def main() -> int:
     # token and by doc slot); per run (a word's tokens in one doc) its entry,
     # max-topic counts and doc info (16 + 4 + 8); phi staging once per word;
     # one read of every scanned DT entry (one scan per run with a full draw;
-    # 1-byte entries, 30 per 32-byte half-block after its 2-byte base topic,
-    # when K <= 1024; 2-byte entries when K <= 8192, else 4)
+    # 2-byte entries when K <= 8192, else 4)
     VD = int(info.num_head_words)
     R = int(info.num_runs)
-    eb = 32 / 30 if K <= 1024 else 2 if K <= 8192 else 4
+    eb = 2 if K <= 8192 else 4
     s_scanned = n_scanned / K_steps
     b_sample = 4 * N + 28 * R + 4 * K * VD + 4 * tail_nnz + 4 * K + eb * s_scanned
     t_sample = stage_ms["sample"] / K_steps / 1e3 if not chunked else None
@@ -469,7 +467,7 @@ def main() -> int:
     # the work, are then the same, sweep by sweep)
     run_cfg = {"workload": args.workload, "K": K, "num_head_words": VD, "run_cap": int(info.run_cap),
                "grid_dim": int(info.grid_dim), "block_dim": int(info.block_dim), "doc_tiles": int(info.doc_tiles),
-               "sweeps_before_timing": sweeps_before, "steps": K_steps, "dt_entry_bytes": eb}
+               "sweeps_before_timing": sweeps_before, "steps": K_steps}
     tr = None if chunked else measured_traffic(args.workload, run_cfg)
     out = {
         "metric": "sampled tokens/s per sweep at K=1024 (+% HBM peak)" if K == 1024 else
@@ -489,8 +487,7 @@ def main() -> int:
                    "num_head_words": VD, "shortcut": True, "sweeps_before_timing": sweeps_before,
                    "doc_tiles": int(info.doc_tiles), "run_cap": int(info.run_cap), "num_runs": int(info.num_runs),
                    "grid_dim": int(info.grid_dim), "block_dim": int(info.block_dim),
-                   "token_segment": int(info.token_segment), "num_chunks": int(info.num_chunks),
-                   "dt_entry_bytes": eb},
+                   "token_segment": int(info.token_segment), "num_chunks": int(info.num_chunks)},
         "roofline": None if chunked else
                     {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": tr["bytes_per_launch"] if tr else None,

@@ -99,9 +99,9 @@ typedef struct ezlda_info {
     int32_t K;
     int64_t dt_capacity;     /* entries of ezlda_get_dt_raw: sum_d of min(n_d, K),
                                 plus n_d / p when K <= 8192 (rows then hold a
-                                count in pieces of at most p: 7 up to K = 1024
-                                (8-bit rows), else 2^(16 - tb) - 1 with tb =
-                                ceil(log2 K): 31, 15, 7 up to 2048, 4096, 8192) */
+                                count in pieces of at most p = 2^(16 - tb) - 1,
+                                tb = max(10, ceil(log2 K)): 63 up to K = 1024,
+                                31, 15, 7 up to 2048, 4096, 8192) */
     int64_t wt_tail_capacity;/* sum_tail min(n_w, K) */
     int32_t num_sample_items;/* sampler work items (word segments) */
     uint32_t sweep;          /* sweeps done (the Philox sweep counter) */

@@ -15,10 +15,6 @@ static constexpr int kCounterWords = kQueueStride * 9;
 // Documents of at most this many tokens can have their DT rows built by
 // sorting their topics (large K); longer ones use a K-sized histogram.
 static constexpr uint32_t kDtShort = 512;
-// DT rows hold 8-bit entries up to kByteK, 16-bit entries up to kCompactK,
-// 32-bit entries above it.
-static constexpr uint32_t kByteK = 1024;
-static constexpr uint32_t kCompactK = 8192;
 
 // Device image of one corpus + model.  Token ids are word-major positions;
 // N < 2^32.  Sizes: V words, VD head words, D docs, K topics.
@@ -40,10 +36,8 @@ struct DevState {
     // doc-topic rows
     uint2 *doc_info = nullptr;      // D, by doc q: x = DT row offset, y = nnz | len << 16
     int32_t D_long = 0;             // docs of more than kDtShort tokens (the first D_long in doc order)
-    uint32_t *dt = nullptr;         // rows: u32 (topic << 2) | count << 16, u16 topic | piece << topic_bits, or
-                                    // 8-bit: 32-byte half-blocks of a u16 base topic and 30 delta << 3 | count bytes
-    bool compact = false;           // 1024 < K <= 8192: 16-bit entries topic | piece << topic_bits
-    uint32_t row_fmt = 0;           // 0: 32-bit entries, 1: 16-bit (compact), 2: 8-bit (K <= kByteK)
+    uint32_t *dt = nullptr;         // rows: u32 (topic << 2) | count << 16, or (compact) u16 topic | piece << 10
+    bool compact = false;           // K <= 8192: 16-bit entries topic | piece << topic_bits
     uint32_t topic_bits = 16;       // max(10, ceil(log2 K)); pieces of <= 2^(16 - topic_bits) - 1
     // word-topic: dense head rows + sorted sparse tail rows
     int32_t *wt_head = nullptr;     // VD*K

@@ -22,6 +22,13 @@
 #ifndef EZ_SAMPLE_CP
 #define EZ_SAMPLE_CP 12
 #endif
+// Largest K whose sampler keeps a table of the Q running sums inside each
+// 64-topic slab in LDS (a Q draw then bisects it); above it a Q draw
+// recomputes its slab's running sums, which halves the sampler's LDS.
+#ifndef EZ_Q_TABLE_MAX_K
+#define EZ_Q_TABLE_MAX_K 1024
+#endif
+static_assert(EZ_Q_TABLE_MAX_K <= 8192, "the Q-table sampler variants assume compact (K <= 8192) DT rows");
 
 namespace ez {
 
@@ -93,11 +100,6 @@ typedef __attribute__((address_space(3))) const float lds_cfloat;
 __device__ __forceinline__ float phi_at0(uint32_t e) { return *(lds_cfloat *)(e & 0xFFFCu); }
 // phi of a compact entry (the 16-bit entry in the low half of e)
 __device__ __forceinline__ float phi_c16(uint32_t e, uint32_t tb) { return *(lds_cfloat *)(c16_topic(e, tb) << 2); }
-
-// Row formats of the DT rows (DevState::row_fmt), and the entries of a
-// 32-byte half-block of an 8-bit row (after its 2-byte base topic).
-static constexpr int kRow32 = 0, kRow16 = 1, kRow8 = 2;
-static constexpr uint32_t kB8Half = 30;
 
 // 16-bit counters, two per LDS word (counts < 65536 by construction).
 __device__ __forceinline__ void h16_add(uint32_t *h, uint32_t k) {
@@ -401,7 +403,7 @@ hipError_t launch_z_to_doc(const DevState &s, hipStream_t st) {
 // is zero-padded to a multiple of 32 entries.
 static constexpr int kDtU = 8;  // slots per lane per chunk in k_dt_build
 
-template <int RF>
+template <bool COMPACT>
 __global__ void k_dt_build(int q0, int q1, int K, int VD, uint32_t tb, const uint32_t *__restrict__ soff,
                            const uint32_t *__restrict__ slot_wrd, const uint32_t *__restrict__ slot_tok,
                            const uint16_t *__restrict__ z, const uint16_t *__restrict__ z_doc,
@@ -410,9 +412,7 @@ __global__ void k_dt_build(int q0, int q1, int K, int VD, uint32_t tb, const uin
     extern __shared__ uint32_t sh[];
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6, nwv = blockDim.x >> 6;
     const int KH = (K + 1) >> 1, NW = (K + 31) >> 5;
-    constexpr bool COMPACT = RF == kRow16;
-    // largest count of one entry: 7 in 8-bit rows, 2^(16 - tb) - 1 in 16-bit rows
-    const uint32_t piece = RF == kRow8 ? 7u : (1u << (16u - tb)) - 1u;
+    const uint32_t piece = (1u << (16u - tb)) - 1u;  // compact rows: largest count of one entry
     uint32_t *h = sh + wv * (KH + NW);  // 16-bit counts
     uint32_t *bm = h + KH;              // bitmap of the doc's topics
     for (int k = lane; k < KH + NW; k += 64) h[k] = 0;
@@ -486,87 +486,13 @@ __global__ void k_dt_build(int q0, int q1, int K, int VD, uint32_t tb, const uin
             }
         }
         wave_lds_sync();
-        const uint32_t base = doc_info[q].x;
-        uint32_t nz = 0;
-        if (RF == kRow8) {
-            // 8-bit rows: lane j owns topics [32j, 32j + 32) of each 64-word
-            // chunk.  A topic's entries: steps of 31 (count 0) while its gap
-            // from the previous topic (the row starts at topic 0) exceeds 31,
-            // then its count in pieces of 7, the first with the rest of the
-            // gap as delta.  The previous topic of a lane's first one comes
-            // from a wave max-scan of the lanes' last topics; each lane's
-            // entries take their places from a wave prefix sum, and the
-            // entry that opens a half-block writes its base topic.
-            uint8_t *row = reinterpret_cast<uint8_t *>(dt) + base;
-            uint32_t carry = 0;  // the last topic of the previous chunks (0: row start)
-            for (int j = 0; j < NW; j += 64) {
-                const int wi = j + lane;
-                const uint32_t word = wi < NW ? bm[wi] : 0u;
-                const int32_t last = word ? (int32_t)(((uint32_t)wi << 5) + 31u - (uint32_t)__clz(word)) : -1;
-                int32_t pm = last;  // inclusive max-scan over lanes
-                for (int d = 1; d < 64; d <<= 1) {
-                    const int32_t v = __shfl_up(pm, d);
-                    if (lane >= d) pm = max(pm, v);
-                }
-                int32_t before = __shfl_up(pm, 1);
-                if (lane == 0) before = -1;
-                const uint32_t p0 = before >= 0 ? (uint32_t)before : carry;
-                uint32_t m = 0, pt = p0;
-                for (uint32_t x = word; x; x &= x - 1u) {
-                    const uint32_t k = ((uint32_t)wi << 5) + (uint32_t)__ffs(x) - 1u;
-                    const uint32_t g = k - pt;
-                    m += (g ? (g - 1u) / 31u : 0u) + (h16_get(h, k) + piece - 1u) / piece;
-                    pt = k;
-                }
-                uint32_t incl = m;
-                for (int d = 1; d < 64; d <<= 1) {
-                    const uint32_t v = __shfl_up(incl, d);
-                    if (lane >= d) incl += v;
-                }
-                uint32_t pos = nz + incl - m;
-                uint32_t cur = p0;  // the running topic before the lane's next entry
-                auto put = [&](uint32_t delta, uint32_t cnt) {
-                    const uint32_t hb = pos / kB8Half, o = pos - hb * kB8Half;
-                    if (o == 0) *reinterpret_cast<uint16_t *>(row + 32u * hb) = (uint16_t)cur;
-                    row[32u * hb + 2u + o] = (uint8_t)((delta << 3) | cnt);
-                    cur += delta;
-                    ++pos;
-                };
-                for (uint32_t x = word; x; x &= x - 1u) {
-                    const uint32_t k = ((uint32_t)wi << 5) + (uint32_t)__ffs(x) - 1u;
-                    uint32_t g = k - cur;
-                    for (; g > 31u; g -= 31u) put(31u, 0u);
-                    uint32_t c = h16_get(h, k);
-                    const uint32_t c0 = c < piece ? c : piece;
-                    put(g, c0);
-                    for (c -= c0; c > 0u; c -= (c < piece ? c : piece)) put(0u, c < piece ? c : piece);
-                }
-                for (uint32_t x = word; x; x &= x - 1u) {  // the lane's own 16 count words
-                    const uint32_t k = ((uint32_t)wi << 5) + (uint32_t)__ffs(x) - 1u;
-                    h[k >> 1] = 0;
-                }
-                if (wi < NW) bm[wi] = 0;
-                nz += __shfl(incl, 63);
-                const int32_t cl = __shfl(pm, 63);
-                if (cl >= 0) carry = (uint32_t)cl;
-            }
-            wave_lds_sync();
-            // zeros from the last entry to the next 64-byte boundary (count-0
-            // entries; a half-block without entries gets base 0)
-            if (e > b) {
-                const uint32_t hb = nz / kB8Half, o = nz - hb * kB8Half;
-                const uint32_t endb = 32u * hb + (o ? 2u + o : 0u);
-                const uint32_t padb = ((endb + 63u) & ~63u) - endb;
-                if ((uint32_t)lane < padb) row[endb + lane] = 0;
-            }
-            if (lane == 0) doc_info[q].y = nz | ((e - b) << 16);
-            continue;
-        }
         // Compaction over the bitmap: lane j owns topics [32j, 32j + 32) of
         // each 64-word chunk, counts its entries, takes its place from a
         // wave prefix sum and writes them in ascending topic order; then it
         // clears its counts and bitmap word.  Cost follows the doc's topics,
         // not K.
+        const uint32_t base = doc_info[q].x;
+        uint32_t nz = 0;
         for (int j = 0; j < NW; j += 64) {
             const int wi = j + lane;
             const uint32_t word = wi < NW ? bm[wi] : 0u;
@@ -813,94 +739,6 @@ __device__ __forceinline__ C16x2 c16_x2(uint32_t w, uint32_t tb, uint32_t tm2) {
 }
 __device__ __forceinline__ float phi_a(uint32_t a) { return *(lds_cfloat *)a; }
 
-// 8-bit DT rows (K <= kByteK): 32-byte half-blocks, each a 16-bit base topic
-// and kB8Half entries of one byte, delta << 3 | count: an entry's topic is the
-// base plus the deltas up to and including it.  Count-0 entries (steps of 31
-// over wider gaps, padding) add nothing to an FMA chain; counts above 7 are
-// pieces of 7 with delta 0.  Decoding: the counts of a dword by one mask, its
-// deltas times 4 by one shift and mask, then per entry one SDWA byte add to
-// the running topic * 4 (its phi address), one byte-to-float conversion and
-// the FMA (the compiler forms neither byte select by itself).
-template <int J>
-__device__ __forceinline__ float ubyte_f(uint32_t x) {
-    float r;
-    if (J == 0) asm("v_cvt_f32_ubyte0 %0, %1" : "=v"(r) : "v"(x));
-    if (J == 1) asm("v_cvt_f32_ubyte1 %0, %1" : "=v"(r) : "v"(x));
-    if (J == 2) asm("v_cvt_f32_ubyte2 %0, %1" : "=v"(r) : "v"(x));
-    if (J == 3) asm("v_cvt_f32_ubyte3 %0, %1" : "=v"(r) : "v"(x));
-    return r;
-}
-template <int J>
-__device__ __forceinline__ uint32_t add_ubyte(uint32_t a, uint32_t x) {
-    uint32_t r;
-    if (J == 0)
-        asm("v_add_u32_sdwa %0, %1, %2 dst_sel:DWORD dst_unused:UNUSED_PAD src0_sel:DWORD src1_sel:BYTE_0"
-            : "=v"(r) : "v"(a), "v"(x));
-    if (J == 1)
-        asm("v_add_u32_sdwa %0, %1, %2 dst_sel:DWORD dst_unused:UNUSED_PAD src0_sel:DWORD src1_sel:BYTE_1"
-            : "=v"(r) : "v"(a), "v"(x));
-    if (J == 2)
-        asm("v_add_u32_sdwa %0, %1, %2 dst_sel:DWORD dst_unused:UNUSED_PAD src0_sel:DWORD src1_sel:BYTE_2"
-            : "=v"(r) : "v"(a), "v"(x));
-    if (J == 3)
-        asm("v_add_u32_sdwa %0, %1, %2 dst_sel:DWORD dst_unused:UNUSED_PAD src0_sel:DWORD src1_sel:BYTE_3"
-            : "=v"(r) : "v"(a), "v"(x));
-    return r;
-}
-__device__ __forceinline__ uint32_t b8_counts(uint32_t d) { return d & 0x07070707u; }
-__device__ __forceinline__ uint32_t b8_deltas4(uint32_t d) { return (d >> 1) & 0x7C7C7C7Cu; }
-
-// The FMA chain over one half-block, continuing s.
-__device__ __forceinline__ float sum_b8_half(const uint4 &h0, const uint4 &h1, float s) {
-    const uint32_t d[8] = {h0.x, h0.y, h0.z, h0.w, h1.x, h1.y, h1.z, h1.w};
-    uint32_t t4 = (d[0] & 0xFFFFu) << 2;
-#pragma unroll
-    for (int i = 0; i < 8; ++i) {
-        const uint32_t c = b8_counts(d[i]), dl = b8_deltas4(d[i]);
-        if (i != 0) {
-            t4 = add_ubyte<0>(t4, dl);
-            s = __builtin_fmaf(ubyte_f<0>(c), phi_a(t4), s);
-            t4 = add_ubyte<1>(t4, dl);
-            s = __builtin_fmaf(ubyte_f<1>(c), phi_a(t4), s);
-        }
-        t4 = add_ubyte<2>(t4, dl);
-        s = __builtin_fmaf(ubyte_f<2>(c), phi_a(t4), s);
-        t4 = add_ubyte<3>(t4, dl);
-        s = __builtin_fmaf(ubyte_f<3>(c), phi_a(t4), s);
-    }
-    return s;
-}
-
-// S-bucket search over one half-block from running sum `run`: the first
-// entry whose running sum exceeds t2.
-template <int J>
-__device__ __forceinline__ void b8_step(uint32_t c, uint32_t dl, uint32_t &t4, float &run, float t2, uint32_t &newz,
-                                        bool &found) {
-    t4 = add_ubyte<J>(t4, dl);
-    run = __builtin_fmaf(ubyte_f<J>(c), phi_a(t4), run);
-    if (!found && run > t2) {
-        newz = t4 >> 2;
-        found = true;
-    }
-}
-__device__ __forceinline__ void search_b8_half(const uint4 *__restrict__ p, float &run, float t2, uint32_t &newz,
-                                               bool &found) {
-    const uint4 h0 = p[0], h1 = p[1];
-    const uint32_t d[8] = {h0.x, h0.y, h0.z, h0.w, h1.x, h1.y, h1.z, h1.w};
-    uint32_t t4 = (d[0] & 0xFFFFu) << 2;
-#pragma unroll
-    for (int i = 0; i < 8; ++i) {
-        const uint32_t c = b8_counts(d[i]), dl = b8_deltas4(d[i]);
-        if (i != 0) {
-            b8_step<0>(c, dl, t4, run, t2, newz, found);
-            b8_step<1>(c, dl, t4, run, t2, newz, found);
-        }
-        b8_step<2>(c, dl, t4, run, t2, newz, found);
-        b8_step<3>(c, dl, t4, run, t2, newz, found);
-    }
-}
-
-
 // The entries of one 16-byte load, in order.
 template <bool COMPACT>
 __device__ __forceinline__ float sum_u4(const uint4 &e, float s, uint32_t tb) {
@@ -923,17 +761,12 @@ __device__ __forceinline__ float sum_u4(const uint4 &e, float s, uint32_t tb) {
 }
 
 // The block's entries in order; `half` gets the running sum after its
-// first kBlkU4 / 2 loads (for 8-bit rows: after its first half-block).
-template <int RF>
+// first kBlkU4 / 2 loads.
+template <bool COMPACT>
 __device__ __forceinline__ float sum_block(const RowBlock &r, float s, uint32_t tb, float &half) {
-    if (RF == kRow8) {
-        s = sum_b8_half(r.e[0], r.e[1], s);
-        half = s;
-        return sum_b8_half(r.e[2], r.e[3], s);
-    }
 #pragma unroll
     for (int i = 0; i < kBlkU4; ++i) {
-        s = sum_u4<RF == kRow16>(r.e[i], s, tb);
+        s = sum_u4<COMPACT>(r.e[i], s, tb);
         if (i == kBlkU4 / 2 - 1) half = s;
     }
     return s;
@@ -1020,13 +853,13 @@ struct SampleArgs {
     uint32_t mode;  // kSampleHead | kSampleTail: which words' items; | kShortcutOnly: no full draws
 };
 
-template <bool SHORTCUT, int RF, bool QTAB>
+template <bool SHORTCUT, bool COMPACT, bool QTAB>
 __device__ __forceinline__ void sample_item(const SampleArgs &A, uint32_t item, float *smf, float *f123,
                                             uint32_t sweep, uint32_t &n_full, uint32_t &n_scanned) {
     const int K = A.K, NS = (K + 63) >> 6;
     const float alpha = A.alpha, beta = A.beta;
     const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6, nwv = blockDim.x >> 6;
-    constexpr bool qtab = QTAB;  // K <= kByteK
+    constexpr bool qtab = QTAB;  // K <= EZ_Q_TABLE_MAX_K
     float *phi = smf;                         // NS*64
     float *qs = phi + NS * 64;                // NS*64 when qtab: running sums inside each slab
     float *qt = qs + (qtab ? NS * 64 : 0);    // NS: slab totals
@@ -1158,45 +991,36 @@ __device__ __forceinline__ void sample_item(const SampleArgs &A, uint32_t item, 
             }
         }
         if (!need || sc_only) continue;
-        // the row: 32-bit entries, 16-bit entries at a 16-bit offset, or
-        // 8-bit entries (two half-blocks per block) at a byte offset
-        const uint4 *rp = RF == kRow8    ? reinterpret_cast<const uint4 *>(reinterpret_cast<const uint8_t *>(A.dt) + di.x)
-                          : RF == kRow16 ? reinterpret_cast<const uint4 *>(reinterpret_cast<const uint16_t *>(A.dt) + di.x)
-                                         : reinterpret_cast<const uint4 *>(A.dt + di.x);
+        // the row: 32-bit entries, or 16-bit entries at a 16-bit offset
+        const uint4 *rp = COMPACT ? reinterpret_cast<const uint4 *>(reinterpret_cast<const uint16_t *>(A.dt) + di.x)
+                                  : reinterpret_cast<const uint4 *>(A.dt + di.x);
         const uint32_t nnz = di.y & 0xFFFFu;
-        constexpr uint32_t kEPB = RF == kRow8 ? 2u * kB8Half : (RF == kRow16 ? 8u : 4u) * kBlkU4;  // entries per block
+        constexpr uint32_t kEPB = (COMPACT ? 8u : 4u) * kBlkU4;  // entries per block
         const uint32_t nb = (nnz + kEPB - 1u) / kEPB;            // blocks of the row
         n_full += (uint32_t)__popc(need);
         n_scanned += nnz;
-        // checkpointed blocks: 8 in 8-bit rows (480 entries), else kCP
-        constexpr int NCP = RF == kRow8 ? 8 : kCP;
         // S: one FMA per entry, left to right; cp[b] = running sum at the
-        // end of block b (the first NCP blocks) and ch[b] at its middle, so
-        // the S-bucket search re-reads half a block instead of the row up to
-        // the target.  16- and 32-bit rows: block b + 1 is loaded before
-        // block b is summed (only by lanes whose row has it, so a wavefront
-        // stops loading at its longest row).  8-bit rows: a block (60
-        // entries) is loaded when it is summed; a second block's registers
-        // would make the kernel spill.
-        float cp[NCP], ch[NCP];  // running sum at the end / the middle of block b
+        // end of block b (the first kCP blocks), so the S-bucket search
+        // re-reads one block instead of the row up to the target
+        // Block b + 1 is loaded before block b is summed (only by lanes
+        // whose row has it, so a wavefront stops loading at its longest row).
+        float cp[kCP], ch[kCP];  // running sum at the end / the middle of block b
         float S = 0.0f;
         RowBlock cur = load_block(rp);
 #pragma unroll
-        for (int b = 0; b < NCP; ++b) {
+        for (int b = 0; b < kCP; ++b) {
             RowBlock nxt;
-            if ((uint32_t)b + 1u < nb && RF != kRow8) nxt = load_block(rp + kBlkU4 * (b + 1));
-            if (RF == kRow8 && (uint32_t)b < nb) cur = load_block(rp + kBlkU4 * b);  // (b = 0: the load above)
+            if ((uint32_t)b + 1u < nb) nxt = load_block(rp + kBlkU4 * (b + 1));
             ch[b] = S;
-            if ((uint32_t)b < nb) S = sum_block<RF>(cur, S, tb, ch[b]);
+            if ((uint32_t)b < nb) S = sum_block<COMPACT>(cur, S, tb, ch[b]);
             cp[b] = S;
-            cur = nxt;  // 8-bit rows reload it before use
+            cur = nxt;
         }
-        for (uint32_t b = NCP; b < nb; ++b) {
+        for (uint32_t b = kCP; b < nb; ++b) {
             RowBlock nxt;
-            if (b + 1u < nb && RF != kRow8) nxt = load_block(rp + kBlkU4 * (b + 1u));
-            if (RF == kRow8) cur = load_block(rp + kBlkU4 * b);
+            if (b + 1u < nb) nxt = load_block(rp + kBlkU4 * (b + 1u));
             float unused;
-            S = sum_block<RF>(cur, S, tb, unused);
+            S = sum_block<COMPACT>(cur, S, tb, unused);
             cur = nxt;
         }
         const float Aa = P1 + S;
@@ -1218,7 +1042,7 @@ __device__ __forceinline__ void sample_item(const SampleArgs &A, uint32_t item, 
                 const float t2 = target - P1;
                 int bs = -1;
 #pragma unroll
-                for (int b = NCP - 1; b >= 0; --b)
+                for (int b = kCP - 1; b >= 0; --b)
                     if (cp[b] > t2) bs = b;
                 float run = 0.0f;
                 bool found = false;
@@ -1228,53 +1052,30 @@ __device__ __forceinline__ void sample_item(const SampleArgs &A, uint32_t item, 
                     // its middle, at most half the block's loads
                     float mid = ch[0];
 #pragma unroll
-                    for (int b = 0; b < NCP - 1; ++b)
+                    for (int b = 0; b < kCP - 1; ++b)
                         if (b + 1 == bs) {
                             run = cp[b];
                             mid = ch[b + 1];
                         }
                     const uint4 *bp = rp + kBlkU4 * (uint32_t)bs;
-                    if (RF == kRow8) {  // a half-block carries its own base topic
+                    if (kHalfCp) {
                         if (mid > t2) {
-                            search_b8_half(bp, run, t2, newz, found);
+                            search_u4<COMPACT, kBlkU4 / 2>(bp, run, t2, tb, newz, found);
                         } else {
                             run = mid;
-                            search_b8_half(bp + kBlkU4 / 2, run, t2, newz, found);
-                        }
-                    } else if (kHalfCp) {
-                        if (mid > t2) {
-                            search_u4<RF == kRow16, kBlkU4 / 2>(bp, run, t2, tb, newz, found);
-                        } else {
-                            run = mid;
-                            search_u4<RF == kRow16, kBlkU4 / 2>(bp + kBlkU4 / 2, run, t2, tb, newz, found);
+                            search_u4<COMPACT, kBlkU4 / 2>(bp + kBlkU4 / 2, run, t2, tb, newz, found);
                         }
                     } else {
-                        search_u4<RF == kRow16, kBlkU4>(bp, run, t2, tb, newz, found);
+                        search_u4<COMPACT, kBlkU4>(bp, run, t2, tb, newz, found);
                     }
-                } else if (nb > (uint32_t)NCP) {
+                } else if (nb > (uint32_t)kCP) {
                     // past the checkpoints: the remaining blocks in order
-                    run = cp[NCP - 1];
-                    for (uint32_t gb = NCP; gb < nb && !found; ++gb) {
-                        if (RF == kRow8) {
-                            search_b8_half(rp + kBlkU4 * gb, run, t2, newz, found);
-                            if (!found) search_b8_half(rp + kBlkU4 * gb + 2, run, t2, newz, found);
-                        } else {
-                            search_u4<RF == kRow16, kBlkU4>(rp + kBlkU4 * gb, run, t2, tb, newz, found);
-                        }
-                    }
+                    run = cp[kCP - 1];
+                    for (uint32_t gb = kCP; gb < nb && !found; ++gb)
+                        search_u4<COMPACT, kBlkU4>(rp + kBlkU4 * gb, run, t2, tb, newz, found);
                 }
                 if (!found) {  // rounding left every running sum at or below t2
-                    if (RF == kRow8) {
-                        const uint8_t *rr = reinterpret_cast<const uint8_t *>(A.dt) + di.x;
-                        uint32_t tp = 0;
-                        for (uint32_t i = 0; i < nnz; ++i) {
-                            const uint32_t h = i / kB8Half, o = i - h * kB8Half;
-                            if (o == 0) tp = *reinterpret_cast<const uint16_t *>(rr + 32u * h);
-                            const uint32_t e = rr[32u * h + 2u + o];
-                            tp += e >> 3;
-                            if ((e & 7u) != 0u && phi_a(tp << 2) > 0.0f) newz = tp;
-                        }
-                    } else if (RF == kRow16) {
+                    if (COMPACT) {
                         const uint16_t *rr = reinterpret_cast<const uint16_t *>(A.dt) + di.x;
                         for (uint32_t i = 0; i < nnz; ++i)
                             if (c16_count(rr[i], tb) > 0.0f && phi_c16(rr[i], tb) > 0.0f) newz = c16_topic(rr[i], tb);
@@ -1351,7 +1152,7 @@ __device__ __forceinline__ void sample_item(const SampleArgs &A, uint32_t item, 
 // n_queues (the items of one XCD when items are dealt round-robin to XCDs),
 // a workgroup drains its own queue (blockIdx mod n_queues) and then the
 // others.  Every workgroup leaves once every queue is past its last item.
-template <bool SHORTCUT, int RF, bool QTAB>
+template <bool SHORTCUT, bool COMPACT, bool QTAB>
 __global__ void __launch_bounds__(1024) k_sample(const SampleArgs A) {
     // the only LDS of the kernel, so phi starts at address 0 and a DT entry's
     // low half is directly the LDS address of its phi value
@@ -1362,7 +1163,7 @@ __global__ void __launch_bounds__(1024) k_sample(const SampleArgs A) {
     const uint32_t sweep = A.counters[0];
     uint32_t n_full = 0, n_scanned = 0;
     if (A.n_queues == 0) {
-        sample_item<SHORTCUT, RF, QTAB>(A, blockIdx.x, smf, f123, sweep, n_full, n_scanned);
+        sample_item<SHORTCUT, COMPACT, QTAB>(A, blockIdx.x, smf, f123, sweep, n_full, n_scanned);
     } else {
         const uint32_t nq = A.n_queues, x0 = blockIdx.x % nq;
         for (uint32_t h = 0; h < nq; ++h) {
@@ -1373,7 +1174,7 @@ __global__ void __launch_bounds__(1024) k_sample(const SampleArgs A) {
                 __syncthreads();
                 const uint64_t item = (uint64_t)q + (uint64_t)s_item * nq;
                 if (item >= A.n_items) break;
-                sample_item<SHORTCUT, RF, QTAB>(A, (uint32_t)item, smf, f123, sweep, n_full, n_scanned);
+                sample_item<SHORTCUT, COMPACT, QTAB>(A, (uint32_t)item, smf, f123, sweep, n_full, n_scanned);
             }
         }
     }
@@ -1402,7 +1203,7 @@ __global__ void __launch_bounds__(1024) k_ll(const uint4 *__restrict__ segs, con
                                              const uint32_t *__restrict__ wt_tail,
                                              const uint32_t *__restrict__ tail_nnz, const int32_t *__restrict__ nk,
                                              const uint2 *__restrict__ doc_info, const uint32_t *__restrict__ dt,
-                                             int row_fmt, uint32_t tb, double *__restrict__ part) {
+                                             int compact, uint32_t tb, double *__restrict__ part) {
     extern __shared__ double smd[];
     __shared__ double red[16];
     const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6, nwv = blockDim.x >> 6;
@@ -1448,22 +1249,7 @@ __global__ void __launch_bounds__(1024) k_ll(const uint4 *__restrict__ segs, con
         const uint2 di = doc_info[rn.z];
         const uint32_t nnz = di.y & 0xFFFFu;
         double s = 0.0;
-        if (row_fmt == kRow8) {  // half-blocks: base topic, then delta << 3 | count bytes
-            const uint4 *rp = reinterpret_cast<const uint4 *>(reinterpret_cast<const uint8_t *>(dt) + di.x);
-            for (uint32_t hb = 0; hb < (nnz + kB8Half - 1u) / kB8Half; ++hb) {
-                const uint4 h0 = rp[2 * hb], h1 = rp[2 * hb + 1];
-                const uint32_t d[8] = {h0.x, h0.y, h0.z, h0.w, h1.x, h1.y, h1.z, h1.w};
-                uint32_t t = d[0] & 0xFFFFu;
-#pragma unroll
-                for (int i = 0; i < 8; ++i)
-#pragma unroll
-                    for (int j = i == 0 ? 2 : 0; j < 4; ++j) {
-                        const uint32_t e = (d[i] >> (8 * j)) & 0xFFu;
-                        t += e >> 3;
-                        s += (double)(e & 7u) * phi[t < (uint32_t)K ? t : 0u];
-                    }
-            }
-        } else if (row_fmt == kRow16) {
+        if (compact) {
             const uint4 *rp = reinterpret_cast<const uint4 *>(reinterpret_cast<const uint16_t *>(dt) + di.x);
             for (uint32_t i = 0; i < (nnz + 7u) / 8u; ++i) {
                 const uint4 v = rp[i];
@@ -2307,23 +2093,18 @@ hipError_t launch_build_dt(const DevState &s, hipStream_t st) {
     // where a K-sized histogram per wavefront leaves fewer than 16 on a CU,
     // the short documents (the last ones: docs come longest first) are sorted
     // instead
-    // (never with 8-bit rows: K <= 1024 tables always fit)
-    const bool sort_short = s.row_fmt != kRow8 && (size_t)s.lds_cu / dt_wave_bytes(s.K) < 16;
+    const bool sort_short = (size_t)s.lds_cu / dt_wave_bytes(s.K) < 16;
     const int q1 = sort_short ? s.D_long : s.D;
     if (q1 > 0) {
         const WaveGrid g = wave_grid(s, (size_t)q1, dt_wave_bytes(s.K), 1 << 18);
-        if (s.row_fmt == kRow8)
-            hipLaunchKernelGGL(k_dt_build<kRow8>, dim3(g.grid), dim3(64 * g.nwv), g.nwv * dt_wave_bytes(s.K), st, 0,
-                               q1, s.K, s.VD, s.topic_bits, s.soff, s.slot_wrd, s.slot_tok, s.z, s.z_doc, s.wtop,
-                               s.aux_doc, s.doc_info, s.dt);
-        else if (s.row_fmt == kRow16)
-            hipLaunchKernelGGL(k_dt_build<kRow16>, dim3(g.grid), dim3(64 * g.nwv), g.nwv * dt_wave_bytes(s.K), st, 0,
-                               q1, s.K, s.VD, s.topic_bits, s.soff, s.slot_wrd, s.slot_tok, s.z, s.z_doc, s.wtop,
-                               s.aux_doc, s.doc_info, s.dt);
+        if (s.compact)
+            hipLaunchKernelGGL(k_dt_build<true>, dim3(g.grid), dim3(64 * g.nwv), g.nwv * dt_wave_bytes(s.K), st, 0, q1,
+                               s.K, s.VD, s.topic_bits, s.soff, s.slot_wrd, s.slot_tok, s.z, s.z_doc, s.wtop, s.aux_doc,
+                               s.doc_info, s.dt);
         else
-            hipLaunchKernelGGL(k_dt_build<kRow32>, dim3(g.grid), dim3(64 * g.nwv), g.nwv * dt_wave_bytes(s.K), st, 0,
-                               q1, s.K, s.VD, s.topic_bits, s.soff, s.slot_wrd, s.slot_tok, s.z, s.z_doc, s.wtop,
-                               s.aux_doc, s.doc_info, s.dt);
+            hipLaunchKernelGGL(k_dt_build<false>, dim3(g.grid), dim3(64 * g.nwv), g.nwv * dt_wave_bytes(s.K), st, 0, q1,
+                               s.K, s.VD, s.topic_bits, s.soff, s.slot_wrd, s.slot_tok, s.z, s.z_doc, s.wtop, s.aux_doc,
+                               s.doc_info, s.dt);
     }
     if (q1 < s.D) {
         const size_t lds = 4 * kDtShort * sizeof(uint32_t);  // four wavefronts, 2 KB each
@@ -2341,17 +2122,17 @@ hipError_t launch_build_dt(const DevState &s, hipStream_t st) {
 size_t sample_lds_bytes(int K, int bd) {
     const int NS = (K + 63) >> 6;
     (void)bd;
-    const size_t tab = K <= (int)kByteK ? 2 : 1;  // phi (and the slab running sums)
+    const size_t tab = K <= EZ_Q_TABLE_MAX_K ? 2 : 1;  // phi (and the slab running sums)
     return (tab * (size_t)NS * 64 + 2 * (size_t)NS + 4) * sizeof(float);  // + qt, qacc, f123[3], item
 }
 
 size_t sampler_static_lds() {
-    const void *fs[6] = {reinterpret_cast<const void *>(&k_sample<true, kRow8, true>),
-                         reinterpret_cast<const void *>(&k_sample<true, kRow16, false>),
-                         reinterpret_cast<const void *>(&k_sample<true, kRow32, false>),
-                         reinterpret_cast<const void *>(&k_sample<false, kRow8, true>),
-                         reinterpret_cast<const void *>(&k_sample<false, kRow16, false>),
-                         reinterpret_cast<const void *>(&k_sample<false, kRow32, false>)};
+    const void *fs[6] = {reinterpret_cast<const void *>(&k_sample<true, true, true>),
+                         reinterpret_cast<const void *>(&k_sample<true, true, false>),
+                         reinterpret_cast<const void *>(&k_sample<true, false, false>),
+                         reinterpret_cast<const void *>(&k_sample<false, true, true>),
+                         reinterpret_cast<const void *>(&k_sample<false, true, false>),
+                         reinterpret_cast<const void *>(&k_sample<false, false, false>)};
     size_t m = 0;
     for (const void *f : fs) {
         hipFuncAttributes a;
@@ -2409,19 +2190,19 @@ hipError_t launch_sample(const DevState &s, bool shortcut, hipStream_t st, uint3
                            (size_t)kQueueStride * A.n_queues, 0u);
         grid = (unsigned)s.grid_dim;
     }
-    // 8-bit rows (K <= kByteK) go with the Q table
-    if (shortcut && s.row_fmt == kRow8)
-        hipLaunchKernelGGL((k_sample<true, kRow8, true>), dim3(grid), dim3(bd), lds, st, A);
-    else if (shortcut && s.row_fmt == kRow16)
-        hipLaunchKernelGGL((k_sample<true, kRow16, false>), dim3(grid), dim3(bd), lds, st, A);
+    const bool qtab = s.K <= EZ_Q_TABLE_MAX_K;  // implies compact rows
+    if (shortcut && qtab)
+        hipLaunchKernelGGL((k_sample<true, true, true>), dim3(grid), dim3(bd), lds, st, A);
+    else if (shortcut && s.compact)
+        hipLaunchKernelGGL((k_sample<true, true, false>), dim3(grid), dim3(bd), lds, st, A);
     else if (shortcut)
-        hipLaunchKernelGGL((k_sample<true, kRow32, false>), dim3(grid), dim3(bd), lds, st, A);
-    else if (s.row_fmt == kRow8)
-        hipLaunchKernelGGL((k_sample<false, kRow8, true>), dim3(grid), dim3(bd), lds, st, A);
-    else if (s.row_fmt == kRow16)
-        hipLaunchKernelGGL((k_sample<false, kRow16, false>), dim3(grid), dim3(bd), lds, st, A);
+        hipLaunchKernelGGL((k_sample<true, false, false>), dim3(grid), dim3(bd), lds, st, A);
+    else if (qtab)
+        hipLaunchKernelGGL((k_sample<false, true, true>), dim3(grid), dim3(bd), lds, st, A);
+    else if (s.compact)
+        hipLaunchKernelGGL((k_sample<false, true, false>), dim3(grid), dim3(bd), lds, st, A);
     else
-        hipLaunchKernelGGL((k_sample<false, kRow32, false>), dim3(grid), dim3(bd), lds, st, A);
+        hipLaunchKernelGGL((k_sample<false, false, false>), dim3(grid), dim3(bd), lds, st, A);
     return hipGetLastError();
 }
 
@@ -2439,7 +2220,7 @@ hipError_t launch_loglik(const DevState &s, hipStream_t st) {
     hipLaunchKernelGGL(k_ll, dim3(s.n_smp_segs), dim3(bd), (size_t)s.K * sizeof(double), st, s.smp_segs, s.runs, s.VD,
                        s.K,
                        (double)s.alpha, (double)s.beta, (double)s.V * (double)s.beta, s.wt_head, s.tail_off,
-                       s.wt_tail, s.tail_nnz, s.nk, s.doc_info, s.dt, (int)s.row_fmt, s.topic_bits, s.ll_part);
+                       s.wt_tail, s.tail_nnz, s.nk, s.doc_info, s.dt, (int)s.compact, s.topic_bits, s.ll_part);
     hipLaunchKernelGGL(k_ll_final, dim3(1), dim3(256), 0, st, s.ll_part, s.n_smp_segs, s.ll_out);
     return hipGetLastError();
 }

@@ -47,12 +47,10 @@ void parallel_for(int64_t n, F f) {
     for (auto &x : th) x.join();
 }
 
-// DT rows: 8-bit entries for K <= kByteK (counts in pieces of at most 7),
-// 16-bit entries for K <= kCompactK (the oracle's ORC_COMPACT_K), tb =
-// max(10, ceil(log2 K)) topic bits, counts in pieces of at most
-// 2^(16 - tb) - 1; 32-bit entries above.
-using ez::kByteK;
-using ez::kCompactK;
+// DT rows: 16-bit entries for K <= kCompactK (the oracle's ORC_COMPACT_K),
+// tb = max(10, ceil(log2 K)) topic bits, counts in pieces of at most
+// 2^(16 - tb) - 1.
+constexpr uint32_t kCompactK = 8192;
 
 uint32_t topic_bits(int K) {
     uint32_t b = 10;
@@ -60,7 +58,7 @@ uint32_t topic_bits(int K) {
     return b;
 }
 
-uint32_t dt_piece(int K) { return (uint32_t)K <= kByteK ? 7u : (1u << (16u - topic_bits(K))) - 1u; }
+uint32_t dt_piece(int K) { return (1u << (16u - topic_bits(K))) - 1u; }
 
 // Entries of a doc's DT row at most: min(n, K), plus one per full piece in
 // compact rows.
@@ -68,28 +66,6 @@ uint32_t dt_cap(uint32_t n, int K) {
     const uint32_t m = std::min<uint32_t>(n, (uint32_t)K);
     if ((uint32_t)K > kCompactK) return m;
     return std::min<uint32_t>(m + n / dt_piece(K), n);
-}
-
-uint32_t row_format(int K) { return (uint32_t)K <= kByteK ? 2u : (uint32_t)K <= kCompactK ? 1u : 0u; }
-
-// Device capacity of a doc's DT row in the row's units (bytes, 16-bit or
-// 32-bit entries), a multiple of 64 bytes: 8-bit rows add the count-0 steps
-// over gaps above 31 (at most (K - 1) / 31) and a 2-byte base topic per
-// 30 entries.
-uint32_t row_units(uint32_t n, int K) {
-    const uint32_t f = row_format(K);
-    if (f == 2) {
-        const uint32_t ent = dt_cap(n, K) + ((uint32_t)K - 1u) / 31u;
-        return (32u * ((ent + 29u) / 30u) + 63u) & ~63u;
-    }
-    return f == 1 ? (dt_cap(n, K) + 63u) & ~63u : (dt_cap(n, K) + 31u) & ~31u;
-}
-
-// 32-bit words holding `units` row units and 1 KB of zero padding past the
-// last row.
-size_t row_words(uint64_t units, int K) {
-    const uint32_t f = row_format(K);
-    return f == 2 ? (size_t)((units + 1024 + 3) / 4) : f == 1 ? (size_t)((units + 512 + 1) / 2) : (size_t)(units + 256);
 }
 
 // Device address of the static log-likelihood array on each device, looked
@@ -520,6 +496,7 @@ int load_chunked(ezlda_ctx *c, const std::vector<uint32_t> &tok_doc, const std::
     std::vector<uint32_t> loc(N), cnt(C, 0);
     for (uint32_t t = 0; t < N; ++t) loc[t] = cnt[chunk_of_q[tok_doc[t]]]++;
     const int VDi = head_words(c, V, K);  // item tiling as in the resident layout
+    const bool compact = s.compact;
     c->chunks.assign(C, EzChunk{});
     c->chunk_slot0.assign(C, 0);
     size_t max_bytes = 0;
@@ -556,7 +533,7 @@ int load_chunked(ezlda_ctx *c, const std::vector<uint32_t> &tok_doc, const std::
         for (uint32_t q = 0; q < ch.D; ++q) {
             const uint32_t nq = soffc[q + 1] - soffc[q];
             di[q] = make_uint2((uint32_t)dtc, nq << 16);
-            dtc += row_units(nq, K);
+            dtc += compact ? (dt_cap(nq, K) + 63u) & ~63u : (dt_cap(nq, K) + 31u) & ~31u;
         }
         if (dtc >= 0xFFFFFFF0ull) return fail(c, EZLDA_EINVAL, "DT capacity of a chunk exceeds 2^32 entries");
         ch.dt_entries = dtc;
@@ -621,7 +598,7 @@ int load_chunked(ezlda_ctx *c, const std::vector<uint32_t> &tok_doc, const std::
     c->tok_slot_g = tok_slot;
     // device: two image buffers, the model and the next sweep's counts
     int r = 0;
-    const size_t dt_words = row_words(max_dt, K);
+    const size_t dt_words = compact ? (max_dt + 256 + 1) / 2 : max_dt + 256;
     for (int j = 0; j < 2; ++j) {
         if ((r = dalloc(c, &c->cimg[j], max_bytes))) return r;
         if ((r = dalloc(c, &c->cdt[j], dt_words))) return r;
@@ -1007,9 +984,8 @@ int ezlda_load_docs(ezlda_ctx *c, int32_t V, int32_t D, const int64_t *doc_off, 
     });
     if (c->a.num_chunks > 1 && D > 1) {  // out of core: chunk images in pinned host memory
         ez::DevState &s = c->s;
-        s.row_fmt = row_format(K);
-        s.compact = s.row_fmt == 1u;
-        s.topic_bits = s.row_fmt != 0u ? topic_bits(K) : 16u;
+        s.compact = K <= (int)kCompactK;
+        s.topic_bits = s.compact ? topic_bits(K) : 16u;
         s.K = K; s.V = V; s.D = D; s.N = N;
         s.alpha = c->a.alpha; s.beta = c->a.beta;
         s.vbeta = (float)((double)V * (double)c->a.beta);
@@ -1026,18 +1002,18 @@ int ezlda_load_docs(ezlda_ctx *c, int32_t V, int32_t D, const int64_t *doc_off, 
     // the rest): K <= 8192 keeps 16-bit entries with counts in pieces
     // (capacity dt_cap(n, K) rounded up to 64 entries), larger K 32-bit
     // entries (min(n_d, K) rounded up to 32)
+    const bool compact = K <= (int)kCompactK;
     std::vector<uint2> doc_info(D);
     uint64_t dtc = 0;
     for (int q = 0; q < D; ++q) {
         const uint32_t n = c->soff[q + 1] - c->soff[q];
         doc_info[q] = make_uint2((uint32_t)dtc, n << 16);
-        dtc += row_units(n, K);
+        dtc += compact ? (dt_cap(n, K) + 63u) & ~63u : (dt_cap(n, K) + 31u) & ~31u;
     }
     if (dtc >= 0xFFFFFFF0ull) return fail(c, EZLDA_EINVAL, "DT capacity exceeds 2^32 entries");
     ez::DevState &s = c->s;
-    s.row_fmt = row_format(K);
-    s.compact = s.row_fmt == 1u;
-    s.topic_bits = s.row_fmt != 0u ? topic_bits(K) : 16u;
+    s.compact = compact;
+    s.topic_bits = compact ? topic_bits(K) : 16u;
     s.K = K; s.V = V; s.D = D; s.N = N;
     s.alpha = c->a.alpha; s.beta = c->a.beta;
     s.vbeta = (float)((double)V * (double)c->a.beta);
@@ -1056,8 +1032,8 @@ int ezlda_load_docs(ezlda_ctx *c, int32_t V, int32_t D, const int64_t *doc_off, 
     if ((r = upload(c, &s.z_doc, z_doc))) return r;
     if ((r = dalloc(c, &s.aux_doc, N))) return r;
     if ((r = upload(c, &s.doc_info, doc_info))) return r;
-    // zero padding past the last row
-    const size_t dt_words = row_words(dtc, K);
+    // 256 entries of zero padding past the last row
+    const size_t dt_words = compact ? (dtc + 256 + 1) / 2 : dtc + 256;
     if ((r = dalloc(c, &s.dt, dt_words))) return r;
     EZ_TRY_HIP(c, hipMemset(s.dt, 0, dt_words * sizeof(uint32_t)));
     if ((r = dalloc(c, &s.nk, K))) return r;
@@ -1754,7 +1730,7 @@ int ezlda_get_dt_raw(ezlda_ctx *c, uint32_t *entries, int32_t *nnz) try {
     if (is_chunked(c)) return refuse_chunked(c, "ezlda_get_dt_raw");
     const ez::DevState &s = c->s;
     std::vector<uint2> di(s.D);
-    std::vector<uint32_t> dt(s.row_fmt == 2u ? (c->dt_cap + 3) / 4 : s.compact ? (c->dt_cap + 1) / 2 : c->dt_cap);
+    std::vector<uint32_t> dt(s.compact ? (c->dt_cap + 1) / 2 : c->dt_cap);
     if (s.D) EZ_TRY_HIP(c, hipMemcpy(di.data(), s.doc_info, di.size() * sizeof(uint2), hipMemcpyDeviceToHost));
     if (!dt.empty()) EZ_TRY_HIP(c, hipMemcpy(dt.data(), s.dt, dt.size() * 4, hipMemcpyDeviceToHost));
     const uint16_t *dt16 = reinterpret_cast<const uint16_t *>(dt.data());
@@ -1764,21 +1740,6 @@ int ezlda_get_dt_raw(ezlda_ctx *c, uint32_t *entries, int32_t *nnz) try {
     for (int d = 0; d < s.D; ++d) {
         const uint2 x = di[qof[d]];
         nnz[d] = (int32_t)(x.y & 0xFFFFu);
-        if (s.row_fmt == 2u) {  // 8-bit rows: half-blocks of a base topic and 30 delta << 3 | count bytes
-            const uint8_t *row = reinterpret_cast<const uint8_t *>(dt.data()) + x.x;
-            const uint32_t ent = x.y & 0xFFFFu;
-            uint32_t t = 0, k = 0;
-            for (uint32_t i = 0; i < ent; ++i) {
-                const uint32_t hb = i / 30u, o = i - 30u * hb;
-                if (o == 0) t = (uint32_t)row[32u * hb] | ((uint32_t)row[32u * hb + 1u] << 8);
-                const uint32_t e = row[32u * hb + 2u + o];
-                t += e >> 3;
-                if (e & 7u) entries[canon + k++] = t | ((e & 7u) << 16);  // count-0 steps are not entries
-            }
-            nnz[d] = (int32_t)k;
-            canon += dt_cap(c->doc_off[d + 1] - c->doc_off[d], s.K);
-            continue;
-        }
         for (int32_t i = 0; i < nnz[d]; ++i) {  // device rows hold topic << 2, or 16-bit entries
             if (s.compact) {
                 const uint32_t e = dt16[x.x + i];

@@ -581,15 +581,9 @@ def selftest_gather(nbytes: int, n_reads: int, reps: int = 10) -> float:
 COMPACT_K = 8192
 
 
-BYTE_K = 1024
-
-
 def dt_piece(K: int) -> int:
-    """Largest count of one compact DT entry: 7 for K <= 1024 (8-bit rows),
-    else 2^(16 - tb) - 1 with tb = ceil(log2 K) topic bits (31 for K <= 2048,
-    15 for K <= 4096, 7 for K <= 8192)."""
-    if K <= BYTE_K:
-        return 7
+    """Largest count of one compact DT entry: 2^(16 - tb) - 1 with tb =
+    max(10, ceil(log2 K)) topic bits (63 for K <= 1024, 31 for K <= 2048...)."""
     tb = 10
     while (1 << tb) < K:
         tb += 1

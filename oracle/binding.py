@@ -167,10 +167,9 @@ class Oracle:
     def get_dt_raw(self):
         n = np.diff(self.doc_off).astype(np.int64)
         m = np.minimum(n, self.K)
-        # dt_cap() of the oracle: compact rows (K <= 8192) add one entry per full
-        # piece (7 up to K = 1024, else 2^(16 - tb) - 1)
+        # dt_cap() of the oracle: compact rows (K <= 8192) add one entry per full piece
         tb = max(10, int(np.ceil(np.log2(self.K)))) if self.K > 1 else 10
-        piece = 7 if self.K <= 1024 else (1 << (16 - tb)) - 1
+        piece = (1 << (16 - tb)) - 1
         cap = int((m if self.K > 8192 else np.minimum(m + n // piece, n)).sum())
         ent = np.zeros(max(cap, 1), np.uint32)
         nnz = np.zeros(max(self.D, 1), np.int32)

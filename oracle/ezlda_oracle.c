@@ -64,26 +64,19 @@
 
 #define ORC_MARGIN 0.998046875f /* 1 - 2^-9 */
 #define ORC_NONE 0xFFFFu
-/* Compact DT rows (K <= ORC_COMPACT_K): a topic's count is stored in pieces
- * of at most `piece`, one entry per piece, in order, and every S sum and
- * running sum walks the pieces.  The device's 8-bit rows (K <= ORC_BYTE_K:
- * delta << 3 | count bytes) hold pieces of 7; its 16-bit rows (tb =
- * max(10, ceil(log2 K)) topic bits, 16 - tb count bits) pieces of
- * 2^(16 - tb) - 1.  Larger K keeps one entry per topic.  (The 8-bit rows'
- * count-0 steps over wide topic gaps add exactly zero to an FMA chain and
- * have no entry here.) */
+/* Compact DT rows (K <= ORC_COMPACT_K): 16-bit entries of tb topic bits,
+ * tb = max(10, ceil(log2 K)), and 16 - tb count bits; a topic's count is
+ * stored in pieces of at most piece = 2^(16 - tb) - 1, one entry per piece,
+ * in order, and every S sum and running sum walks the pieces.  Larger K
+ * keeps one entry per topic. */
 #define ORC_COMPACT_K 8192
-#define ORC_BYTE_K 1024
 
 static int topic_bits(int K) {
     int b = 10;
     while ((1 << b) < K) ++b;
     return b;
 }
-static uint32_t dt_piece(int K) {
-    if (K <= ORC_BYTE_K) return 7u;
-    return K <= ORC_COMPACT_K ? (1u << (16 - topic_bits(K))) - 1u : 0xFFFFFFFFu;
-}
+static uint32_t dt_piece(int K) { return K <= ORC_COMPACT_K ? (1u << (16 - topic_bits(K))) - 1u : 0xFFFFFFFFu; }
 
 /* Entries of a doc's DT row at most: min(n, K), plus one per full piece in
  * compact rows. */

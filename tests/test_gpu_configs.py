@@ -18,7 +18,7 @@ n_k against the oracle, the FP64 log-likelihood within 1e-6 relative:
   vocabulary, 64 log-normal requests in [10, 5000] through 2,048 slots,
   alone, in one mixed batch and on the plain path;
 * long documents whose rows pass the sampler's checkpointed blocks (more
-  than 480 8-bit entries at K=1024; more than 512 slots with 32-bit rows).
+  than 384 compact entries at K=1024; more than 512 slots with 32-bit rows).
 
 The oracle here is the OpenMP build (bit-identical to the single-threaded
 one; tests/test_oracle.py pins that)."""
@@ -96,7 +96,7 @@ def test_cfg2_full_size_default_launch_shape():
         if s == 0:
             ctx.build_dt()
             _, nnz = ctx.get_dt_raw()
-            assert nnz.max() > 480  # rows past the sampler's checkpointed blocks (8 of 60 byte entries)
+            assert nnz.max() > 384  # rows past the sampler's checkpointed blocks
     assert ctx.loglik() == pytest.approx(orc.loglik(), rel=LL_REL)
 
 
@@ -199,9 +199,9 @@ def test_long_documents_pass_checkpointed_blocks(K, lo, hi):
         ctx.build_dt()
         _, nnz = ctx.get_dt_raw()
         if s == 0:
-            # 8-bit rows (K <= 1024): 8 checkpointed blocks of 60 entries;
-            # 32-bit rows: documents of more than 512 slots in the DT rebuild
-            assert nnz.max() > (480 if K <= 1024 else 384 if K <= 8192 else 512), nnz.max()
+            # compact rows: 12 checkpointed blocks of 32 entries; 32-bit rows:
+            # documents of more than 512 slots in the DT rebuild
+            assert nnz.max() > (384 if K <= 8192 else 512), nnz.max()
         ctx.sample()
         orc.sweep(1)
         assert np.array_equal(ctx.get_topics(), orc.get_topics()), (K, s)

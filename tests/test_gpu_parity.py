@@ -274,9 +274,8 @@ def test_export_model(tmp_path):
 
 @pytest.mark.parametrize("K", [1024, 2048, 4096, 8192])
 def test_compact_rows_with_pieces(K):
-    """Counts far above an entry's piece (7 in 8-bit rows at K=1024; 31, 15, 7
-    in 16-bit rows): rows, topics and log-likelihood bit-exact with the
-    oracle through several sweeps."""
+    """Counts far above a compact entry's piece (63/31/15/7): rows, topics
+    and log-likelihood bit-exact with the oracle through several sweeps."""
     off, words = make_corpus(9, 40, 300, 500, zipf=1.2)
     z0 = (np.arange(len(words)) % 3 == 0).astype(np.uint16) * 5  # topics 0 and 5 only
     ctx, orc = _pair(off, words, 300, K, z0=z0)
@@ -368,41 +367,3 @@ def test_dt_rows_and_aux_of_sorted_short_docs(K):
         ctx.sample()
         orc.sample()
         assert np.array_equal(ctx.get_topics(), orc.get_topics()), (K, s)
-
-
-def test_byte_rows_edge_layouts():
-    """K=1024 (8-bit rows: 32-byte half-blocks of a base topic and 30 delta
-    bytes): documents whose first-sweep rows hit the format's edges -- a lone
-    topic 0 or 1023 (33 count-0 steps), gaps of exactly 31 and 32, rows of
-    exactly 30/31/60/61 entries, counts at the piece boundaries 7/8/14/15 and
-    a row of all 1024 topics (past the checkpointed blocks) -- decode to the
-    oracle's rows, and the sweeps over them give the oracle's topics."""
-    K, V = 1024, 50
-    docs = [[0], [1023], [0, 1023], [31, 62, 94, 126, 127, 1023],
-            list(range(0, 30 * 7, 7)), list(range(0, 31 * 7, 7)), list(range(60)), list(range(61)),
-            [3] * 7 + [9] * 8 + [40] * 14 + [41] * 15 + [900] * 22,
-            list(range(1024)), [5, 5, 5, 1000, 1000, 6]]
-    rng = np.random.default_rng(3)
-    z0 = np.concatenate([np.array(d, np.uint16) for d in docs])
-    off = np.concatenate([[0], np.cumsum([len(d) for d in docs])]).astype(np.int64)
-    words = rng.integers(0, V, int(off[-1])).astype(np.int32)
-    ctx, orc = _pair(off, words, V, K, z0=z0)
-    for s in range(3):
-        ctx.build_wt()
-        orc.build_wt()
-        ctx.topk()
-        orc.topk()
-        ctx.build_dt()
-        orc.build_dt()
-        ge, gn = ctx.get_dt_raw()
-        oe, on = orc.get_dt_raw()
-        assert np.array_equal(gn, on), s
-        doff = ezlda.dt_offsets(off, K)
-        for d in range(len(gn)):
-            assert np.array_equal(ge[doff[d]: doff[d] + gn[d]], oe[doff[d]: doff[d] + on[d]]), (s, d)
-        if s == 0:
-            assert gn.max() == 1024 and gn[0] == 1 and gn[1] == 1
-        ctx.sample()
-        orc.sample()
-        assert np.array_equal(ctx.get_topics(), orc.get_topics()), s
-    assert ctx.loglik() == pytest.approx(orc.loglik(), rel=1e-9)

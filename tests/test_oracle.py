@@ -118,7 +118,7 @@ def test_counts_bruteforce():
 
 def _check_rows(o, off, K):
     """Rows hold each topic once in ascending order (compact rows: in pieces
-    of dt_piece(K) and a remainder), counts summing to n_d, within the capacity."""
+    of 63 and a remainder), counts summing to n_d, within the capacity."""
     ent, nnz = o.get_dt_raw()
     doff = ezlda.dt_offsets(off, K)
     dense = o.get_dt_dense()
@@ -141,7 +141,7 @@ def _check_rows(o, off, K):
 
 @pytest.mark.parametrize("K", [2, 5, 1025, 2048, 4097, 8193])
 def test_dt_rows_split_large_counts(K):
-    # long docs over few topics: counts far above any piece size
+    # long docs over few topics: counts far above 63 (split when K <= 1024)
     off, words = make_corpus(7, 12, 30, 600)
     o = ob.Oracle(K, float(np.float32(ALPHA(K))), BETA, 3, 30, off, words,
                   z0=(np.arange(len(words)) % 2).astype(np.uint16))
