@@ -690,8 +690,10 @@ __global__ void __launch_bounds__(256) k_dt_sort(int q0, int q1, int VD, uint32_
 
 // ------------------------------------------------------------- sampler
 // Checkpoints of the S running sum kept per lane (one per block of kBlkU4
-// 16-byte loads, i.e. 4*kBlkU4 DT entries, and with kHalfCp one more in the
-// middle of each block, so the S-bucket search re-reads half a block).
+// 16-byte loads, i.e. 4*kBlkU4 DT entries, and with kHalfCp, in 16-bit rows,
+// one more in the middle of each block, so the S-bucket search re-reads half a
+// block: 6.09 -> 5.98 ms a sweep on the 99M-token corpus at K = 1024; with
+// 32-bit rows it cost 1.5 ms of 63.6 at K = 10000, so they search whole blocks).
 static constexpr int kCP = EZ_SAMPLE_CP;
 static constexpr int kBlkU4 = EZ_SAMPLE_BLK;
 #ifndef EZ_SAMPLE_HALF_CP
@@ -700,7 +702,10 @@ static constexpr int kBlkU4 = EZ_SAMPLE_BLK;
 static constexpr bool kHalfCp = EZ_SAMPLE_HALF_CP != 0 && kBlkU4 % 2 == 0;
 // Uniforms kept in registers per run between the shortcut test and the draw
 // (tokens past them draw theirs again).
-static constexpr int kUKeep = 4;
+// (32-bit rows keep one: at K > 8192 the registers of four cost more than the
+// draws they save, 63.9 against 63.5 ms a sweep on 180M tokens at K = 10000)
+static constexpr int kUKeep = 4;    // 16-bit rows
+static constexpr int kUKeep32 = 1;  // 32-bit rows
 
 // Running FMA sum over one block of a DT row: kBlkU4 16-byte loads issued
 // together, then the entries in order.
@@ -960,11 +965,12 @@ __device__ __forceinline__ void sample_item(const SampleArgs &A, uint32_t item, 
         // tokens of the run that need the full draw; shortcut tokens take k1.
         // The uniforms of the first kUKeep tokens are kept for the draw.
         uint32_t need = 0;
-        float uk[kUKeep];
+        constexpr int UK = COMPACT ? kUKeep : kUKeep32;
+        float uk[UK];
         const float maxS = (float)n2 * f2 + (float)(nd - n1 - n2) * f3;
         const float B = (P1 + maxS) + Q;
 #pragma unroll
-        for (uint32_t j = 0; j < (uint32_t)kUKeep; ++j) {
+        for (uint32_t j = 0; j < (uint32_t)UK; ++j) {
             uk[j] = 0.0f;
             if (j < len) {
                 const float u = ez_u24(ez_philox_x0(A.seed, t0 + j + sg.w, sweep, EZ_TAG_SAMPLE));
@@ -977,7 +983,7 @@ __device__ __forceinline__ void sample_item(const SampleArgs &A, uint32_t item, 
                 }
             }
         }
-        for (uint32_t j = kUKeep; j < len; ++j) {
+        for (uint32_t j = UK; j < len; ++j) {
             bool full = true;
             if (SHORTCUT) {
                 const float u = ez_u24(ez_philox_x0(A.seed, t0 + j + sg.w, sweep, EZ_TAG_SAMPLE));
@@ -1031,9 +1037,9 @@ __device__ __forceinline__ void sample_item(const SampleArgs &A, uint32_t item, 
             const uint32_t t = t0 + j;
             float u = uk[0];
 #pragma unroll
-            for (int i = 1; i < kUKeep; ++i)
+            for (int i = 1; i < UK; ++i)
                 if (j == (uint32_t)i) u = uk[i];
-            if (j >= (uint32_t)kUKeep) u = ez_u24(ez_philox_x0(A.seed, t + sg.w, sweep, EZ_TAG_SAMPLE));
+            if (j >= (uint32_t)UK) u = ez_u24(ez_philox_x0(A.seed, t + sg.w, sweep, EZ_TAG_SAMPLE));
             const float target = u * total;
             uint32_t newz = k1;
             if (target < P1) {
@@ -1058,7 +1064,7 @@ __device__ __forceinline__ void sample_item(const SampleArgs &A, uint32_t item, 
                             mid = ch[b + 1];
                         }
                     const uint4 *bp = rp + kBlkU4 * (uint32_t)bs;
-                    if (kHalfCp) {
+                    if (kHalfCp && COMPACT) {
                         if (mid > t2) {
                             search_u4<COMPACT, kBlkU4 / 2>(bp, run, t2, tb, newz, found);
                         } else {
