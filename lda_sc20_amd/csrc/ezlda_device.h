@@ -32,7 +32,7 @@ struct DevState {
     uint32_t *slot_wrd = nullptr;   // N: word rank
     uint32_t *slot_tok = nullptr;   // N: token id of the slot
     uint16_t *z_doc = nullptr;      // N: z by slot, kept by the sampler for head-word slots
-    uint32_t *aux_doc = nullptr;    // N: n_dk1 | n_dk2 << 16 of the slot's word
+    uint2 *aux_doc = nullptr;       // N: x = n_dk1 | n_dk2 << 16 of the slot's word, y = its doc's nnz | len << 16
     // doc-topic rows
     uint2 *doc_info = nullptr;      // D, by doc q: x = DT row offset, y = nnz | len << 16
     int32_t D_long = 0;             // docs of more than kDtShort tokens (the first D_long in doc order)
@@ -53,7 +53,7 @@ struct DevState {
     int32_t n_wt_segs = 0;
     uint4 *smp_segs = nullptr;      // sampler items (w, first run, end run, draw-key offset)
     int32_t n_smp_segs = 0;
-    uint4 *runs = nullptr;          // (first token, first slot, doc q, tokens): a word's tokens in one doc
+    uint4 *runs = nullptr;          // (first token, first slot, doc's DT row offset, tokens): a word's tokens in one doc
     int64_t n_runs = 0;
     int32_t n_queues = 1;           // persistent sampler: item queues (<= 8)
     uint32_t *counters = nullptr;   // kCounterWords: [0] sweep, queue q's head at [kQueueStride * (q + 1)]

@@ -46,6 +46,7 @@ struct ezlda_ctx {
     std::vector<int32_t> dord;       // doc order q -> input doc (length desc, id asc)
     std::vector<uint32_t> soff;      // D+1: slots of doc q
     std::vector<uint32_t> slot_tok;  // N: slot -> token id
+    std::vector<uint32_t> row_off;   // D: DT row offset of doc q (the runs carry it)
     hipGraph_t graph = nullptr;
     hipGraphExec_t graph_exec = nullptr;
     int n_xcd = 1;      // XCDs of the device (round-robin workgroup dispatch)
@@ -56,7 +57,7 @@ struct ezlda_ctx {
     std::vector<uint32_t> tok_slot_g;  // token id -> corpus slot
     uint8_t *cimg[2] = {nullptr, nullptr};  // device image buffers
     uint32_t *cdt[2] = {nullptr, nullptr};  // DT rows of the chunk in flight
-    uint32_t *caux[2] = {nullptr, nullptr};
+    uint2 *caux[2] = {nullptr, nullptr};
     uint32_t *cctr[2] = {nullptr, nullptr};  // per-buffer sweep counter and item queues
     double *cll[2] = {nullptr, nullptr};     // per-item log-likelihood partials
     double *cllo[2] = {nullptr, nullptr};

@@ -105,6 +105,7 @@ __device__ __forceinline__ float phi_c16(uint32_t e, uint32_t tb) { return *(lds
 __device__ __forceinline__ void h16_add(uint32_t *h, uint32_t k) {
     atomicAdd(&h[k >> 1], 1u << ((k & 1u) << 4));
 }
+
 __device__ __forceinline__ uint32_t h16_get(const uint32_t *h, uint32_t k) {
     return (h[k >> 1] >> ((k & 1u) << 4)) & 0xFFFFu;
 }
@@ -397,10 +398,11 @@ hipError_t launch_z_to_doc(const DevState &s, hipStream_t st) {
 // (docs in descending length order) owns slots [soff[q], soff[q+1]), its
 // tokens grouped by word, and the sampler keeps z_doc in step with z, so
 // every read and write here is a coalesced stream.  One wavefront per doc
-// builds a 16-bit LDS histogram, writes each slot's (n_dk1, n_dk2) at its
-// word's top-2 topics (the re-pack of :125-161), and compacts the row in
-// ascending topic order (:183-238).  A row starts on a 128-byte boundary and
-// is zero-padded to a multiple of 32 entries.
+// builds a 16-bit LDS histogram, compacts the row in ascending topic order
+// (:183-238), and writes each slot's (n_dk1, n_dk2) at its word's top-2
+// topics (the re-pack of :125-161) with the row's length beside them.  A row
+// starts on a 128-byte boundary and is zero-padded to a multiple of 32
+// entries.
 static constexpr int kDtU = 8;  // slots per lane per chunk in k_dt_build
 
 template <bool COMPACT>
@@ -408,7 +410,7 @@ __global__ void k_dt_build(int q0, int q1, int K, int VD, uint32_t tb, const uin
                            const uint32_t *__restrict__ slot_wrd, const uint32_t *__restrict__ slot_tok,
                            const uint16_t *__restrict__ z, const uint16_t *__restrict__ z_doc,
                            const ushort4 *__restrict__ wtop,
-                           uint32_t *__restrict__ aux_doc, uint2 *__restrict__ doc_info, uint32_t *__restrict__ dt) {
+                           uint2 *__restrict__ aux_doc, uint2 *__restrict__ doc_info, uint32_t *__restrict__ dt) {
     extern __shared__ uint32_t sh[];
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6, nwv = blockDim.x >> 6;
     const int KH = (K + 1) >> 1, NW = (K + 31) >> 5;
@@ -453,44 +455,10 @@ __global__ void k_dt_build(int q0, int q1, int K, int VD, uint32_t tb, const uin
             }
         }
         wave_lds_sync();
-        if (one) {
-#pragma unroll
-            for (int u = 0; u < kDtU; ++u) {
-                const uint32_t i = b + lane + 64u * u;
-                if (i < e) {
-                    const uint32_t k2 = tp[u] >> 16;
-                    aux_doc[i] = h16_get(h, tp[u] & 0xFFFFu) | ((k2 != kNone ? h16_get(h, k2) : 0u) << 16);
-                }
-            }
-        } else {
-            for (uint32_t c = b; c < e; c += 64u * kDtU) {
-                uint32_t x[kDtU];
-#pragma unroll
-                for (int u = 0; u < kDtU; ++u) {
-                    const uint32_t i = c + lane + 64u * u;
-                    x[u] = i < e ? slot_wrd[i] : 0u;
-                }
-#pragma unroll
-                for (int u = 0; u < kDtU; ++u) {
-                    const ushort4 y = wtop[x[u]];
-                    x[u] = y.x | ((uint32_t)y.y << 16);
-                }
-#pragma unroll
-                for (int u = 0; u < kDtU; ++u) {
-                    const uint32_t i = c + lane + 64u * u;
-                    if (i < e) {
-                        const uint32_t k2 = x[u] >> 16;
-                        aux_doc[i] = h16_get(h, x[u] & 0xFFFFu) | ((k2 != kNone ? h16_get(h, k2) : 0u) << 16);
-                    }
-                }
-            }
-        }
-        wave_lds_sync();
         // Compaction over the bitmap: lane j owns topics [32j, 32j + 32) of
         // each 64-word chunk, counts its entries, takes its place from a
-        // wave prefix sum and writes them in ascending topic order; then it
-        // clears its counts and bitmap word.  Cost follows the doc's topics,
-        // not K.
+        // wave prefix sum and writes them in ascending topic order.  Cost
+        // follows the doc's topics, not K.
         const uint32_t base = doc_info[q].x;
         uint32_t nz = 0;
         for (int j = 0; j < NW; j += 64) {
@@ -522,14 +490,8 @@ __global__ void k_dt_build(int q0, int q1, int K, int VD, uint32_t tb, const uin
                     dt[base + pos++] = (k << 2) | (c << 16);
                 }
             }
-            for (uint32_t x = word; x; x &= x - 1u) {  // the lane's own 16 count words
-                const uint32_t k = ((uint32_t)wi << 5) + (uint32_t)__ffs(x) - 1u;
-                h[k >> 1] = 0;
-            }
-            if (wi < NW) bm[wi] = 0;
             nz += __shfl(incl, 63);
         }
-        wave_lds_sync();
         // zero entries up to the next 128 bytes: the sampler reads rows in
         // 128-byte blocks and a zero entry adds nothing
         if (e > b) {
@@ -540,7 +502,54 @@ __global__ void k_dt_build(int q0, int q1, int K, int VD, uint32_t tb, const uin
                 if ((uint32_t)lane < ((32u - (nz & 31u)) & 31u)) dt[base + nz + lane] = 0u;
             }
         }
-        if (lane == 0) doc_info[q].y = nz | ((e - b) << 16);
+        const uint32_t info = nz | ((e - b) << 16);
+        if (lane == 0) doc_info[q].y = info;
+        // each slot's (n_dk1, n_dk2) with the row's length beside them (the
+        // sampler reads both with one load)
+        if (one) {
+#pragma unroll
+            for (int u = 0; u < kDtU; ++u) {
+                const uint32_t i = b + lane + 64u * u;
+                if (i < e) {
+                    const uint32_t k2 = tp[u] >> 16;
+                    aux_doc[i] = make_uint2(h16_get(h, tp[u] & 0xFFFFu) | ((k2 != kNone ? h16_get(h, k2) : 0u) << 16),
+                                            info);
+                }
+            }
+        } else {
+            for (uint32_t c = b; c < e; c += 64u * kDtU) {
+                uint32_t x[kDtU];
+#pragma unroll
+                for (int u = 0; u < kDtU; ++u) {
+                    const uint32_t i = c + lane + 64u * u;
+                    x[u] = i < e ? slot_wrd[i] : 0u;
+                }
+#pragma unroll
+                for (int u = 0; u < kDtU; ++u) {
+                    const ushort4 y = wtop[x[u]];
+                    x[u] = y.x | ((uint32_t)y.y << 16);
+                }
+#pragma unroll
+                for (int u = 0; u < kDtU; ++u) {
+                    const uint32_t i = c + lane + 64u * u;
+                    if (i < e) {
+                        const uint32_t k2 = x[u] >> 16;
+                        aux_doc[i] = make_uint2(
+                            h16_get(h, x[u] & 0xFFFFu) | ((k2 != kNone ? h16_get(h, k2) : 0u) << 16), info);
+                    }
+                }
+            }
+        }
+        wave_lds_sync();
+        // each lane clears its own count words and bitmap words
+        for (int wi = lane; wi < NW; wi += 64) {
+            for (uint32_t x = bm[wi]; x; x &= x - 1u) {
+                const uint32_t k = ((uint32_t)wi << 5) + (uint32_t)__ffs(x) - 1u;
+                h[k >> 1] = 0;
+            }
+            bm[wi] = 0;
+        }
+        wave_lds_sync();
     }
 }
 
@@ -557,7 +566,7 @@ __global__ void __launch_bounds__(256) k_dt_sort(int q0, int q1, int VD, uint32_
                                                  const uint32_t *__restrict__ soff, const uint32_t *__restrict__ slot_wrd,
                                                  const uint32_t *__restrict__ slot_tok, const uint16_t *__restrict__ z,
                                                  const uint16_t *__restrict__ z_doc, const ushort4 *__restrict__ wtop,
-                                                 uint32_t *__restrict__ aux_doc, uint2 *__restrict__ doc_info,
+                                                 uint2 *__restrict__ aux_doc, uint2 *__restrict__ doc_info,
                                                  uint32_t *__restrict__ dt) {
     extern __shared__ uint32_t sh[];
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6, nwv = blockDim.x >> 6;
@@ -681,7 +690,7 @@ __global__ void __launch_bounds__(256) k_dt_sort(int q0, int q1, int VD, uint32_
                     }
                     if (lo < n && key[lo] == want[h]) cnt[h] = run[lo];
                 }
-                aux_doc[b + i] = cnt[0] | (cnt[1] << 16);
+                aux_doc[b + i] = make_uint2(cnt[0] | (cnt[1] << 16), nz | (n << 16));
             }
         }
         wave_lds_sync();  // the next doc reuses the keys
@@ -837,7 +846,7 @@ __device__ __forceinline__ void search_u4(const uint4 *__restrict__ p, float &ru
 // tail words, which differ only in how phi_w is staged.
 struct SampleArgs {
     const uint4 *segs;
-    const uint4 *runs;  // (first token, first doc slot, doc, tokens)
+    const uint4 *runs;  // (first token, first doc slot, the doc's DT row offset, tokens)
     uint32_t n_items;
     int VD, K;
     float alpha, beta;
@@ -845,8 +854,7 @@ struct SampleArgs {
     const uint32_t *tail_off, *wt_tail, *tail_nnz;
     const float *inv;
     const ushort4 *wtop;
-    const uint32_t *aux_doc;  // by doc slot: n_dk1 | n_dk2 << 16
-    const uint2 *doc_info;
+    const uint2 *aux_doc;  // by doc slot: n_dk1 | n_dk2 << 16, the doc's nnz | len << 16
     const uint32_t *dt;
     uint16_t *z, *z_doc;      // by token (word-major) and by doc slot
     const uint32_t *counters;
@@ -958,8 +966,12 @@ __device__ __forceinline__ void sample_item(const SampleArgs &A, uint32_t item, 
         if (r >= sg.z) break;
         const uint4 rn = A.runs[r];
         const uint32_t t0 = rn.x, p0 = rn.y, len = rn.w;
-        const uint32_t a = A.aux_doc[p0];
-        const uint2 di = A.doc_info[rn.z];
+        // the max-topic counts and the row's length in one load (a tail
+        // word's docs are random ones: a separate per-doc record would cost
+        // a cache line of its own); the row's offset comes with the run
+        const uint2 ax = A.aux_doc[p0];
+        const uint32_t a = ax.x;
+        const uint2 di = make_uint2(rn.z, ax.y);
         const uint32_t n1 = a & 0xFFFFu, n2 = a >> 16, nd = di.y >> 16;
         const float P1 = ((float)n1 + alpha) * f1;
         // tokens of the run that need the full draw; shortcut tokens take k1.
@@ -1208,7 +1220,7 @@ __global__ void __launch_bounds__(1024) k_ll(const uint4 *__restrict__ segs, con
                                              const uint32_t *__restrict__ tail_off,
                                              const uint32_t *__restrict__ wt_tail,
                                              const uint32_t *__restrict__ tail_nnz, const int32_t *__restrict__ nk,
-                                             const uint2 *__restrict__ doc_info, const uint32_t *__restrict__ dt,
+                                             const uint2 *__restrict__ aux_doc, const uint32_t *__restrict__ dt,
                                              int compact, uint32_t tb, double *__restrict__ part) {
     extern __shared__ double smd[];
     __shared__ double red[16];
@@ -1252,7 +1264,7 @@ __global__ void __launch_bounds__(1024) k_ll(const uint4 *__restrict__ segs, con
     const uint32_t tmask = (1u << tb) - 1u;
     for (uint32_t r = sg.y + (uint32_t)tid; r < sg.z; r += (uint32_t)blockDim.x) {
         const uint4 rn = runs[r];
-        const uint2 di = doc_info[rn.z];
+        const uint2 di = make_uint2(rn.z, aux_doc[rn.y].y);  // row offset, nnz | len << 16
         const uint32_t nnz = di.y & 0xFFFFu;
         double s = 0.0;
         if (compact) {
@@ -2177,7 +2189,6 @@ hipError_t launch_sample(const DevState &s, bool shortcut, hipStream_t st, uint3
     A.inv = s.inv;
     A.wtop = s.wtop;
     A.aux_doc = s.aux_doc;
-    A.doc_info = s.doc_info;
     A.dt = s.dt;
     A.z = s.z;
     A.z_doc = s.z_doc;
@@ -2226,7 +2237,7 @@ hipError_t launch_loglik(const DevState &s, hipStream_t st) {
     hipLaunchKernelGGL(k_ll, dim3(s.n_smp_segs), dim3(bd), (size_t)s.K * sizeof(double), st, s.smp_segs, s.runs, s.VD,
                        s.K,
                        (double)s.alpha, (double)s.beta, (double)s.V * (double)s.beta, s.wt_head, s.tail_off,
-                       s.wt_tail, s.tail_nnz, s.nk, s.doc_info, s.dt, (int)s.compact, s.topic_bits, s.ll_part);
+                       s.wt_tail, s.tail_nnz, s.nk, s.aux_doc, s.dt, (int)s.compact, s.topic_bits, s.ll_part);
     hipLaunchKernelGGL(k_ll_final, dim3(1), dim3(256), 0, st, s.ll_part, s.n_smp_segs, s.ll_out);
     return hipGetLastError();
 }

@@ -218,7 +218,8 @@ void free_device(ezlda_ctx *c) {
     c->tok_slot_g.clear();
     for (int j = 0; j < 2; ++j) {
         c->cimg[j] = nullptr;
-        c->cdt[j] = c->caux[j] = c->cctr[j] = nullptr;
+        c->cdt[j] = c->cctr[j] = nullptr;
+        c->caux[j] = nullptr;
         c->cll[j] = c->cllo[j] = nullptr;
     }
     c->wt_next = c->nk_next = nullptr;
@@ -428,7 +429,8 @@ int build_layout(ezlda_ctx *c, const std::vector<uint32_t> &tok_doc, const std::
     (void)D;
     std::vector<uint4> smp_segs =
         build_sample_items(c->a, c->word_off, tok_doc, c->soff, slot_wrd, VD, K, c->n_xcd, &c->doc_tiles);
-    const std::vector<uint4> runs = build_runs(smp_segs, tok_doc, tok_slot, (uint32_t)c->a.run_cap);
+    std::vector<uint4> runs = build_runs(smp_segs, tok_doc, tok_slot, (uint32_t)c->a.run_cap);
+    for (uint4 &rn : runs) rn.z = c->row_off[rn.z];  // doc q -> its DT row offset
     int r = 0;
     if ((r = ldalloc(c, &s.wt_head, (size_t)VD * K))) return r;
     if ((r = lupload(c, &s.tail_off, tail_off))) return r;
@@ -539,7 +541,8 @@ int load_chunked(ezlda_ctx *c, const std::vector<uint32_t> &tok_doc, const std::
         ch.dt_entries = dtc;
         int tiles = 0;
         std::vector<uint4> items = build_sample_items(c->a, woff, tdoc, soffc, swrd, VDi, K, c->n_xcd, &tiles);
-        const std::vector<uint4> runs = build_runs(items, tdoc, tslot, (uint32_t)c->a.run_cap);
+        std::vector<uint4> runs = build_runs(items, tdoc, tslot, (uint32_t)c->a.run_cap);
+        for (uint4 &rn : runs) rn.z = di[rn.z].x;  // doc q -> its DT row offset in the chunk
         for (uint4 &it : items)
             if (it.x != 0xFFFFFFFFu) it.w = glob[woff[it.x]] - woff[it.x];
         ch.n_queues = tiles > 0 ? (uint32_t)std::min(c->n_xcd, 8) : 1u;
@@ -1011,6 +1014,8 @@ int ezlda_load_docs(ezlda_ctx *c, int32_t V, int32_t D, const int64_t *doc_off, 
         dtc += compact ? (dt_cap(n, K) + 63u) & ~63u : (dt_cap(n, K) + 31u) & ~31u;
     }
     if (dtc >= 0xFFFFFFF0ull) return fail(c, EZLDA_EINVAL, "DT capacity exceeds 2^32 entries");
+    c->row_off.resize(D);
+    for (int q = 0; q < D; ++q) c->row_off[q] = doc_info[q].x;
     ez::DevState &s = c->s;
     s.compact = compact;
     s.topic_bits = compact ? topic_bits(K) : 16u;
@@ -1774,9 +1779,10 @@ int ezlda_get_top(ezlda_ctx *c, uint16_t *top) try {
 int ezlda_get_aux(ezlda_ctx *c, uint32_t *aux) try {
     if (int r = need_loaded(c)) return r;
     if (is_chunked(c)) return refuse_chunked(c, "ezlda_get_aux");
-    std::vector<uint32_t> a(c->s.N), at(c->s.N);
-    EZ_TRY_HIP(c, hipMemcpy(a.data(), c->s.aux_doc, a.size() * 4, hipMemcpyDeviceToHost));
-    for (size_t p = 0; p < a.size(); ++p) at[c->slot_tok[p]] = a[p];
+    std::vector<uint2> a(c->s.N);
+    std::vector<uint32_t> at(c->s.N);
+    EZ_TRY_HIP(c, hipMemcpy(a.data(), c->s.aux_doc, a.size() * sizeof(uint2), hipMemcpyDeviceToHost));
+    for (size_t p = 0; p < a.size(); ++p) at[c->slot_tok[p]] = a[p].x;
     for (size_t i = 0; i < a.size(); ++i) aux[i] = at[c->doc_tok[i]];
     return EZLDA_OK;
 } catch (...) {
