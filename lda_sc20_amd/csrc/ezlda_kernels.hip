@@ -157,26 +157,31 @@ __device__ __forceinline__ void hist_add_tokens(uint32_t *sh, const uint16_t *__
 // n_k = column sums of WT (WTDen_Sum_Update_Kernel utility.cu:1104-1126 plus
 // the tail's atomicAdd in sparseMatrixAdd :1132-1204): one LDS histogram of
 // z per workgroup, flushed with one atomic per nonzero topic.
-__global__ void k_topic_hist(const uint16_t *__restrict__ z, uint32_t N, int K, int32_t *__restrict__ nk) {
-    extern __shared__ uint32_t sh[];
+// (block `blk` of `nblk`; the kernel below, or a role of k_wt_all)
+__device__ __forceinline__ void topic_hist_block(uint32_t *sh, const uint16_t *__restrict__ z, uint32_t N, int K,
+                                                 int32_t *__restrict__ nk, uint32_t blk, uint32_t nblk) {
     for (int k = threadIdx.x; k < K; k += blockDim.x) sh[k] = 0;
     __syncthreads();
-    hist_add_tokens(sh, z, N, (size_t)blockIdx.x * blockDim.x + threadIdx.x, (size_t)gridDim.x * blockDim.x);
+    hist_add_tokens(sh, z, N, (size_t)blk * blockDim.x + threadIdx.x, (size_t)nblk * blockDim.x);
     __syncthreads();
     for (int k = threadIdx.x; k < K; k += blockDim.x) {
         uint32_t c = sh[k];
         if (c) atomicAdd(&nk[k], (int32_t)c);
     }
 }
+__global__ void k_topic_hist(const uint16_t *__restrict__ z, uint32_t N, int K, int32_t *__restrict__ nk) {
+    extern __shared__ uint32_t sh[];
+    topic_hist_block(sh, z, N, K, nk, blockIdx.x, gridDim.x);
+}
 
 // ------------------------------------------------------------- head WT
 // Dense head rows (WTDen_Update_Kernel, utility.cu:1067-1102): one workgroup
 // per segment of a head word's tokens, LDS histogram, atomic flush into the
 // (pre-zeroed) row.
-__global__ void k_wt_head(const uint16_t *__restrict__ z, const uint4 *__restrict__ segs, int K,
-                          int32_t *__restrict__ wt_head) {
-    extern __shared__ uint32_t sh[];
-    const uint4 sg = segs[blockIdx.x];
+__device__ __forceinline__ void wt_head_block(uint32_t *sh, const uint16_t *__restrict__ z,
+                                              const uint4 *__restrict__ segs, int K, int32_t *__restrict__ wt_head,
+                                              uint32_t seg) {
+    const uint4 sg = segs[seg];
     for (int k = threadIdx.x; k < K; k += blockDim.x) sh[k] = 0;
     __syncthreads();
     hist_add_tokens(sh, z + sg.y, sg.z - sg.y, threadIdx.x, blockDim.x);
@@ -187,6 +192,11 @@ __global__ void k_wt_head(const uint16_t *__restrict__ z, const uint4 *__restric
         if (c) atomicAdd(&row[k], (int32_t)c);
     }
 }
+__global__ void k_wt_head(const uint16_t *__restrict__ z, const uint4 *__restrict__ segs, int K,
+                          int32_t *__restrict__ wt_head) {
+    extern __shared__ uint32_t sh[];
+    wt_head_block(sh, z, segs, K, wt_head, blockIdx.x);
+}
 
 // ------------------------------------------------------------- tail WT
 // Sorted sparse tail rows (WT_Update_Kernel, utility.cu:10-76): one
@@ -194,10 +204,10 @@ __global__ void k_wt_head(const uint16_t *__restrict__ z, const uint4 *__restric
 // word's topics, then a compaction over the bitmap in ascending topic order
 // (lane j owns topics [32j, 32j + 32) of each 64-word chunk and takes its
 // place from a wave prefix sum), clearing its counts and bitmap as it goes.
-__global__ void k_wt_tail(const uint16_t *__restrict__ z, const uint32_t *__restrict__ word_off, int VD, int V,
-                          int K, const uint32_t *__restrict__ tail_off, uint32_t *__restrict__ wt_tail,
-                          uint32_t *__restrict__ tail_nnz) {
-    extern __shared__ uint32_t sh[];
+__device__ __forceinline__ void wt_tail_block(uint32_t *sh, const uint16_t *__restrict__ z,
+                                              const uint32_t *__restrict__ word_off, int VD, int V, int K,
+                                              const uint32_t *__restrict__ tail_off, uint32_t *__restrict__ wt_tail,
+                                              uint32_t *__restrict__ tail_nnz, int blk, int nblk) {
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6, nwv = blockDim.x >> 6;
     const int KH = (K + 1) >> 1, NW = (K + 31) >> 5;
     uint32_t *h = sh + wv * (KH + NW);  // 16-bit counts
@@ -205,7 +215,7 @@ __global__ void k_wt_tail(const uint16_t *__restrict__ z, const uint32_t *__rest
     for (int k = lane; k < KH + NW; k += 64) h[k] = 0;
     wave_lds_sync();
     const int ntail = V - VD;
-    for (int tw = blockIdx.x * nwv + wv; tw < ntail; tw += gridDim.x * nwv) {
+    for (int tw = blk * nwv + wv; tw < ntail; tw += nblk * nwv) {
         const int w = VD + tw;
         const uint32_t b = word_off[w], e = word_off[w + 1];
         if (b == e) {
@@ -244,6 +254,33 @@ __global__ void k_wt_tail(const uint16_t *__restrict__ z, const uint32_t *__rest
         wave_lds_sync();
         if (lane == 0) tail_nnz[tw] = nz;
     }
+}
+__global__ void k_wt_tail(const uint16_t *__restrict__ z, const uint32_t *__restrict__ word_off, int VD, int V,
+                          int K, const uint32_t *__restrict__ tail_off, uint32_t *__restrict__ wt_tail,
+                          uint32_t *__restrict__ tail_nnz) {
+    extern __shared__ uint32_t sh[];
+    wt_tail_block(sh, z, word_off, VD, V, K, tail_off, wt_tail, tail_nnz, blockIdx.x, gridDim.x);
+}
+
+// The three count builds of a sweep in one launch (they touch disjoint
+// outputs): blocks [0, nb_hist) build n_k, the next nb_head the head rows,
+// the rest the tail rows, so each role's last blocks overlap the next role's
+// instead of draining the device between launches.
+__global__ void __launch_bounds__(256) k_wt_all(int nb_hist, int nb_head, const uint16_t *__restrict__ z,
+                                                uint32_t N, int K, int32_t *__restrict__ nk,
+                                                const uint4 *__restrict__ segs, int32_t *__restrict__ wt_head,
+                                                const uint32_t *__restrict__ word_off, int VD, int V,
+                                                const uint32_t *__restrict__ tail_off, uint32_t *__restrict__ wt_tail,
+                                                uint32_t *__restrict__ tail_nnz) {
+    extern __shared__ uint32_t sh[];
+    const int b = (int)blockIdx.x;
+    if (b < nb_hist)
+        topic_hist_block(sh, z, N, K, nk, (uint32_t)b, (uint32_t)nb_hist);
+    else if (b < nb_hist + nb_head)
+        wt_head_block(sh, z, segs, K, wt_head, (uint32_t)(b - nb_hist));
+    else
+        wt_tail_block(sh, z, word_off, VD, V, K, tail_off, wt_tail, tail_nnz, b - nb_hist - nb_head,
+                      (int)gridDim.x - nb_hist - nb_head);
 }
 
 // ----------------------------------------------------- inv and top-3
@@ -2052,10 +2089,21 @@ hipError_t launch_build_wt(const DevState &s, hipStream_t st) {
 
 hipError_t launch_accumulate_wt(const DevState &s, hipStream_t st) {
     const size_t ldsK = (size_t)s.K * sizeof(uint32_t);
+    const int ntail = s.V - s.VD;
+    if (s.N && s.n_wt_segs && ntail > 0 && (size_t)s.lds_cu / dt_wave_bytes(s.K) >= 16) {
+        // one launch of 256-thread blocks (the tail rows' four wavefronts a
+        // block, as their own launch takes when 16 wavefronts' tables fit a CU)
+        const int nb_hist = (int)grid_for(s.N, 256 * 16, 2048);
+        const int nb_tail = (int)grid_for((size_t)ntail, 4, 1 << 16);
+        const size_t lds = std::max(ldsK, 4 * dt_wave_bytes(s.K));
+        hipLaunchKernelGGL(k_wt_all, dim3(nb_hist + s.n_wt_segs + nb_tail), dim3(256), lds, st, nb_hist, s.n_wt_segs,
+                           s.z, s.N, s.K, s.nk, s.wt_segs, s.wt_head, s.word_off, s.VD, s.V, s.tail_off, s.wt_tail,
+                           s.tail_nnz);
+        return hipGetLastError();
+    }
     if (s.N) hipLaunchKernelGGL(k_topic_hist, dim3(grid_for(s.N, 256 * 16, 2048)), dim3(256), ldsK, st, s.z, s.N, s.K, s.nk);
     if (s.n_wt_segs)
         hipLaunchKernelGGL(k_wt_head, dim3(s.n_wt_segs), dim3(256), ldsK, st, s.z, s.wt_segs, s.K, s.wt_head);
-    const int ntail = s.V - s.VD;
     if (ntail > 0) {
         const WaveGrid g = wave_grid(s, (size_t)ntail, dt_wave_bytes(s.K), 1 << 16);
         hipLaunchKernelGGL(k_wt_tail, dim3(g.grid), dim3(64 * g.nwv), g.nwv * dt_wave_bytes(s.K), st,
