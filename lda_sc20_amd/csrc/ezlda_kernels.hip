@@ -298,6 +298,12 @@ __device__ __forceinline__ bool better(float va, uint32_t ka, float vb, uint32_t
     return va > vb || (va == vb && ka < kb);
 }
 
+// A key whose unsigned order is better()'s: value bits (v >= 0, so their
+// order is the value's) above the complemented topic; empty (v < 0) is 0.
+__device__ __forceinline__ uint64_t top_key(float v, uint32_t k) {
+    return v < 0.0f ? 0ull : ((uint64_t)(__float_as_uint(v) + 1u) << 32) | (uint64_t)(0xFFFFFFFFu - k);
+}
+
 __device__ __forceinline__ void top3_insert(Top3 &t, float v, uint32_t k) {
     if (!better(v, k, t.v2, t.k2)) return;
     if (better(v, k, t.v1, t.k1)) {
@@ -402,17 +408,28 @@ __global__ void k_word_top(int w0, int w1, int VD, int K, float beta, const int3
             for (uint32_t i = lane; i < n; i += 64) h[(wt_tail[b + i] & 0xFFFFu) >> 1] = 0;
             wave_lds_sync();
         }
+        // the wavefront's best three: three max-reductions of the lanes'
+        // list heads, the winner's lane popping its head each time (a lane's
+        // topics are its own, so the winner is one lane)
+        uint64_t a0 = top_key(t.v0, t.k0), a1 = top_key(t.v1, t.k1), a2 = top_key(t.v2, t.k2);
+        uint32_t best[3];
 #pragma unroll
-        for (int off = 32; off >= 1; off >>= 1) {
-            Top3 o;
-            o.v0 = __shfl_xor(t.v0, off); o.k0 = __shfl_xor(t.k0, off);
-            o.v1 = __shfl_xor(t.v1, off); o.k1 = __shfl_xor(t.k1, off);
-            o.v2 = __shfl_xor(t.v2, off); o.k2 = __shfl_xor(t.k2, off);
-            top3_insert(t, o.v0, o.k0);
-            top3_insert(t, o.v1, o.k1);
-            top3_insert(t, o.v2, o.k2);
+        for (int r = 0; r < 3; ++r) {
+            uint64_t m = a0;
+#pragma unroll
+            for (int off = 32; off >= 1; off >>= 1) {
+                const uint64_t o = __shfl_xor(m, off);
+                m = o > m ? o : m;
+            }
+            best[r] = m ? 0xFFFFFFFFu - (uint32_t)m : kNone;
+            if (a0 == m) {
+                a0 = a1;
+                a1 = a2;
+                a2 = 0;
+            }
         }
-        if (lane == 0) wtop[w] = make_ushort4((unsigned short)t.k0, (unsigned short)t.k1, (unsigned short)t.k2, 0);
+        if (lane == 0)
+            wtop[w] = make_ushort4((unsigned short)best[0], (unsigned short)best[1], (unsigned short)best[2], 0);
     }
 }
 
