@@ -442,21 +442,21 @@ def main() -> int:
 
     # algorithmic bytes of one sampler launch: the two z writes per token (by
     # token and by doc slot); per run (a word's tokens in one doc) its entry
-    # and its slot's record of max-topic counts and row length (16 + 8); phi
-    # staging once per word; one read of every scanned DT entry (one scan per
-    # run with a full draw; 2-byte entries when K <= 8192, else 4)
+    # and its slot's record of n_dk1 and the row length (16 + 4); phi staging
+    # once per word; one read of every scanned DT entry (one scan per run
+    # with a full draw; 2-byte entries when K <= 8192, else 4)
     VD = int(info.num_head_words)
     R = int(info.num_runs)
     eb = 2 if K <= 8192 else 4
     s_scanned = n_scanned / K_steps
-    b_sample = 4 * N + 24 * R + 4 * K * VD + 4 * tail_nnz + 4 * K + eb * s_scanned
+    b_sample = 4 * N + 20 * R + 4 * K * VD + 4 * tail_nnz + 4 * K + eb * s_scanned
     t_sample = stage_ms["sample"] / K_steps / 1e3 if not chunked else None
     achieved = b_sample / t_sample / 1e9 if not chunked else None
     # whole-sweep algorithmic bytes (same accounting for every stage)
     b_sweep = (b_sample                                                      # sample
                + 2 * N + 2 * N + 4 * K * VD + 4 * tail_nnz                  # n_k + WT rebuild
                + 4 * K * VD + 4 * tail_nnz + 8 * info.num_words             # top-3
-               + 2 * N + 4 * N + 8 * N + eb * dt_nnz + 8 * info.num_docs)   # DT + records (z_doc, word, record)
+               + 2 * N + 4 * N + 4 * N + eb * dt_nnz + 8 * info.num_docs)   # DT + records (z_doc, word, record)
     ll = ctx.loglik()  # first call rebuilds the counts of the final topics
     device_sync()
     t = time.perf_counter()

@@ -32,7 +32,7 @@ struct DevState {
     uint32_t *slot_wrd = nullptr;   // N: word rank
     uint32_t *slot_tok = nullptr;   // N: token id of the slot
     uint16_t *z_doc = nullptr;      // N: z by slot, kept by the sampler for head-word slots
-    uint2 *aux_doc = nullptr;       // N: x = n_dk1 | n_dk2 << 16 of the slot's word, y = its doc's nnz | len << 16
+    uint32_t *aux_doc = nullptr;    // N: n_dk1 of the slot's word | its doc's row length (nnz) << 16
     // doc-topic rows
     uint2 *doc_info = nullptr;      // D, by doc q: x = DT row offset, y = nnz | len << 16
     int32_t D_long = 0;             // docs of more than kDtShort tokens (the first D_long in doc order)
@@ -53,7 +53,8 @@ struct DevState {
     int32_t n_wt_segs = 0;
     uint4 *smp_segs = nullptr;      // sampler items (w, first run, end run, draw-key offset)
     int32_t n_smp_segs = 0;
-    uint4 *runs = nullptr;          // (first token, first slot, doc's DT row offset, tokens): a word's tokens in one doc
+    uint4 *runs = nullptr;          // (first token, first slot, doc's DT row offset, tokens | doc length << 16):
+                                    // a word's tokens in one doc
     int64_t n_runs = 0;
     int32_t n_queues = 1;           // persistent sampler: item queues (<= 8)
     uint32_t *counters = nullptr;   // kCounterWords: [0] sweep, queue q's head at [kQueueStride * (q + 1)]
@@ -71,7 +72,10 @@ hipError_t launch_build_wt(const DevState &s, hipStream_t st);
 // Adds the counts of s.z (n_k, dense head rows, tail rows) without zeroing.
 hipError_t launch_accumulate_wt(const DevState &s, hipStream_t st);
 hipError_t launch_topk(const DevState &s, hipStream_t st);
-hipError_t launch_build_dt(const DevState &s, hipStream_t st);
+// DT rows, doc_info and the per-slot records of the sampler (n_dk1 | nnz << 16
+// in s.aux_doc); with n_dk2 set, (n_dk1 | n_dk2 << 16) instead (the reference's
+// per-token pair, which the sampler does not need).
+hipError_t launch_build_dt(const DevState &s, hipStream_t st, bool n_dk2 = false);
 hipError_t launch_z_to_doc(const DevState &s, hipStream_t st);
 // Sampler stages (mode bits): the items of head words, of tail words, and
 // the max-topic shortcut alone (tokens it accepts get their top topic; the

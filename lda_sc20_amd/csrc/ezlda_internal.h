@@ -57,7 +57,7 @@ struct ezlda_ctx {
     std::vector<uint32_t> tok_slot_g;  // token id -> corpus slot
     uint8_t *cimg[2] = {nullptr, nullptr};  // device image buffers
     uint32_t *cdt[2] = {nullptr, nullptr};  // DT rows of the chunk in flight
-    uint2 *caux[2] = {nullptr, nullptr};
+    uint32_t *caux[2] = {nullptr, nullptr};
     uint32_t *cctr[2] = {nullptr, nullptr};  // per-buffer sweep counter and item queues
     double *cll[2] = {nullptr, nullptr};     // per-item log-likelihood partials
     double *cllo[2] = {nullptr, nullptr};

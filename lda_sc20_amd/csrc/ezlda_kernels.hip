@@ -453,18 +453,18 @@ hipError_t launch_z_to_doc(const DevState &s, hipStream_t st) {
 // tokens grouped by word, and the sampler keeps z_doc in step with z, so
 // every read and write here is a coalesced stream.  One wavefront per doc
 // builds a 16-bit LDS histogram, compacts the row in ascending topic order
-// (:183-238), and writes each slot's (n_dk1, n_dk2) at its word's top-2
-// topics (the re-pack of :125-161) with the row's length beside them.  A row
-// starts on a 128-byte boundary and is zero-padded to a multiple of 32
-// entries.
+// (:183-238), and writes each slot's record: n_dk1 at its word's top topic
+// with the row's length (the sampler's), or with n_dk2 = n_dk at its word's
+// second topic (the re-pack of :125-161, for the getter).  A row starts on a
+// 128-byte boundary and is zero-padded to a multiple of 32 entries.
 static constexpr int kDtU = 8;  // slots per lane per chunk in k_dt_build
 
-template <bool COMPACT>
+template <bool COMPACT, bool N_DK2>
 __global__ void k_dt_build(int q0, int q1, int K, int VD, uint32_t tb, const uint32_t *__restrict__ soff,
                            const uint32_t *__restrict__ slot_wrd, const uint32_t *__restrict__ slot_tok,
                            const uint16_t *__restrict__ z, const uint16_t *__restrict__ z_doc,
                            const ushort4 *__restrict__ wtop,
-                           uint2 *__restrict__ aux_doc, uint2 *__restrict__ doc_info, uint32_t *__restrict__ dt) {
+                           uint32_t *__restrict__ aux_doc, uint2 *__restrict__ doc_info, uint32_t *__restrict__ dt) {
     extern __shared__ uint32_t sh[];
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6, nwv = blockDim.x >> 6;
     const int KH = (K + 1) >> 1, NW = (K + 31) >> 5;
@@ -556,18 +556,18 @@ __global__ void k_dt_build(int q0, int q1, int K, int VD, uint32_t tb, const uin
                 if ((uint32_t)lane < ((32u - (nz & 31u)) & 31u)) dt[base + nz + lane] = 0u;
             }
         }
-        const uint32_t info = nz | ((e - b) << 16);
-        if (lane == 0) doc_info[q].y = info;
-        // each slot's (n_dk1, n_dk2) with the row's length beside them (the
-        // sampler reads both with one load)
+        if (lane == 0) doc_info[q].y = nz | ((e - b) << 16);
+        // each slot's record: n_dk1 and the row's length (a tail word's run
+        // is in a random doc, whose doc_info would cost a cache line of its
+        // own), or n_dk1 and n_dk2
         if (one) {
 #pragma unroll
             for (int u = 0; u < kDtU; ++u) {
                 const uint32_t i = b + lane + 64u * u;
                 if (i < e) {
                     const uint32_t k2 = tp[u] >> 16;
-                    aux_doc[i] = make_uint2(h16_get(h, tp[u] & 0xFFFFu) | ((k2 != kNone ? h16_get(h, k2) : 0u) << 16),
-                                            info);
+                    const uint32_t hi = N_DK2 ? (k2 != kNone ? h16_get(h, k2) : 0u) : nz;
+                    aux_doc[i] = h16_get(h, tp[u] & 0xFFFFu) | (hi << 16);
                 }
             }
         } else {
@@ -588,8 +588,8 @@ __global__ void k_dt_build(int q0, int q1, int K, int VD, uint32_t tb, const uin
                     const uint32_t i = c + lane + 64u * u;
                     if (i < e) {
                         const uint32_t k2 = x[u] >> 16;
-                        aux_doc[i] = make_uint2(
-                            h16_get(h, x[u] & 0xFFFFu) | ((k2 != kNone ? h16_get(h, k2) : 0u) << 16), info);
+                        const uint32_t hi = N_DK2 ? (k2 != kNone ? h16_get(h, k2) : 0u) : nz;
+                        aux_doc[i] = h16_get(h, x[u] & 0xFFFFu) | (hi << 16);
                     }
                 }
             }
@@ -615,13 +615,13 @@ __global__ void k_dt_build(int q0, int q1, int K, int VD, uint32_t tb, const uin
 // histogram's compaction writes them) and finds each slot's (n_dk1, n_dk2)
 // by bisection.  Rows, pieces, padding and aux are those of k_dt_build.
 
-template <bool COMPACT>
+template <bool COMPACT, bool N_DK2>
 __global__ void __launch_bounds__(256) k_dt_sort(int q0, int q1, int VD, uint32_t tb,
                                                  const uint32_t *__restrict__ soff, const uint32_t *__restrict__ slot_wrd,
                                                  const uint32_t *__restrict__ slot_tok, const uint16_t *__restrict__ z,
                                                  const uint16_t *__restrict__ z_doc, const ushort4 *__restrict__ wtop,
-                                                 uint2 *__restrict__ aux_doc, uint2 *__restrict__ doc_info,
-                                                 uint32_t *__restrict__ dt) {
+                                                 uint32_t *__restrict__ aux_doc,
+                                                 uint2 *__restrict__ doc_info, uint32_t *__restrict__ dt) {
     extern __shared__ uint32_t sh[];
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6, nwv = blockDim.x >> 6;
     uint16_t *key = reinterpret_cast<uint16_t *>(sh + wv * kDtShort);  // kDtShort sorted topics
@@ -744,7 +744,7 @@ __global__ void __launch_bounds__(256) k_dt_sort(int q0, int q1, int VD, uint32_
                     }
                     if (lo < n && key[lo] == want[h]) cnt[h] = run[lo];
                 }
-                aux_doc[b + i] = make_uint2(cnt[0] | (cnt[1] << 16), nz | (n << 16));
+                aux_doc[b + i] = cnt[0] | ((N_DK2 ? cnt[1] : nz) << 16);
             }
         }
         wave_lds_sync();  // the next doc reuses the keys
@@ -900,7 +900,7 @@ __device__ __forceinline__ void search_u4(const uint4 *__restrict__ p, float &ru
 // tail words, which differ only in how phi_w is staged.
 struct SampleArgs {
     const uint4 *segs;
-    const uint4 *runs;  // (first token, first doc slot, the doc's DT row offset, tokens)
+    const uint4 *runs;  // (first token, first doc slot, the doc's DT row offset, tokens | doc length << 16)
     uint32_t n_items;
     int VD, K;
     float alpha, beta;
@@ -908,7 +908,7 @@ struct SampleArgs {
     const uint32_t *tail_off, *wt_tail, *tail_nnz;
     const float *inv;
     const ushort4 *wtop;
-    const uint2 *aux_doc;  // by doc slot: n_dk1 | n_dk2 << 16, the doc's nnz | len << 16
+    const uint32_t *aux_doc;  // by doc slot: n_dk1 | the doc's row length (nnz) << 16
     const uint32_t *dt;
     uint16_t *z, *z_doc;      // by token (word-major) and by doc slot
     const uint32_t *counters;
@@ -1007,7 +1007,7 @@ __device__ __forceinline__ void sample_item(const SampleArgs &A, uint32_t item, 
     }
     __syncthreads();
     const float Q = qacc[NS - 1];
-    const float f1 = f123[0], f2 = f123[1], f3 = f123[2];
+    const float f1 = f123[0], f2 = f123[1];
     const uint32_t k1 = tp.x;
     // head words also write the doc-slot copy of z (their items share a doc
     // tile, so the copy's lines fill in L2); tail tokens, in random docs,
@@ -1019,21 +1019,28 @@ __device__ __forceinline__ void sample_item(const SampleArgs &A, uint32_t item, 
         const uint32_t r = base + (uint32_t)lane;
         if (r >= sg.z) break;
         const uint4 rn = A.runs[r];
-        const uint32_t t0 = rn.x, p0 = rn.y, len = rn.w;
-        // the max-topic counts and the row's length in one load (a tail
-        // word's docs are random ones: a separate per-doc record would cost
-        // a cache line of its own); the row's offset comes with the run
-        const uint2 ax = A.aux_doc[p0];
-        const uint32_t a = ax.x;
-        const uint2 di = make_uint2(rn.z, ax.y);
-        const uint32_t n1 = a & 0xFFFFu, n2 = a >> 16, nd = di.y >> 16;
+        const uint32_t t0 = rn.x, p0 = rn.y, nd = rn.w >> 16;
+        uint32_t len = rn.w & 0xFFFFu;
+        // (the empty asm hides len's bound from the optimizer, whose
+        // rewrites of the token loops from it otherwise cost a spill)
+        asm volatile("" : "+v"(len));
+        // n_dk1 and the row's length in one load (a tail word's docs are
+        // random ones: a per-doc record would cost a cache line of its own);
+        // the row's offset and the doc's length come with the run
+        const uint32_t a = A.aux_doc[p0];
+        const uint2 di = make_uint2(rn.z, a >> 16);  // row offset, nnz
+        const uint32_t n1 = a & 0xFFFFu;
         const float P1 = ((float)n1 + alpha) * f1;
         // tokens of the run that need the full draw; shortcut tokens take k1.
         // The uniforms of the first kUKeep tokens are kept for the draw.
         uint32_t need = 0;
         constexpr int UK = COMPACT ? kUKeep : kUKeep32;
         float uk[UK];
-        const float maxS = (float)n2 * f2 + (float)(nd - n1 - n2) * f3;
+        // S <= (n_d - n_dk1) * phi_k2: every other topic's phi is at most the
+        // second largest.  Any bound >= S leaves every draw as the full path
+        // makes it (the margin covers the rounding), so the shortcut only
+        // skips work; this one needs no n_dk2.
+        const float maxS = (float)(nd - n1) * f2;
         const float B = (P1 + maxS) + Q;
 #pragma unroll
         for (uint32_t j = 0; j < (uint32_t)UK; ++j) {
@@ -1274,7 +1281,7 @@ __global__ void __launch_bounds__(1024) k_ll(const uint4 *__restrict__ segs, con
                                              const uint32_t *__restrict__ tail_off,
                                              const uint32_t *__restrict__ wt_tail,
                                              const uint32_t *__restrict__ tail_nnz, const int32_t *__restrict__ nk,
-                                             const uint2 *__restrict__ aux_doc, const uint32_t *__restrict__ dt,
+                                             const uint32_t *__restrict__ aux_doc, const uint32_t *__restrict__ dt,
                                              int compact, uint32_t tb, double *__restrict__ part) {
     extern __shared__ double smd[];
     __shared__ double red[16];
@@ -1318,7 +1325,7 @@ __global__ void __launch_bounds__(1024) k_ll(const uint4 *__restrict__ segs, con
     const uint32_t tmask = (1u << tb) - 1u;
     for (uint32_t r = sg.y + (uint32_t)tid; r < sg.z; r += (uint32_t)blockDim.x) {
         const uint4 rn = runs[r];
-        const uint2 di = make_uint2(rn.z, aux_doc[rn.y].y);  // row offset, nnz | len << 16
+        const uint2 di = make_uint2(rn.z, aux_doc[rn.y] >> 16);  // row offset, nnz
         const uint32_t nnz = di.y & 0xFFFFu;
         double s = 0.0;
         if (compact) {
@@ -1342,7 +1349,7 @@ __global__ void __launch_bounds__(1024) k_ll(const uint4 *__restrict__ segs, con
                 for (int u = 0; u < 4; ++u) s += (double)(e[u] >> 16) * phi[dt_topic(e[u])];
             }
         }
-        acc += (double)rn.w * log((s + q) / ((double)(di.y >> 16) + kalpha));
+        acc += (double)(rn.w & 0xFFFFu) * log((s + q) / ((double)(rn.w >> 16) + kalpha));
     }
     for (int off = 32; off >= 1; off >>= 1) acc += __shfl_xor(acc, off);
     if (lane == 0) red[wv] = acc;
@@ -2171,8 +2178,8 @@ hipError_t launch_topk(const DevState &s, hipStream_t st) {
     return hipGetLastError();
 }
 
-hipError_t launch_build_dt(const DevState &s, hipStream_t st) {
-    if (s.D == 0) return hipSuccess;
+template <bool COMPACT, bool N_DK2>
+static void build_dt_kernels(const DevState &s, hipStream_t st) {
     // where a K-sized histogram per wavefront leaves fewer than 16 on a CU,
     // the short documents (the last ones: docs come longest first) are sorted
     // instead
@@ -2180,24 +2187,26 @@ hipError_t launch_build_dt(const DevState &s, hipStream_t st) {
     const int q1 = sort_short ? s.D_long : s.D;
     if (q1 > 0) {
         const WaveGrid g = wave_grid(s, (size_t)q1, dt_wave_bytes(s.K), 1 << 18);
-        if (s.compact)
-            hipLaunchKernelGGL(k_dt_build<true>, dim3(g.grid), dim3(64 * g.nwv), g.nwv * dt_wave_bytes(s.K), st, 0, q1,
-                               s.K, s.VD, s.topic_bits, s.soff, s.slot_wrd, s.slot_tok, s.z, s.z_doc, s.wtop, s.aux_doc,
-                               s.doc_info, s.dt);
-        else
-            hipLaunchKernelGGL(k_dt_build<false>, dim3(g.grid), dim3(64 * g.nwv), g.nwv * dt_wave_bytes(s.K), st, 0, q1,
-                               s.K, s.VD, s.topic_bits, s.soff, s.slot_wrd, s.slot_tok, s.z, s.z_doc, s.wtop, s.aux_doc,
-                               s.doc_info, s.dt);
+        hipLaunchKernelGGL((k_dt_build<COMPACT, N_DK2>), dim3(g.grid), dim3(64 * g.nwv), g.nwv * dt_wave_bytes(s.K), st,
+                           0, q1, s.K, s.VD, s.topic_bits, s.soff, s.slot_wrd, s.slot_tok, s.z, s.z_doc, s.wtop,
+                           s.aux_doc, s.doc_info, s.dt);
     }
     if (q1 < s.D) {
         const size_t lds = 4 * kDtShort * sizeof(uint32_t);  // four wavefronts, 2 KB each
         const unsigned grid = (unsigned)grid_for((size_t)(s.D - q1), 4, 1 << 18);
-        if (s.compact)
-            hipLaunchKernelGGL(k_dt_sort<true>, dim3(grid), dim3(256), lds, st, q1, s.D, s.VD, s.topic_bits, s.soff,
-                               s.slot_wrd, s.slot_tok, s.z, s.z_doc, s.wtop, s.aux_doc, s.doc_info, s.dt);
-        else
-            hipLaunchKernelGGL(k_dt_sort<false>, dim3(grid), dim3(256), lds, st, q1, s.D, s.VD, s.topic_bits, s.soff,
-                               s.slot_wrd, s.slot_tok, s.z, s.z_doc, s.wtop, s.aux_doc, s.doc_info, s.dt);
+        hipLaunchKernelGGL((k_dt_sort<COMPACT, N_DK2>), dim3(grid), dim3(256), lds, st, q1, s.D, s.VD, s.topic_bits,
+                           s.soff, s.slot_wrd, s.slot_tok, s.z, s.z_doc, s.wtop, s.aux_doc, s.doc_info, s.dt);
+    }
+}
+
+hipError_t launch_build_dt(const DevState &s, hipStream_t st, bool n_dk2) {
+    if (s.D == 0) return hipSuccess;
+    if (s.compact) {
+        if (n_dk2) build_dt_kernels<true, true>(s, st);
+        else build_dt_kernels<true, false>(s, st);
+    } else {
+        if (n_dk2) build_dt_kernels<false, true>(s, st);
+        else build_dt_kernels<false, false>(s, st);
     }
     return hipGetLastError();
 }

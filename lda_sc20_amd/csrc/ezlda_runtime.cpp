@@ -218,8 +218,7 @@ void free_device(ezlda_ctx *c) {
     c->tok_slot_g.clear();
     for (int j = 0; j < 2; ++j) {
         c->cimg[j] = nullptr;
-        c->cdt[j] = c->cctr[j] = nullptr;
-        c->caux[j] = nullptr;
+        c->cdt[j] = c->caux[j] = c->cctr[j] = nullptr;
         c->cll[j] = c->cllo[j] = nullptr;
     }
     c->wt_next = c->nk_next = nullptr;
@@ -430,7 +429,10 @@ int build_layout(ezlda_ctx *c, const std::vector<uint32_t> &tok_doc, const std::
     std::vector<uint4> smp_segs =
         build_sample_items(c->a, c->word_off, tok_doc, c->soff, slot_wrd, VD, K, c->n_xcd, &c->doc_tiles);
     std::vector<uint4> runs = build_runs(smp_segs, tok_doc, tok_slot, (uint32_t)c->a.run_cap);
-    for (uint4 &rn : runs) rn.z = c->row_off[rn.z];  // doc q -> its DT row offset
+    for (uint4 &rn : runs) {  // doc q -> its DT row offset; the doc's length beside the run's
+        rn.w |= (c->soff[rn.z + 1] - c->soff[rn.z]) << 16;
+        rn.z = c->row_off[rn.z];
+    }
     int r = 0;
     if ((r = ldalloc(c, &s.wt_head, (size_t)VD * K))) return r;
     if ((r = lupload(c, &s.tail_off, tail_off))) return r;
@@ -542,7 +544,10 @@ int load_chunked(ezlda_ctx *c, const std::vector<uint32_t> &tok_doc, const std::
         int tiles = 0;
         std::vector<uint4> items = build_sample_items(c->a, woff, tdoc, soffc, swrd, VDi, K, c->n_xcd, &tiles);
         std::vector<uint4> runs = build_runs(items, tdoc, tslot, (uint32_t)c->a.run_cap);
-        for (uint4 &rn : runs) rn.z = di[rn.z].x;  // doc q -> its DT row offset in the chunk
+        for (uint4 &rn : runs) {  // doc q -> its DT row offset in the chunk; the doc's length
+            rn.w |= (soffc[rn.z + 1] - soffc[rn.z]) << 16;
+            rn.z = di[rn.z].x;
+        }
         for (uint4 &it : items)
             if (it.x != 0xFFFFFFFFu) it.w = glob[woff[it.x]] - woff[it.x];
         ch.n_queues = tiles > 0 ? (uint32_t)std::min(c->n_xcd, 8) : 1u;
@@ -1779,10 +1784,34 @@ int ezlda_get_top(ezlda_ctx *c, uint16_t *top) try {
 int ezlda_get_aux(ezlda_ctx *c, uint32_t *aux) try {
     if (int r = need_loaded(c)) return r;
     if (is_chunked(c)) return refuse_chunked(c, "ezlda_get_aux");
-    std::vector<uint2> a(c->s.N);
-    std::vector<uint32_t> at(c->s.N);
-    EZ_TRY_HIP(c, hipMemcpy(a.data(), c->s.aux_doc, a.size() * sizeof(uint2), hipMemcpyDeviceToHost));
-    for (size_t p = 0; p < a.size(); ++p) at[c->slot_tok[p]] = a[p].x;
+    // (n_dk1, n_dk2) of the current topics: the DT rebuild in its other record
+    // form, into scratch rows, doc_info and records, so no state changes
+    const ez::DevState &s = c->s;
+    ez::DevState t = s;
+    const size_t dt_words = s.compact ? ((size_t)c->dt_cap + 256 + 1) / 2 : (size_t)c->dt_cap + 256;
+    t.aux_doc = nullptr;
+    t.dt = nullptr;
+    t.doc_info = nullptr;
+    int rc = EZLDA_OK;
+    auto chk = [&](hipError_t e, const char *what) {
+        if (rc == EZLDA_OK && e != hipSuccess) rc = fail(c, EZLDA_EHIP, std::string(what) + ": " + hipGetErrorString(e));
+    };
+    chk(hipMalloc((void **)&t.aux_doc, std::max<size_t>(s.N, 1) * sizeof(uint32_t)), "hipMalloc");
+    chk(hipMalloc((void **)&t.dt, dt_words * sizeof(uint32_t)), "hipMalloc");
+    chk(hipMalloc((void **)&t.doc_info, std::max<size_t>(s.D, 1) * sizeof(uint2)), "hipMalloc");
+    if (rc == EZLDA_OK && s.D)
+        chk(hipMemcpyAsync(t.doc_info, s.doc_info, (size_t)s.D * sizeof(uint2), hipMemcpyDeviceToDevice, c->st),
+            "hipMemcpyAsync");
+    if (rc == EZLDA_OK) chk(ez::launch_build_dt(t, c->st, true), "DT rebuild");
+    std::vector<uint32_t> a(s.N), at(s.N);
+    if (rc == EZLDA_OK && s.N)
+        chk(hipMemcpyAsync(a.data(), t.aux_doc, a.size() * 4, hipMemcpyDeviceToHost, c->st), "hipMemcpyAsync");
+    if (rc == EZLDA_OK) chk(hipStreamSynchronize(c->st), "hipStreamSynchronize");
+    if (t.aux_doc) (void)hipFree(t.aux_doc);
+    if (t.dt) (void)hipFree(t.dt);
+    if (t.doc_info) (void)hipFree(t.doc_info);
+    if (rc) return rc;
+    for (size_t p = 0; p < a.size(); ++p) at[c->slot_tok[p]] = a[p];
     for (size_t i = 0; i < a.size(); ++i) aux[i] = at[c->doc_tok[i]];
     return EZLDA_OK;
 } catch (...) {
