@@ -261,7 +261,11 @@ int ezlda_export_model(ezlda_ctx *ctx, const char *dir);
 int ezlda_get_dt_raw(ezlda_ctx *ctx, uint32_t *entries, int32_t *nnz);
 /* Top-3 topics per word (3 x uint16 per input word id; 0xFFFF = none). */
 int ezlda_get_top(ezlda_ctx *ctx, uint16_t *top);
-/* Per-token n_dk1 | n_dk2 << 16, doc-major order. */
+/* Per-token n_dk1 | n_dk2 << 16, doc-major order, of the current topics and
+ * counts (DT_Update_Kernel's per-token pair, EZLDA/utility.cu:125-161).  The
+ * sampler keeps n_dk1 with the row length instead, so this runs the DT
+ * rebuild in its n_dk2 form into scratch device buffers (it allocates; no
+ * state changes). */
 int ezlda_get_aux(ezlda_ctx *ctx, uint32_t *aux);
 int ezlda_get_inv(ezlda_ctx *ctx, float *inv);
 
