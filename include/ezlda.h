@@ -150,7 +150,8 @@ int ezlda_build_wt(ezlda_ctx *ctx);
 /* 1/(n_k + V beta) and every word's top-3 topics by phi.
  * Replaces MaxTopicKernel (SamplingKernel.cuh:12). */
 int ezlda_topk(ezlda_ctx *ctx);
-/* Sorted DT rows and each token's (n_dk1, n_dk2).
+/* Sorted DT rows and each token's record for the sampler (n_dk1 and its
+ * doc's row length; ezlda_get_aux gives the reference's (n_dk1, n_dk2)).
  * Replaces UpdateDTKernel (DTUpdateKernel.cuh:13). */
 int ezlda_build_dt(ezlda_ctx *ctx);
 /* Draws a new topic for every token and advances the sweep counter.
