@@ -40,6 +40,7 @@ def lib(omp: bool = False) -> C.CDLL:
         "orc_destroy": (None, [P]),
         "orc_num_tokens": (C.c_int64, [P]),
         "orc_get_sweep": (C.c_uint32, [P]),
+        "orc_full_draws": (C.c_int64, [P]),
         "orc_set_sweep": (None, [P, C.c_uint32]),
         "orc_build_wt": (None, [P]),
         "orc_topk": (None, [P]),
@@ -86,7 +87,9 @@ class Oracle:
     """One oracle state; mirrors the stages of the product context."""
 
     def __init__(self, K: int, alpha: float, beta: float, seed: int, V: int, doc_off: np.ndarray, words: np.ndarray,
-                 z0: Optional[np.ndarray] = None, shortcut: bool = True, omp: bool = False):
+                 z0: Optional[np.ndarray] = None, shortcut: int = 1, omp: bool = False):
+        """shortcut: 0 off, 1 (or True) the reference's bound, 2 the GPU path's looser
+        bound (n_d - n_dk1) phi_k2 -- every mode gives the same topics."""
         self.L = lib(omp)
         self.K, self.V = K, V
         self.doc_off = np.ascontiguousarray(doc_off, dtype=np.int64)
@@ -122,6 +125,10 @@ class Oracle:
 
     def loglik(self) -> float:
         return float(self.L.orc_loglik(self.h))
+
+    def full_draws(self) -> int:
+        """Tokens that took the full draw (not the shortcut), over all sweeps."""
+        return int(self.L.orc_full_draws(self.h))
 
     @property
     def sweep_index(self) -> int:

@@ -47,7 +47,10 @@
  *      (n_d - n_dk1 - n_dk2) phi_k3 >= S.  The full path's target carries at
  *      most (nnz + 4) roundings of 2^-24 (nnz <= K <= 16383), below the
  *      2^-9 margin, so the shortcut only ever accepts tokens the full path
- *      also sends to k1: shortcut on and off give identical z.
+ *      also sends to k1: shortcut on and off give identical z.  The same
+ *      holds for any bound >= S; shortcut mode 2 uses the GPU path's
+ *      maxS = (n_d - n_dk1) phi_k2 (every topic but k1 has phi <= phi_k2),
+ *      which needs no n_dk2: same z, more full draws.
  *   7. log-likelihood in FP64: sum_t log(sum_k theta_dk phi_wk), the per-token
  *      term of profilePerp/utility.cu:2398,2810 (log((S+Q)/(n_d+K alpha))),
  *      summed over all tokens (profilePerp/utility.cu:2858-2928).
@@ -122,7 +125,8 @@ typedef struct {
     int64_t N;
     float alpha, beta, vbeta;
     uint64_t seed;
-    int shortcut;
+    int shortcut;       /* 0 off, 1 the reference's bound, 2 the GPU path's looser bound */
+    int64_t n_full;     /* tokens that took the full draw, all sweeps */
     uint32_t sweep;
     int64_t *word_off;  /* V+1, word-major token ranges (rank order) */
     int32_t *tok_doc;   /* N */
@@ -246,6 +250,7 @@ void orc_destroy(void *h) {
 
 int64_t orc_num_tokens(void *h) { return ((orc_t *)h)->N; }
 uint32_t orc_get_sweep(void *h) { return ((orc_t *)h)->sweep; }
+int64_t orc_full_draws(void *h) { return ((orc_t *)h)->n_full; }
 void orc_set_sweep(void *h, uint32_t s) { ((orc_t *)h)->sweep = s; }
 
 /* --------------------------------------------- 1. WT and n_k from z */
@@ -410,6 +415,7 @@ void orc_sample(void *h) {
         float *qs = (float *)xcalloc(K, sizeof(float));
         float *qt = (float *)xcalloc(nsq, sizeof(float));
         float *qacc = (float *)xcalloc(nsq, sizeof(float));
+        int64_t n_full = 0;
 #pragma omp for schedule(dynamic, 16)
         for (int w = 0; w < o->V; ++w) {
             if (o->word_off[w + 1] == o->word_off[w]) continue;
@@ -440,17 +446,21 @@ void orc_sample(void *h) {
                 const float u = u24(philox_x0(o->seed, (uint64_t)t, o->sweep, 0u));
                 const float P1 = ((float)n1 + alpha) * f1;
                 if (o->shortcut) {
-                    const float maxS = (float)n2 * f2 + (float)(nd - n1 - n2) * f3;
+                    const float maxS = o->shortcut == 2 ? (float)(nd - n1) * f2
+                                                        : (float)n2 * f2 + (float)(nd - n1 - n2) * f3;
                     const float B = (P1 + maxS) + Q;
                     if (u * B < P1 * ORC_MARGIN) {
                         o->z[t] = (uint16_t)k1;
                         continue;
                     }
                 }
+                ++n_full;
                 o->z[t] = (uint16_t)draw_full(o, phi, qs, qt, qacc, nsq, k1, P1, Q, u, d);
             }
         }
         free(phi); free(qs); free(qt); free(qacc);
+#pragma omp atomic
+        o->n_full += n_full;
     }
     o->sweep++;
     o->counts_fresh = 0;

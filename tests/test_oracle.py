@@ -288,6 +288,26 @@ def test_shortcut_is_exact():
             assert np.array_equal(a.get_topics(), b.get_topics()), K
 
 
+def test_shortcut_bounds_give_the_same_topics():
+    """The GPU path's looser bound (n_d - n_dk1) phi_k2 (mode 2) and the
+    reference's bound (mode 1) both accept only tokens the full draw sends to
+    k1: the topics equal those without a shortcut, sweep by sweep; the looser
+    bound takes at least as many full draws, and both skip some."""
+    off, words = make_corpus(8, 150, 120, 80, zipf=1.3)
+    N = len(words)
+    for K in (8, 64, 257):
+        o = [_oracle(off, words, K, shortcut=m, V=120) for m in (0, 1, 2)]
+        for s in range(8):
+            for x in o:
+                x.sweep(1)
+            z = o[0].get_topics()
+            assert np.array_equal(o[1].get_topics(), z), (K, s)
+            assert np.array_equal(o[2].get_topics(), z), (K, s)
+        f0, f1, f2 = (x.full_draws() for x in o)
+        assert f0 == 8 * N, K
+        assert f1 <= f2 < f0, (K, f1, f2, f0)
+
+
 def test_openmp_build_is_bit_identical():
     off, words = make_corpus(7, 300, 200, 60)
     a = _oracle(off, words, 48, omp=False, V=200)
